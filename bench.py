@@ -1,0 +1,196 @@
+"""Benchmark of the suzerain banded implicit-solve hot path on one MI355X.
+
+Workload: the batched FP64 implicit solve -- S=5 equations x n=96 k=8
+B-spline coefficients (N = 480, KL = KU = 34), 1152 (kx, kz) systems, one
+right hand side each.  One step = the whole per-substep implicit pass over
+the batch: operator assembly into LU-ready band storage, right-hand-side
+permutation P b, banded LU with partial pivoting, forward/back substitution,
+and the inverse permutation P^T x.  Inputs (operator bands, wavenumbers,
+coefficients, right hand sides) are resident in HBM before timing.
+
+Prints one JSON line (rank 0).  `roofline` reports the dominant kernel (the
+batched LU factorization) timed with HIP events on its own stream;
+`cpu_baseline` times the oracle's C restatement of the reference CPU path on
+a bounded sample of the same systems across host threads.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--cpu-sample", type=int, default=1152,
+                   help="systems in the CPU baseline sample")
+    p.add_argument("--no-cpu", action="store_true")
+    return p.parse_args()
+
+
+def cpu_baseline(inp, op, nsample):
+    """Oracle (C restatement of pack + permute + ZGBTRF + ZGBTRS + permute)
+    over `nsample` systems on host threads; returns (solves/s, threads)."""
+    import concurrent.futures
+    import oracle
+    import oracle.model as om
+    from oracle.bspline import band_of
+    S, n = inp["S"], inp["n"]
+    kl = max(op.ku[:3])
+    ku = max(op.kl[:3])
+    KL, KU = S * (kl + 1) - 1, S * (ku + 1) - 1
+    N = S * n
+    ldab = 2 * KL + KU + 1
+    M, D1, D2 = op.dense(0), op.dense(1), op.dense(2)
+    idx = np.linspace(0, len(inp["kx"]) - 1, nsample).astype(int)
+    # submatrix bands prepared outside the timed region
+    bands = {}
+    for s in idx:
+        bands[s] = [[band_of(om.submatrix(ih, jh, M, D1, D2, inp["kx"][s],
+                                          inp["kz"][s], inp["cij"], inp["uij"],
+                                          inp["phidt"], inp["nu"]),
+                             kl, ku).reshape(-1) for jh in range(S)]
+                    for ih in range(S)]
+
+    def one(s):
+        ab = np.zeros(N * ldab, dtype=np.complex128)
+        view = ab[KL:]
+        for ih in range(S):
+            for jh in range(S):
+                oracle.zpack(S, n, ih, jh, kl, ku, 1.0, bands[s][ih][jh],
+                             kl + 1 + ku, KL, KU, view, ldab)
+        pb = np.zeros(N, dtype=np.complex128)
+        oracle.zaPxpby("N", S, n, 1, inp["rhs"][s], 0, pb)
+        info, ipiv = oracle.zgbtrf(N, KL, KU, ab, ldab)
+        oracle.zgbtrs("N", N, KL, KU, ab, ldab, ipiv, pb)
+        x = np.zeros(N, dtype=np.complex128)
+        oracle.zaPxpby("T", S, n, 1, pb, 0, x)
+        return info
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    one(idx[0])
+    t0 = time.perf_counter()
+    with concurrent.futures.ThreadPoolExecutor(threads) as ex:
+        infos = list(ex.map(one, idx))
+    dt = time.perf_counter() - t0
+    assert all(i == 0 for i in infos)
+    return nsample / dt, threads, dt
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the "
+              "implicit solves shard by wavenumber with no communication, so "
+              "whole-job throughput is the one-GPU value times the GPU count.",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import suzerain_amd as sa
+    from suzerain_amd.workload import implicit_solve_inputs
+
+    dev = torch.device("cuda:0")
+    inp = implicit_solve_inputs()
+    op = sa.BsplineOp.channel(inp["n"], inp["k"], inp["htdelta"])
+    S, n = inp["S"], inp["n"]
+    kl = max(op.ku[:3])
+    ku = max(op.kl[:3])
+    KL, KU = S * (kl + 1) - 1, S * (ku + 1) - 1
+    N, nsys = S * n, len(inp["kx"])
+    ldab = 2 * KL + KU + 1
+
+    kx = torch.from_numpy(inp["kx"]).to(dev)
+    kz = torch.from_numpy(inp["kz"]).to(dev)
+    cij = torch.from_numpy(inp["cij"]).to(dev)
+    uij = torch.from_numpy(inp["uij"]).to(dev)
+    rhs = torch.from_numpy(inp["rhs"]).to(dev)
+    lu = torch.zeros((nsys, N, ldab), dtype=torch.complex128, device=dev)
+    pb = torch.zeros((nsys, 1, N), dtype=torch.complex128, device=dev)
+    x = torch.zeros((nsys, N), dtype=torch.complex128, device=dev)
+    ipiv = torch.zeros((nsys, N), dtype=torch.int32, device=dev)
+    info = torch.zeros((nsys,), dtype=torch.int32, device=dev)
+    op.device_ptr()
+
+    stream = torch.cuda.current_stream()
+    ev = []
+
+    def step(record):
+        sa.model_assemble(op, S, kx, kz, cij, uij, inp["phidt"], inp["nu"], lu,
+                          KL, KU)
+        sa.aPxpby("N", S, n, 1, rhs, 0, pb.view(nsys, N))
+        if record:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+        sa.gbtrf(lu, KL, KU, ipiv, info)
+        if record:
+            e1.record(stream)
+            ev.append((e0, e1))
+        sa.gbtrs("N", lu, KL, KU, ipiv, pb)
+        sa.aPxpby("T", S, n, 1, pb.view(nsys, N), 0, x)
+
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    assert int(info.abs().sum()) == 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = elapsed * 1e3 / args.steps
+    solves_per_s = nsys * args.steps / elapsed
+
+    trf_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    # algorithmic HBM bytes of one factorization launch: read the band of A
+    # (KL+KU+1 rows), write the factors (2KL+KU+1 rows) and the pivots
+    bytes_sys = N * (KL + KU + 1) * 16 + N * ldab * 16 + N * 4
+    achieved = bytes_sys * nsys / (trf_ms * 1e-3) / 1e9
+    flops_sys = 8.0 * N * KL * (KL + KU)  # complex rank-1 updates, leading term
+    out = {
+        "metric": "implicit wavenumber solves/s",
+        "value": round(solves_per_s, 1),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "complex128 (f64)",
+        "data": "synthetic (seeded operator and right hand sides)",
+        "config": {"workload": "batched FP64 implicit solve: S=5 x n=96 "
+                               "(k=8, 90 stretched breakpoints), N=480, "
+                               "KL=KU=34, 1152 (kx,kz) systems, 1 RHS",
+                   "systems": nsys, "N": N, "KL": KL, "KU": KU,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "k_gbtrf<double>",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "avg_launch_ms": round(trf_ms, 4),
+                     "fp64_tflops": round(flops_sys * nsys / (trf_ms * 1e-3) / 1e12, 3)},
+    }
+    if not args.no_cpu:
+        rate, threads, dt = cpu_baseline(inp, op, args.cpu_sample)
+        out["cpu_baseline"] = {
+            "value": round(rate, 1), "unit": "solves/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{args.cpu_sample} of the {nsys} systems (evenly "
+                      f"spaced wavenumbers), pack+permute+zgbtrf+zgbtrs+"
+                      f"permute in the oracle's C restatement, {dt:.1f} s"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,170 @@
+/*
+ * suzerain_amd: MI355X (gfx950) implementation of the suzerain spectral-DNS
+ * substep hot path behind a plain C ABI.
+ *
+ * Conventions shared by every entry point:
+ *   - Array arguments named x, y, b, ab, lu, papt, ... are DEVICE pointers;
+ *     complex values are interleaved (re, im) doubles, bit-compatible with C99
+ *     `double complex`, std::complex<double> and numpy complex128.  Scalar
+ *     complex arguments (alpha, beta) are HOST pointers to two doubles.
+ *   - `stream` is a hipStream_t (NULL = the default stream).  Every entry
+ *     point only enqueues work; nothing synchronizes unless documented.
+ *   - Batched entry points process `nbatch` independent problems whose data
+ *     sit at a fixed element stride (`stride*`) from one another.
+ *   - Return value: 0 on success; -i when argument i is invalid (LAPACK
+ *     style, numbered as in the reference routine each entry replaces);
+ *     SZ_ERR_* below for launch, LDS and allocation failures.  Per-system
+ *     numerical outcomes (zero pivots, refinement counts) are written to
+ *     device arrays, never returned, and never abort.
+ */
+#ifndef SUZERAIN_AMD_H
+#define SUZERAIN_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SZ_ERR_LAUNCH (-1001)
+#define SZ_ERR_LDS    (-1002)
+#define SZ_ERR_ALLOC  (-1003)
+
+/* ------------------------------------------------------------------ bsmbsm
+ * Replaces suzerain_bsmbsm_construct / _q / _qinv (suzerain/bsmbsm.h:104-186).
+ */
+typedef struct sz_bsmbsm {
+    int S, n, kl, ku, ld, N, KL, KU, LD;
+} sz_bsmbsm;
+
+sz_bsmbsm sz_bsmbsm_construct(int S, int n, int kl, int ku);
+int sz_bsmbsm_q(int S, int n, int i);
+int sz_bsmbsm_qinv(int S, int n, int i);
+
+/* y_b <- alpha P x_b + beta y_b ('N') or alpha P^T x_b + beta y_b ('T') for
+ * every b < nbatch.  Replaces suzerain_bsmbsm_zaPxpby (suzerain/bsmbsm.h:
+ * 247-257, bsmbsm_aPxpby_complex.def:36-329). */
+int sz_bsmbsm_zaPxpby_batched(char trans, int S, int n, const double *alpha,
+                              const void *x, int incx, long long stridex,
+                              const double *beta, void *y, int incy,
+                              long long stridey, int nbatch, void *stream);
+
+/* Scatter alpha*B^{ihat,jhat} into band storage of P A P^T for every system.
+ * Replaces suzerain_bsmbsm_zpack / zdpack (suzerain/bsmbsm.h:360-381,
+ * bsmbsm_pack.def:36-122); zpackc/zpackf are the calls with ldpapt = LD and
+ * papt offset by KL with ldpapt = LD + KL respectively. */
+int sz_bsmbsm_zpack_batched(int S, int n, int ihat, int jhat, int kl, int ku,
+                            const double *alpha, const void *b, int ldb,
+                            long long strideb, int KL, int KU, void *papt,
+                            int ldpapt, long long stridepapt, int nbatch,
+                            void *stream);
+int sz_bsmbsm_zdpack_batched(int S, int n, int ihat, int jhat, int kl, int ku,
+                             const double *alpha, const double *b, int ldb,
+                             long long strideb, int KL, int KU, void *papt,
+                             int ldpapt, long long stridepapt, int nbatch,
+                             void *stream);
+
+/* Model implicit operator of the batched-solve workload assembled straight
+ * into LU-ready band storage (rows KL..2KL+KU of leading dimension ldlu):
+ *   B_ij = d_ij M - phidt (nu D2 - nu (kx^2 + kz^2) M + c_ij D1 + i kx u_ij M)
+ * with M, D1, D2 the transposed collocation bands of sz_bsplineop
+ * (d_t = sz_bsplineop_D_T_device, dstride = n*ld, kuaddr = max_ku, dkl/dku
+ * host arrays of the first nsub derivative bandwidths), kx/kz device arrays
+ * of nsys wavenumbers, cij/uij device arrays of S*S coefficients (row-major
+ * in (i, j)).  Fuses what suzerain_rholut_imexop_packf does with 25
+ * suzerain_bsmbsm_zpackf calls (suzerain/rholut_imexop.def:41-612) for this
+ * operator. */
+int sz_model_assemble_batched(int S, int n, const double *d_t, int ld,
+                              int kuaddr, int nsub, const int *dkl,
+                              const int *dku, long long dstride,
+                              const double *kx, const double *kz,
+                              const double *cij, const double *uij,
+                              double phidt, double nu, int KL, int KU,
+                              void *lu, int ldlu, long long stridelu, int nsys,
+                              void *stream);
+
+/* ------------------------------------------------------- banded LU (LAPACK)
+ * In-place banded LU with partial pivoting for every system; info[b] = 0 or
+ * the 1-based column of the first exactly zero pivot.  Replaces
+ * suzerain_lapack_zgbtrf / cgbtrf (suzerain/blas_et_al/lapack.c:185-198,
+ * lapack.h) as called by bsmbsm_solver_zgbsv::solve_hook
+ * (suzerain/bsmbsm_solver.cpp:155-182). */
+int sz_zgbtrf_batched(int n, int kl, int ku, void *ab, int ldab,
+                      long long strideab, int *ipiv, long long strideipiv,
+                      int *info, int nbatch, void *stream);
+int sz_cgbtrf_batched(int n, int kl, int ku, void *ab, int ldab,
+                      long long strideab, int *ipiv, long long strideipiv,
+                      int *info, int nbatch, void *stream);
+
+/* op(A) X = B with the factors of *gbtrf_batched; trans 'N', 'T' or 'C'.
+ * Replaces suzerain_lapack_zgbtrs / cgbtrs (lapack.c:261-278). */
+int sz_zgbtrs_batched(char trans, int n, int kl, int ku, int nrhs,
+                      const void *ab, int ldab, long long strideab,
+                      const int *ipiv, long long strideipiv, void *b, int ldb,
+                      long long strideb, int nbatch, void *stream);
+int sz_cgbtrs_batched(char trans, int n, int kl, int ku, int nrhs,
+                      const void *ab, int ldab, long long strideab,
+                      const int *ipiv, long long strideipiv, void *b, int ldb,
+                      long long strideb, int nbatch, void *stream);
+
+/* Factor and solve; systems with info > 0 keep b unchanged.  Replaces
+ * suzerain_lapackext_zgbsv (suzerain/blas_et_al/lapackext.c:123-147). */
+int sz_zgbsv_batched(char trans, int n, int kl, int ku, int nrhs, void *ab,
+                     int ldab, long long strideab, int *ipiv,
+                     long long strideipiv, void *b, int ldb, long long strideb,
+                     int *info, int nbatch, void *stream);
+
+/* --------------------------------------------------- B-spline operators
+ * Greville collocation derivative operators, built on the host, applied on
+ * the device.  Replaces suzerain_bsplineop_alloc/free (suzerain/bsplineop.c:
+ * 101-220) with SUZERAIN_BSPLINEOP_COLLOCATION_GREVILLE and
+ * suzerain_bsplineop_accumulate[_complex] (bsplineop.c:222-297). */
+typedef struct sz_bsplineop sz_bsplineop;
+
+double sz_htstretch1(double delta, double L, double x);
+double sz_htstretch2(double delta, double L, double x);
+/* Breakpoints of create_bsplines (suzerain/support/support.cpp:245-311):
+ * writes ndof-k+2 values, returns that count (or a negative error). */
+int sz_bspline_breakpoints(int ndof, int k, double left, double right,
+                           double htdelta, double *out);
+sz_bsplineop *sz_bsplineop_alloc(int k, int nbreak, const double *breakpts,
+                                 int nderiv, int *info);
+void sz_bsplineop_free(sz_bsplineop *w);
+int sz_bsplineop_n(const sz_bsplineop *w);
+int sz_bsplineop_k(const sz_bsplineop *w);
+int sz_bsplineop_nderiv(const sz_bsplineop *w);
+int sz_bsplineop_ld(const sz_bsplineop *w);
+int sz_bsplineop_max_kl(const sz_bsplineop *w);
+int sz_bsplineop_max_ku(const sz_bsplineop *w);
+int sz_bsplineop_kl(const sz_bsplineop *w, int d);
+int sz_bsplineop_ku(const sz_bsplineop *w, int d);
+double sz_bsplineop_collocation_point(const sz_bsplineop *w, int i);
+const double *sz_bsplineop_D_T(const sz_bsplineop *w, int d);
+const double *sz_bsplineop_D_T_device(sz_bsplineop *w);
+int sz_bsplineop_accumulate_complex_batched(sz_bsplineop *w, int nderiv,
+                                            int nrhs, const double *alpha,
+                                            const void *x, int incx,
+                                            long long ldx, const double *beta,
+                                            void *y, int incy, long long ldy,
+                                            void *stream);
+int sz_bsplineop_accumulate_batched(sz_bsplineop *w, int nderiv, int nrhs,
+                                    double alpha, const double *x, int incx,
+                                    long long ldx, double beta, double *y,
+                                    int incy, long long ldy, void *stream);
+
+/* ---------------------------------------------------------------- diffwave
+ * Replaces suzerain_diffwave_apply / accumulate (suzerain/diffwave.h:90-151,
+ * diffwave.c:66-198) on the (Y, X, Z) column-major wave layout. */
+int sz_diffwave_apply(int dxcnt, int dzcnt, const double *alpha, void *x,
+                      double Lx, double Lz, int Ny, int Nx, int dNx, int dkbx,
+                      int dkex, int Nz, int dNz, int dkbz, int dkez,
+                      void *stream);
+int sz_diffwave_accumulate(int dxcnt, int dzcnt, const double *alpha,
+                           const void *x, const double *beta, void *y,
+                           double Lx, double Lz, int Ny, int Nx, int dNx,
+                           int dkbx, int dkex, int Nz, int dNz, int dkbz,
+                           int dkez, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SUZERAIN_AMD_H */

@@ -1,0 +1,110 @@
+"""ctypes binding of libsuzerain_amd.so (the C ABI in include/suzerain_amd.h).
+
+The library is the product; importing this module fails loudly when it is
+missing instead of falling back to any CPU path.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libsuzerain_amd.so")
+
+vp, i32, i64, f64 = C.c_void_p, C.c_int, C.c_longlong, C.c_double
+pd = C.POINTER(C.c_double)
+pi = C.POINTER(C.c_int)
+ch = C.c_char
+
+
+class sz_bsmbsm(C.Structure):
+    """Mirror of the sz_bsmbsm / suzerain_bsmbsm geometry struct."""
+    _fields_ = [(k, C.c_int) for k in
+                ("S", "n", "kl", "ku", "ld", "N", "KL", "KU", "LD")]
+
+    def asdict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# name: (restype, [argtypes])
+SIGNATURES = {
+    "sz_bsmbsm_construct": (sz_bsmbsm, [i32, i32, i32, i32]),
+    "sz_bsmbsm_q": (i32, [i32, i32, i32]),
+    "sz_bsmbsm_qinv": (i32, [i32, i32, i32]),
+    "sz_bsmbsm_zaPxpby_batched": (i32, [ch, i32, i32, pd, vp, i32, i64, pd, vp,
+                                        i32, i64, i32, vp]),
+    "sz_bsmbsm_zpack_batched": (i32, [i32, i32, i32, i32, i32, i32, pd, vp, i32,
+                                      i64, i32, i32, vp, i32, i64, i32, vp]),
+    "sz_bsmbsm_zdpack_batched": (i32, [i32, i32, i32, i32, i32, i32, pd, vp,
+                                       i32, i64, i32, i32, vp, i32, i64, i32,
+                                       vp]),
+    "sz_model_assemble_batched": (i32, [i32, i32, vp, i32, i32, i32, pi, pi,
+                                        i64, vp, vp, vp, vp, f64, f64, i32, i32,
+                                        vp, i32, i64, i32, vp]),
+    "sz_zgbtrf_batched": (i32, [i32, i32, i32, vp, i32, i64, vp, i64, vp, i32,
+                                vp]),
+    "sz_cgbtrf_batched": (i32, [i32, i32, i32, vp, i32, i64, vp, i64, vp, i32,
+                                vp]),
+    "sz_zgbtrs_batched": (i32, [ch, i32, i32, i32, i32, vp, i32, i64, vp, i64,
+                                vp, i32, i64, i32, vp]),
+    "sz_cgbtrs_batched": (i32, [ch, i32, i32, i32, i32, vp, i32, i64, vp, i64,
+                                vp, i32, i64, i32, vp]),
+    "sz_zgbsv_batched": (i32, [ch, i32, i32, i32, i32, vp, i32, i64, vp, i64,
+                               vp, i32, i64, vp, i32, vp]),
+    "sz_htstretch1": (f64, [f64, f64, f64]),
+    "sz_htstretch2": (f64, [f64, f64, f64]),
+    "sz_bspline_breakpoints": (i32, [i32, i32, f64, f64, f64, pd]),
+    "sz_bsplineop_alloc": (vp, [i32, i32, pd, i32, pi]),
+    "sz_bsplineop_free": (None, [vp]),
+    "sz_bsplineop_n": (i32, [vp]),
+    "sz_bsplineop_k": (i32, [vp]),
+    "sz_bsplineop_nderiv": (i32, [vp]),
+    "sz_bsplineop_ld": (i32, [vp]),
+    "sz_bsplineop_max_kl": (i32, [vp]),
+    "sz_bsplineop_max_ku": (i32, [vp]),
+    "sz_bsplineop_kl": (i32, [vp, i32]),
+    "sz_bsplineop_ku": (i32, [vp, i32]),
+    "sz_bsplineop_collocation_point": (f64, [vp, i32]),
+    "sz_bsplineop_D_T": (pd, [vp, i32]),
+    "sz_bsplineop_D_T_device": (vp, [vp]),
+    "sz_bsplineop_accumulate_complex_batched": (i32, [vp, i32, i32, pd, vp, i32,
+                                                      i64, pd, vp, i32, i64,
+                                                      vp]),
+    "sz_bsplineop_accumulate_batched": (i32, [vp, i32, i32, f64, vp, i32, i64,
+                                              f64, vp, i32, i64, vp]),
+    "sz_diffwave_apply": (i32, [i32, i32, pd, vp, f64, f64, i32, i32, i32, i32,
+                                i32, i32, i32, i32, i32, vp]),
+    "sz_diffwave_accumulate": (i32, [i32, i32, pd, vp, pd, vp, f64, f64, i32,
+                                     i32, i32, i32, i32, i32, i32, i32, i32,
+                                     vp]),
+}
+
+_lib = None
+
+
+def load():
+    """Return the loaded library, declaring every signature once."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise ImportError(
+            f"{LIBPATH} is missing: build it with `python -m suzerain_amd.build`"
+            " (there is no CPU fallback)")
+    lib = C.CDLL(LIBPATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def cplx(z):
+    """Host complex scalar as the (re, im) double pair the ABI expects."""
+    z = complex(z)
+    return (C.c_double * 2)(z.real, z.imag)
+
+
+def check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed with code {rc}")
+    return rc
