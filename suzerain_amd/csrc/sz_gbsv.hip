@@ -27,7 +27,6 @@ using sz::cpx;
 
 constexpr int TRF_NT = 256;
 constexpr int TRS_NT = 64;
-constexpr int TRS_CH = 16;   // columns staged per chunk in the solve
 constexpr size_t LDS_MAX = 160 * 1024;
 
 template <typename R>
@@ -38,6 +37,30 @@ __device__ inline void argmax_cabs1(R &best, int &bidx)
         const int oi = __shfl_xor(bidx, off);
         if (ob > best || (ob == best && oi < bidx)) { best = ob; bidx = oi; }
     }
+}
+
+// Barrier ordering LDS only: global loads and stores in flight stay in
+// flight (the factorization never communicates through global memory).
+#define LDS_BARRIER()                                                         \
+    do {                                                                      \
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");       \
+        __builtin_amdgcn_s_barrier();                                         \
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");       \
+    } while (0)
+
+constexpr int TRF_PF = 8;   // incoming rows prefetched ahead of their step
+
+template <typename R>
+__device__ __forceinline__ cpx<R> ld_or_zero(const cpx<R> *p, bool ok)
+{
+    cpx<R> v;
+    v.x = R(0);
+    v.y = R(0);
+    if (ok) {
+        v.x = p->x;
+        v.y = p->y;
+    }
+    return v;
 }
 
 template <typename R>
@@ -58,123 +81,187 @@ void k_gbtrf(int n, int kl, int ku, cpx<R> *__restrict__ ab, int ldab,
     T *W = reinterpret_cast<T *>(smem);   // W[rowslot*WC + colslot]
     T *Urow = W + WR * WC;                // pivot row after the swap
     T *Lcol = Urow + WC;                  // multipliers of column j
-    __shared__ int s_p, s_ju, s_nz, s_info;
+    __shared__ int s_ju, s_nz, s_info;
 
     for (int e = tid; e < WR * WC; e += TRF_NT) {
         const int r = e / WC, c = e - r * WC;
-        T v = {0, 0};
-        if (r < n && c < n && r - c <= kl && c - r <= ku)
-            v = ab[(size_t)c * ldab + kv + r - c];
-        W[e] = v;
+        const bool ok = r < n && c < n && r - c <= kl && c - r <= ku;
+        W[e] = ld_or_zero(ab + (ok ? (size_t)c * ldab + kv + r - c : 0), ok);
     }
     if (tid == 0) { s_ju = 0; s_info = 0; }
-    __syncthreads();
+    // Every thread issues the same global loads and stores at every step
+    // (duplicates for tid >= WC hit the same addresses with the same values),
+    // so the compiler can count outstanding memory operations exactly and
+    // the prefetched rows below are waited for only when they are consumed.
+    // Thread column cl brings in element cl of the row entering at each step:
+    // row i = j + WR, column k = j + 1 + cl, i.e. band row kv + kl - cl.
+    const int cl = tid % WC;
+    const int brow = kv + kl - cl;
+    const int rl = 1 + tid % (kl > 0 ? kl : 1);
+    T q[TRF_PF];
+    // q[u] holds the raw element (read from a clamped, always valid
+    // address); rows or columns beyond n are zeroed when it is consumed.
+#pragma unroll
+    for (int u = 0; u < TRF_PF; ++u) {
+        const int i = u + WR, k = u + 1 + cl;
+        const bool ok = i < n && k < n;
+        q[u] = ab[ok ? (size_t)k * ldab + brow : 0];
+    }
+    __shared__ int s_p;
+    LDS_BARRIER();
 
     int rs0 = 0, cs0 = 0;   // slots holding row j and column j
-    for (int j = 0; j < n; ++j) {
-        const int km = min(kl, n - 1 - j);
-        if (tid < 64) {
-            // ---- pivot search (IZAMAX semantics: NaN never wins unless
-            // it sits in the first position, where it wins outright)
-            R best = R(-1);
-            int bidx = 0x7fffffff;
-            for (int r = tid; r <= km; r += 64) {
-                int rs = rs0 + r; if (rs >= WR) rs -= WR;
-                R a = sz::cabs1(W[rs * WC + cs0]);
-                if (a != a) a = R(-1);
-                if (a > best || (a == best && r < bidx)) { best = a; bidx = r; }
-            }
-            argmax_cabs1(best, bidx);
-            const T a0 = W[rs0 * WC + cs0];
-            int p = bidx;
-            if (sz::cabs1(a0) != sz::cabs1(a0) || bidx == 0x7fffffff) p = 0;
-            int prs = rs0 + p; if (prs >= WR) prs -= WR;
-            const T piv = W[prs * WC + cs0];
-            const bool nz = sz::cnonzero(piv);
-            int ju = s_ju;
-            if (nz) ju = max(ju, min(j + ku + p, n - 1));
-            const bool swap = nz && p != 0;
-            // ---- row j after the swap -> Urow; old row j -> row slot p
-            for (int c = tid; c <= kv; c += 64) {
-                int cs = cs0 + c; if (cs >= WC) cs -= WC;
-                const T a = W[rs0 * WC + cs];
-                T u = a;
-                if (swap && c <= ju - j) {
-                    u = W[prs * WC + cs];
-                    W[prs * WC + cs] = a;
-                }
-                Urow[c] = u;
-            }
-            // ---- multipliers (row p now holds old row j's entry a0)
-            const T rcp = nz ? sz::crcp(piv) : T{0, 0};
-            for (int r = 1 + tid; r <= km; r += 64) {
-                int rs = rs0 + r; if (rs >= WR) rs -= WR;
-                const T v = (swap && r == p) ? a0 : W[rs * WC + cs0];
-                Lcol[r] = nz ? sz::cmul(v, rcp) : v;
-            }
-            if (tid == 0) {
-                ipiv[j] = j + p + 1;
-                s_p = p;
-                s_ju = ju;
-                s_nz = nz ? 1 : 0;
-                if (!nz && s_info == 0) s_info = j + 1;
-            }
-        }
-        __syncthreads();
-
-        const int ju = s_ju;
-        const bool nz = s_nz != 0;
-        // ---- rank-1 update of rows j+1..j+km, columns j+1..ju
-        if (nz) {
-            const int w = ju - j;
-            const int lane = tid & 63;
-            for (int c = 1 + lane; c <= w; c += 64) {
-                int cs = cs0 + c; if (cs >= WC) cs -= WC;
-                const T u = Urow[c];
-                for (int r = 1 + (tid >> 6); r <= km; r += TRF_NT / 64) {
+    for (int j0 = 0; j0 < n; j0 += TRF_PF) {
+#pragma unroll
+        for (int u = 0; u < TRF_PF; ++u) {
+            const int j = j0 + u;
+            if (j >= n) break;
+            const int km = min(kl, n - 1 - j);
+            if (tid < 64) {
+                // ---- pivot search (IZAMAX: first maximum of |Re|+|Im|; a
+                // NaN never beats a number, a NaN in the first slot wins)
+                R best = R(-1);
+                int bidx = 0x7fffffff;
+                for (int r = tid; r <= km; r += 64) {
                     int rs = rs0 + r; if (rs >= WR) rs -= WR;
-                    T *pa = W + rs * WC + cs;
-                    *pa = sz::cfnms(*pa, Lcol[r], u);
+                    R a = sz::cabs1(W[rs * WC + cs0]);
+                    if (a != a) a = R(-1);
+                    if (a > best || (a == best && r < bidx)) { best = a; bidx = r; }
+                }
+                argmax_cabs1(best, bidx);
+                const T a0 = W[rs0 * WC + cs0];
+                const R a0abs = sz::cabs1(a0);
+                const int p = (a0abs != a0abs) ? 0 : bidx;
+                int prs = rs0 + p; if (prs >= WR) prs -= WR;
+                const T piv = W[prs * WC + cs0];
+                const bool nz = sz::cnonzero(piv);
+                int ju = s_ju;
+                if (nz) ju = max(ju, min(j + ku + p, n - 1));
+                const bool swap = nz && p != 0;
+                // ---- row j after the swap -> Urow; old row j -> slot p
+                for (int c = tid; c <= kv; c += 64) {
+                    int cs = cs0 + c; if (cs >= WC) cs -= WC;
+                    const T a = W[rs0 * WC + cs];
+                    const T b = W[prs * WC + cs];
+                    const bool sw = swap && c <= ju - j;
+                    T uu;
+                    uu.x = sw ? b.x : a.x;
+                    uu.y = sw ? b.y : a.y;
+                    if (sw) W[prs * WC + cs] = a;
+                    Urow[c] = uu;
+                }
+                // ---- multipliers (row p now holds old row j's entry a0)
+                T rcp;
+                rcp.x = R(0);
+                rcp.y = R(0);
+                if (nz) rcp = sz::crcp(piv);
+                for (int r = 1 + tid; r <= km; r += 64) {
+                    int rs = rs0 + r; if (rs >= WR) rs -= WR;
+                    const T w = W[rs * WC + cs0];
+                    const bool use_a0 = swap && r == p;
+                    T v;
+                    v.x = use_a0 ? a0.x : w.x;
+                    v.y = use_a0 ? a0.y : w.y;
+                    const T l = sz::cmul(v, rcp);
+                    T out;
+                    out.x = nz ? l.x : v.x;
+                    out.y = nz ? l.y : v.y;
+                    Lcol[r] = out;
+                }
+                if (tid == 0) {
+                    s_p = p;
+                    s_ju = ju;
+                    s_nz = nz ? 1 : 0;
+                    if (!nz && s_info == 0) s_info = j + 1;
                 }
             }
-        }
-        // ---- U row j and multipliers of column j to HBM
-        {
-            const int cmax = min(kv, n - 1 - j);
-            for (int c = tid; c <= cmax; c += TRF_NT)
-                ab[(size_t)(j + c) * ldab + kv - c] = Urow[c];
-            for (int r = 1 + tid; r <= km; r += TRF_NT)
-                ab[(size_t)j * ldab + kv + r] = Lcol[r];
-        }
-        // ---- column slot cs0 becomes column j+kv+1 (fill-in zeros) and row
-        // slot rs0 becomes row j+kl+1 (read from HBM)
-        for (int r = tid; r < WR; r += TRF_NT)
-            if (r != rs0) W[r * WC + cs0] = T{0, 0};
-        {
-            const int i = j + WR;
-            for (int c = tid; c <= kv; c += TRF_NT) {
-                const int k = j + 1 + c;
-                int cs = cs0 + 1 + c; if (cs >= WC) cs -= WC;
-                T v = {0, 0};
-                if (i < n && k < n) v = ab[(size_t)k * ldab + kv + i - k];
-                W[rs0 * WC + cs] = v;
+            LDS_BARRIER();
+
+            const int ju = s_ju;
+            const bool nz = s_nz != 0;
+            // ---- rank-1 update of rows j+1..j+km, columns j+1..ju
+            if (nz) {
+                const int w = ju - j;
+                const int lane = tid & 63;
+                for (int c = 1 + lane; c <= w; c += 64) {
+                    int cs = cs0 + c; if (cs >= WC) cs -= WC;
+                    const T uu = Urow[c];
+                    for (int r = 1 + (tid >> 6); r <= km; r += TRF_NT / 64) {
+                        int rs = rs0 + r; if (rs >= WR) rs -= WR;
+                        T *pa = W + rs * WC + cs;
+                        *pa = sz::cfnms(*pa, Lcol[r], uu);
+                    }
+                }
             }
+            // ---- U row j, multipliers of column j and the pivot index to
+            // HBM: one unconditional store each per thread (redirected onto
+            // U(j,j) when this thread has nothing of its own to write)
+            {
+                const int cmax = min(kv, n - 1 - j);
+                const int ce = cl <= cmax ? cl : 0;
+                ab[(size_t)(j + ce) * ldab + kv - ce] = Urow[ce];
+                const int re = (kl > 0 && rl <= km) ? rl : 0;
+                ab[(size_t)j * ldab + kv + re] = re ? Lcol[re] : Urow[0];
+                ipiv[j] = j + s_p + 1;
+            }
+            // ---- column slot cs0 becomes column j+kv+1 (fill-in zeros) and
+            // row slot rs0 becomes row j+kl+1 (prefetched PF steps ago)
+            for (int r = tid; r < WR; r += TRF_NT)
+                if (r != rs0) W[r * WC + cs0] = T{0, 0};
+            if (tid < WC) {
+                const bool in = (j + WR < n) && (j + 1 + cl < n);
+                int cs = cs0 + 1 + cl; if (cs >= WC) cs -= WC;
+                T qv;
+                qv.x = in ? q[u].x : R(0);
+                qv.y = in ? q[u].y : R(0);
+                W[rs0 * WC + cs] = qv;
+            }
+            {
+                const int i = j + TRF_PF + WR, k = j + TRF_PF + 1 + cl;
+                const bool ok = i < n && k < n;
+                q[u] = ab[ok ? (size_t)k * ldab + brow : 0];
+            }
+            LDS_BARRIER();
+            if (++rs0 == WR) rs0 = 0;
+            if (++cs0 == WC) cs0 = 0;
         }
-        __syncthreads();
-        if (++rs0 == WR) rs0 = 0;
-        if (++cs0 == WC) cs0 = 0;
     }
     if (tid == 0 && info) info[sys] = s_info;
 }
 
-// Stage rows [r0, r0+w) of band columns [c0, c0+nc) into st[jj*w + rr].
-template <typename T>
-__device__ inline void stage_cols(const T *__restrict__ ab, int ldab, int c0,
-                                  int nc, int r0, int w, T *st)
+// Columns of the factors stream through LDS in chunks; the next chunk is
+// read into registers (TRS_PRE elements per lane, addresses clamped so every
+// lane issues the same loads) while the current one is consumed.
+constexpr int TRS_PRE = 18;
+
+template <typename R>
+__device__ __forceinline__ void chunk_load(R (&rx)[TRS_PRE], R (&ry)[TRS_PRE],
+                                           const cpx<R> *__restrict__ ab,
+                                           int ldab, int c0, int nc, int r0,
+                                           int w)
 {
-    for (int e = threadIdx.x; e < nc * w; e += TRS_NT) {
-        const int jj = e / w, rr = e - jj * w;
-        st[e] = ab[(size_t)(c0 + jj) * ldab + r0 + rr];
+    const int tot = nc * w;
+#pragma unroll
+    for (int m = 0; m < TRS_PRE; ++m) {
+        const int e = threadIdx.x + TRS_NT * m;
+        const bool ok = e < tot;
+        const int jj = ok ? e / w : 0, rr = ok ? e - jj * w : 0;
+        const cpx<R> v = ab[(size_t)(c0 + jj) * ldab + r0 + rr];
+        rx[m] = v.x;
+        ry[m] = v.y;
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ void chunk_store(const R (&rx)[TRS_PRE],
+                                            const R (&ry)[TRS_PRE], cpx<R> *st,
+                                            int nc, int w)
+{
+    const int tot = nc * w;
+#pragma unroll
+    for (int m = 0; m < TRS_PRE; ++m) {
+        const int e = threadIdx.x + TRS_NT * m;
+        if (e < tot) st[e] = cpx<R>{rx[m], ry[m]};
     }
 }
 
@@ -188,9 +275,17 @@ __device__ inline cpx<R> wave_csum(cpx<R> v)
     return v;
 }
 
+template <typename R>
+__device__ inline cpx<R> cj_if(cpx<R> v, bool c)
+{
+    cpx<R> r;
+    r.x = v.x;
+    r.y = c ? -v.y : v.y;
+    return r;
+}
+
 // trans: 0 = 'N', 1 = 'T', 2 = 'C'.  One 64-lane workgroup per system; the
-// right-hand side lives in LDS and the factors stream through LDS in chunks
-// of TRS_CH columns.
+// right-hand side lives in LDS, one barrier per elimination step.
 template <typename R>
 __global__ __launch_bounds__(TRS_NT)
 void k_gbtrs(int trans, int n, int kl, int ku, int nrhs,
@@ -210,103 +305,131 @@ void k_gbtrs(int trans, int n, int kl, int ku, int nrhs,
     const int tid = threadIdx.x;
     const int kv = kl + ku;
     T *x = reinterpret_cast<T *>(smem);
-    T *st = x + n;
-    int *pv = reinterpret_cast<int *>(st + TRS_CH * (kv + 1));
+    T *st = x + n;                                  // TRS_PRE*64 elements
+    int *pv = reinterpret_cast<int *>(st + TRS_PRE * TRS_NT);
     const bool cj = (trans == 2);
-    auto cc = [cj](T v) { return cj ? sz::cconj(v) : v; };
+    const int chL = kl > 0 ? min(64, (TRS_PRE * TRS_NT) / kl) : 1;
+    const int chU = min(64, (TRS_PRE * TRS_NT) / (kv + 1));
+    R rx[TRS_PRE], ry[TRS_PRE];   // next chunk of factor columns
 
     for (int s = 0; s < nrhs; ++s) {
         T *bs = b + (size_t)s * ldb;
         for (int i = tid; i < n; i += TRS_NT) x[i] = bs[i];
-        __syncthreads();
+        LDS_BARRIER();
         if (trans == 0) {
             // L y = P b: multipliers are rows kv+1..kv+kl
-            if (kl > 0) {
-                for (int j0 = 0; j0 < n - 1; j0 += TRS_CH) {
-                    const int nc = min(TRS_CH, n - 1 - j0);
-                    stage_cols(ab, ldab, j0, nc, kv + 1, kl, st);
+            if (kl > 0 && n > 1) {
+                chunk_load(rx, ry, ab, ldab, 0, min(chL, n - 1), kv + 1, kl);
+                for (int j0 = 0; j0 < n - 1; j0 += chL) {
+                    const int nc = min(chL, n - 1 - j0);
+                    chunk_store(rx, ry, st, nc, kl);
                     for (int e = tid; e < nc; e += TRS_NT) pv[e] = ipiv[j0 + e];
-                    __syncthreads();
+                    if (j0 + chL < n - 1)
+                        chunk_load(rx, ry, ab, ldab, j0 + chL, min(chL, n - 1 - j0 - chL),
+                                kv + 1, kl);
+                    LDS_BARRIER();
                     for (int jj = 0; jj < nc; ++jj) {
                         const int j = j0 + jj;
-                        if (tid == 0) {
-                            const int l = pv[jj] - 1;
-                            if (l != j && l >= 0 && l < n) { T t = x[l]; x[l] = x[j]; x[j] = t; }
-                        }
-                        __syncthreads();
-                        const T xj = x[j];
+                        int l = pv[jj] - 1;
+                        if (l < j || l >= n) l = j;
+                        const T xl = x[l], xj = x[j];
                         const int lm = min(kl, n - 1 - j);
-                        for (int r = tid; r < lm; r += TRS_NT)
-                            x[j + 1 + r] = sz::cfnms(x[j + 1 + r], st[jj * kl + r], xj);
-                        __syncthreads();
+                        // after the interchange x[j] holds old x[l] and x[l]
+                        // holds old x[j]; then x[j+1+r] -= m_r x[j]
+                        for (int r = tid; r < lm; r += TRS_NT) {
+                            const int i = j + 1 + r;
+                            const T base = (i == l) ? xj : x[i];
+                            x[i] = sz::cfnms(base, st[jj * kl + r], xl);
+                        }
+                        if (tid == 0) x[j] = xl;
+                        LDS_BARRIER();
                     }
                 }
             }
-            // U x = y: column j of U is rows kv-t (t = 0..kv) of band col j
-            for (int j1 = n - 1; j1 >= 0; j1 -= TRS_CH) {
-                const int nc = min(TRS_CH, j1 + 1);
+            // U x = y: column j of U is band rows kv-t (t = 0..kv)
+            chunk_load(rx, ry, ab, ldab, n - min(chU, n), min(chU, n), 0, kv + 1);
+            for (int j1 = n - 1; j1 >= 0; j1 -= chU) {
+                const int nc = min(chU, j1 + 1);
                 const int c0 = j1 - nc + 1;
-                stage_cols(ab, ldab, c0, nc, 0, kv + 1, st);
-                __syncthreads();
+                chunk_store(rx, ry, st, nc, kv + 1);
+                if (c0 > 0) {
+                    const int nn = min(chU, c0);
+                    chunk_load(rx, ry, ab, ldab, c0 - nn, nn, 0, kv + 1);
+                }
+                LDS_BARRIER();
                 for (int jj = nc - 1; jj >= 0; --jj) {
                     const int j = c0 + jj;
                     const T *u = st + jj * (kv + 1) + kv;   // u[-t] = U(j-t, j)
-                    if (tid == 0 && sz::cnonzero(x[j])) x[j] = sz::cdiv(x[j], u[0]);
-                    __syncthreads();
-                    const T xj = x[j];
-                    if (sz::cnonzero(xj)) {
+                    const T xo = x[j];
+                    if (sz::cnonzero(xo)) {
+                        const T xj = sz::cdiv(xo, u[0]);
                         const int tmax = min(kv, j);
                         for (int t = 1 + tid; t <= tmax; t += TRS_NT)
                             x[j - t] = sz::cfnms(x[j - t], u[-t], xj);
+                        if (tid == 0) x[j] = xj;
                     }
-                    __syncthreads();
+                    LDS_BARRIER();
                 }
             }
         } else {
             // U^T y = b, ascending
-            for (int j0 = 0; j0 < n; j0 += TRS_CH) {
-                const int nc = min(TRS_CH, n - j0);
-                stage_cols(ab, ldab, j0, nc, 0, kv + 1, st);
-                __syncthreads();
+            chunk_load(rx, ry, ab, ldab, 0, min(chU, n), 0, kv + 1);
+            for (int j0 = 0; j0 < n; j0 += chU) {
+                const int nc = min(chU, n - j0);
+                chunk_store(rx, ry, st, nc, kv + 1);
+                if (j0 + chU < n)
+                    chunk_load(rx, ry, ab, ldab, j0 + chU, min(chU, n - j0 - chU), 0, kv + 1);
+                LDS_BARRIER();
                 for (int jj = 0; jj < nc; ++jj) {
                     const int j = j0 + jj;
                     const T *u = st + jj * (kv + 1) + kv;
                     T acc = {0, 0};
                     const int tmax = min(kv, j);
                     for (int t = 1 + tid; t <= tmax; t += TRS_NT)
-                        acc = sz::cfma(acc, cc(u[-t]), x[j - t]);
+                        acc = sz::cfma(acc, cj_if(u[-t], cj), x[j - t]);
                     acc = wave_csum(acc);
-                    if (tid == 0) x[j] = sz::cdiv(sz::csub(x[j], acc), cc(u[0]));
-                    __syncthreads();
+                    if (tid == 0) x[j] = sz::cdiv(sz::csub(x[j], acc), cj_if(u[0], cj));
+                    LDS_BARRIER();
                 }
             }
             // L^T x = y then undo the interchanges, descending
-            if (kl > 0) {
-                for (int j1 = n - 2; j1 >= 0; j1 -= TRS_CH) {
-                    const int nc = min(TRS_CH, j1 + 1);
+            if (kl > 0 && n > 1) {
+                {
+                    const int nn = min(chL, n - 1);
+                    chunk_load(rx, ry, ab, ldab, n - 1 - nn, nn, kv + 1, kl);
+                }
+                for (int j1 = n - 2; j1 >= 0; j1 -= chL) {
+                    const int nc = min(chL, j1 + 1);
                     const int c0 = j1 - nc + 1;
-                    stage_cols(ab, ldab, c0, nc, kv + 1, kl, st);
+                    chunk_store(rx, ry, st, nc, kl);
                     for (int e = tid; e < nc; e += TRS_NT) pv[e] = ipiv[c0 + e];
-                    __syncthreads();
+                    if (c0 > 0) {
+                        const int nn = min(chL, c0);
+                        chunk_load(rx, ry, ab, ldab, c0 - nn, nn, kv + 1, kl);
+                    }
+                    LDS_BARRIER();
                     for (int jj = nc - 1; jj >= 0; --jj) {
                         const int j = c0 + jj;
                         const int lm = min(kl, n - 1 - j);
                         T acc = {0, 0};
                         for (int r = tid; r < lm; r += TRS_NT)
-                            acc = sz::cfma(acc, cc(st[jj * kl + r]), x[j + 1 + r]);
+                            acc = sz::cfma(acc, cj_if(st[jj * kl + r], cj), x[j + 1 + r]);
                         acc = wave_csum(acc);
                         if (tid == 0) {
-                            x[j] = sz::csub(x[j], acc);
-                            const int l = pv[jj] - 1;
-                            if (l != j && l >= 0 && l < n) { T t = x[l]; x[l] = x[j]; x[j] = t; }
+                            const T xj = sz::csub(x[j], acc);
+                            int l = pv[jj] - 1;
+                            if (l < j || l >= n) l = j;
+                            const T xl = x[l];
+                            x[l] = xj;
+                            if (l != j) x[j] = xl;
                         }
-                        __syncthreads();
+                        LDS_BARRIER();
                     }
                 }
             }
         }
         for (int i = tid; i < n; i += TRS_NT) bs[i] = x[i];
-        __syncthreads();
+        LDS_BARRIER();
     }
 }
 
@@ -361,8 +484,9 @@ int launch_gbtrs(char trans, int n, int kl, int ku, int nrhs, const void *ab,
     if (sb < (long long)ldb * nrhs) return -13;
     if (nsys < 0) return -14;
     if (nsys == 0 || n == 0 || nrhs == 0) return 0;
-    const size_t lds = ((size_t)n + (size_t)TRS_CH * (kl + ku + 1)) * sizeof(cpx<R>)
-                       + TRS_CH * sizeof(int);
+    if (kl + ku + 1 > TRS_PRE * TRS_NT) return SZ_ERR_LDS;
+    const size_t lds = ((size_t)n + (size_t)TRS_PRE * TRS_NT) * sizeof(cpx<R>)
+                       + 64 * sizeof(int);
     if (lds > LDS_MAX) return SZ_ERR_LDS;
     hipLaunchKernelGGL(k_gbtrs<R>, dim3(nsys), dim3(TRS_NT), lds, stream, tc, n,
                        kl, ku, nrhs, reinterpret_cast<const cpx<R> *>(ab), ldab,
