@@ -63,7 +63,9 @@ int sz_bsmbsm_zdpack_batched(int S, int n, int ihat, int jhat, int kl, int ku,
                              void *stream);
 
 /* Model implicit operator of the batched-solve workload assembled straight
- * into LU-ready band storage (rows KL..2KL+KU of leading dimension ldlu):
+ * into band storage of P A P^T: element (r, c) at lu + c*ldlu + KU + r - c,
+ * so lu = LU + KL with ldlu = LD + KL gives LU-ready (packf) storage and
+ * lu = PAPT with ldlu = LD the plain (packc) form:
  *   B_ij = d_ij M - phidt (nu D2 - nu (kx^2 + kz^2) M + c_ij D1 + i kx u_ij M)
  * with M, D1, D2 the transposed collocation bands of sz_bsplineop
  * (d_t = sz_bsplineop_D_T_device, dstride = n*ld, kuaddr = max_ku, dkl/dku
@@ -162,6 +164,52 @@ int sz_diffwave_accumulate(int dxcnt, int dzcnt, const double *alpha,
                            double Lx, double Lz, int Ny, int Nx, int dNx,
                            int dkbx, int dkex, int Nz, int dNz, int dkbz,
                            int dkez, void *stream);
+
+
+/* ------------------------------------------ mixed-precision refinement
+ * Complex float LU + complex double iterative refinement for every system,
+ * falling back to a complex double LU: suzerain_lapackext_zcgbsvx
+ * (suzerain/blas_et_al/lapackext.h:280-299, dsgbsvx.def:70-313).  A is in
+ * band storage (ldab >= kl+ku+1, never modified); afb (ldafb >= 2kl+ku+1)
+ * receives the factors, the single precision ones as complex float in its
+ * leading bytes.  fact/apprx/afrob are per-system device arrays carried
+ * between calls (factorization reuse); siter/diter/tolsc are the requested
+ * limits and tolerance scale, shared by the batch; siter_out, diter_out,
+ * tolsc_out, res and info receive each system's outcome.  r is an N-element
+ * workspace per system; work is sz_zcgbsvx_workspace_bytes(nbatch) bytes of
+ * device memory. */
+long long sz_zcgbsvx_workspace_bytes(int nbatch);
+int sz_zcgbsvx_batched(char *fact, int *apprx, int aiter, char trans, int n,
+                       int kl, int ku, const void *ab, int ldab,
+                       long long strideab, double *afrob, void *afb, int ldafb,
+                       long long strideafb, int *ipiv, long long strideipiv,
+                       const void *b, long long strideb, void *x,
+                       long long stridex, int siter, int diter, double tolsc,
+                       int *siter_out, int *diter_out, double *tolsc_out,
+                       void *r, long long strider, double *res, int *info,
+                       void *work, int nbatch, void *stream);
+
+/* ------------------------------------------------- solver handle (C view)
+ * C handle over suzerain_amd::bsmbsm_solver (include/suzerain_amd_solver.hpp),
+ * the batched counterpart of suzerain::bsmbsm_solver (suzerain/
+ * bsmbsm_solver.hpp:70-330): build from a specification string
+ * (suzerain/specification_zgbsv.cpp:56-114 grammar), pack into PAPT, supply
+ * right hand sides, solve, demand solutions. */
+int sz_spec_normalize(const char *spec, char *out, int outlen);
+void *sz_solver_create(const char *spec, int S, int n, int kl, int ku,
+                       int nrhs, int nbatch, void *stream, char *err,
+                       int errlen);
+void sz_solver_free(void *h);
+void *sz_solver_storage(void *h, int which, int *ld, long long *stride);
+void *sz_solver_zcgbsvx_stats(void *h, int which);
+int sz_solver_supply_B(void *h, const void *B, int ldB, int incB,
+                       long long strideB);
+int sz_solver_demand_X(void *h, void *X, int ldX, int incX, long long strideX);
+void sz_solver_supplied_PAPT(void *h);
+int sz_solver_apprx(void *h, int acceptable);
+char sz_solver_fact(void *h);
+int sz_solver_solve(void *h, char trans, int nrhs, char *err, int errlen);
+int sz_solver_solve_async(void *h, char trans, int nrhs);
 
 #ifdef __cplusplus
 }

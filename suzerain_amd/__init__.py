@@ -214,12 +214,21 @@ def model_assemble(op, S, kx, kz, cij, uij, phidt, nu, lu, KL, KU):
     """Assemble the batched-solve model operator into LU-ready storage lu
     shaped (nsys, N, 2KL+KU+1); kx/kz float64 device tensors (nsys,),
     cij/uij float64 device tensors (S, S)."""
+    nsys, N, ldlu = lu.shape
+    return model_assemble_ptr(op, S, kx, kz, cij, uij, phidt, nu,
+                              lu.data_ptr() + KL * lu.element_size(), ldlu,
+                              lu.stride(0), nsys, KL, KU)
+
+
+def model_assemble_ptr(op, S, kx, kz, cij, uij, phidt, nu, papt, ldpapt,
+                       stride, nsys, KL, KU):
+    """As model_assemble, into raw band storage: element (r, c) of system s at
+    papt + s*stride + c*ldpapt + KU + r - c (complex128 elements)."""
     nsub = min(3, op.nderiv + 1)
     dkl = (C.c_int * 3)(*op.kl[:nsub], *([0] * (3 - nsub)))
     dku = (C.c_int * 3)(*op.ku[:nsub], *([0] * (3 - nsub)))
-    nsys, N, ldlu = lu.shape
     rc = load().sz_model_assemble_batched(
         S, op.n, op.device_ptr(), op.ld, op.max_ku, nsub, dkl, dku,
         op.n * op.ld, _ptr(kx), _ptr(kz), _ptr(cij), _ptr(uij), phidt, nu, KL,
-        KU, _ptr(lu), ldlu, lu.stride(0), nsys, _stream())
+        KU, C.c_void_p(papt), ldpapt, stride, nsys, _stream())
     return check(rc, "sz_model_assemble_batched")

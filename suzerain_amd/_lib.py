@@ -75,6 +75,24 @@ SIGNATURES = {
     "sz_diffwave_accumulate": (i32, [i32, i32, pd, vp, pd, vp, f64, f64, i32,
                                      i32, i32, i32, i32, i32, i32, i32, i32,
                                      vp]),
+    "sz_zcgbsvx_workspace_bytes": (i64, [i32]),
+    "sz_zcgbsvx_batched": (i32, [vp, vp, i32, ch, i32, i32, i32, vp, i32, i64,
+                                 vp, vp, i32, i64, vp, i64, vp, i64, vp, i64,
+                                 i32, i32, f64, vp, vp, vp, vp, i64, vp, vp,
+                                 vp, i32, vp]),
+    "sz_spec_normalize": (i32, [C.c_char_p, C.c_char_p, i32]),
+    "sz_solver_create": (vp, [C.c_char_p, i32, i32, i32, i32, i32, i32, vp,
+                              C.c_char_p, i32]),
+    "sz_solver_free": (None, [vp]),
+    "sz_solver_storage": (vp, [vp, i32, pi, C.POINTER(C.c_longlong)]),
+    "sz_solver_zcgbsvx_stats": (vp, [vp, i32]),
+    "sz_solver_supply_B": (i32, [vp, vp, i32, i32, i64]),
+    "sz_solver_demand_X": (i32, [vp, vp, i32, i32, i64]),
+    "sz_solver_supplied_PAPT": (None, [vp]),
+    "sz_solver_apprx": (i32, [vp, i32]),
+    "sz_solver_fact": (ch, [vp]),
+    "sz_solver_solve": (i32, [vp, ch, i32, C.c_char_p, i32]),
+    "sz_solver_solve_async": (i32, [vp, ch, i32]),
 }
 
 _lib = None

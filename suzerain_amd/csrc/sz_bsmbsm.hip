@@ -78,8 +78,9 @@ void k_pack(int S, int n, int ihat, int jhat, int kl, int ku, zd alpha,
 //   B_ij = d_ij M - phidt (nu D2 - nu (kx^2+kz^2) M + c_ij D1 + i kx u_ij M)
 // with M, D1, D2 the collocation operators stored transposed (D_T[d](j,i) =
 // D_d(i,j) at i*ld + kuaddr + j - i, nonzero for i-ku_d <= j <= i+kl_d).
-// Every band slot of rows KL..2KL+KU is written (zeros outside the operator's
-// footprint), so the result is ready for the batched LU.
+// Element (r, c) of P A P^T lands at c*ldlu + KU + r - c: pass LU + KL with
+// ld = LD + KL for LU-ready (packf) storage or PAPT with ld = LD (packc).
+// Every band slot is written (zeros outside the operator's footprint).
 struct band3 { int kl[3], ku[3]; };
 
 __global__ __launch_bounds__(NT)
@@ -100,7 +101,7 @@ void k_model_assemble(int S, int n, const double *__restrict__ dt, int ld,
     const long long cs = e / rows;
     const int c = (int)(cs % N);
     const int s = (int)(cs / N);
-    const int r = c + rr - KU;              // PAP^T row of band row KL+rr
+    const int r = c + rr - KU;              // PAP^T row of band row rr
     zd v = {0, 0};
     if (r >= 0 && r < N) {
         const int i = r / S, ih = r - i * S;
@@ -118,7 +119,7 @@ void k_model_assemble(int S, int n, const double *__restrict__ dt, int ld,
         v.x = diag - phidt * inner;
         v.y = -(phidt * (kx[s] * u_ * m));
     }
-    lu[(size_t)s * slu + (size_t)c * ldlu + KL + rr] = v;
+    lu[(size_t)s * slu + (size_t)c * ldlu + rr] = v;
 }
 
 inline int ok_launch() { return hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH; }
@@ -235,7 +236,7 @@ int sz_model_assemble_batched(int S, int n, const double *d_t, int ld,
     }
     if (KL < S * (maxkl + 1) - 1) return -16;
     if (KU < S * (maxku + 1) - 1) return -17;
-    if (ldlu < 2 * KL + KU + 1) return -19;
+    if (ldlu < KL + KU + 1) return -19;
     if (nsys < 0) return -21;
     const long long work = (long long)nsys * S * n * (KL + KU + 1);
     if (work == 0) return 0;

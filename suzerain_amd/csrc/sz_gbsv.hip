@@ -21,11 +21,13 @@
 // row swap and the scaling; then all four waves apply the rank-1 update.
 #include "sz_common.hpp"
 
+#include <cstdlib>
+#include <utility>
+
 namespace {
 
 using sz::cpx;
 
-constexpr int TRF_NT = 256;
 constexpr int TRS_NT = 64;
 constexpr size_t LDS_MAX = 160 * 1024;
 
@@ -63,16 +65,17 @@ __device__ __forceinline__ cpx<R> ld_or_zero(const cpx<R> *p, bool ok)
     return v;
 }
 
-template <typename R>
+template <typename R, int TRF_NT>
 __global__ __launch_bounds__(TRF_NT)
 void k_gbtrf(int n, int kl, int ku, cpx<R> *__restrict__ ab, int ldab,
              long long sab, int *__restrict__ ipiv, long long sipiv,
-             int *__restrict__ info, int nsys)
+             int *__restrict__ info, const int *__restrict__ skip, int nsys)
 {
     using T = cpx<R>;
     extern __shared__ __align__(16) unsigned char smem[];
     const int sys = blockIdx.x;
     if (sys >= nsys) return;
+    if (skip && skip[sys] != 0) return;   // masked out by the caller
     ab += (size_t)sys * (size_t)sab;
     ipiv += (size_t)sys * (size_t)sipiv;
     const int tid = threadIdx.x;
@@ -95,17 +98,25 @@ void k_gbtrf(int n, int kl, int ku, cpx<R> *__restrict__ ab, int ldab,
     // the prefetched rows below are waited for only when they are consumed.
     // Thread column cl brings in element cl of the row entering at each step:
     // row i = j + WR, column k = j + 1 + cl, i.e. band row kv + kl - cl.
-    const int cl = tid % WC;
-    const int brow = kv + kl - cl;
+    constexpr int NCL = TRF_NT >= 128 ? 1 : 2;   // entering-row columns per thread
+    int cl[NCL], brow[NCL];
+#pragma unroll
+    for (int m = 0; m < NCL; ++m) {
+        cl[m] = (tid + TRF_NT * m) % WC;
+        brow[m] = kv + kl - cl[m];
+    }
     const int rl = 1 + tid % (kl > 0 ? kl : 1);
-    T q[TRF_PF];
-    // q[u] holds the raw element (read from a clamped, always valid
+    // q[u][m] holds the raw element (read from a clamped, always valid
     // address); rows or columns beyond n are zeroed when it is consumed.
+    T q[TRF_PF][NCL];
 #pragma unroll
     for (int u = 0; u < TRF_PF; ++u) {
-        const int i = u + WR, k = u + 1 + cl;
-        const bool ok = i < n && k < n;
-        q[u] = ab[ok ? (size_t)k * ldab + brow : 0];
+#pragma unroll
+        for (int m = 0; m < NCL; ++m) {
+            const int i = u + WR, k = u + 1 + cl[m];
+            const bool ok = i < n && k < n;
+            q[u][m] = ab[ok ? (size_t)k * ldab + brow[m] : 0];
+        }
     }
     __shared__ int s_p;
     LDS_BARRIER();
@@ -198,8 +209,11 @@ void k_gbtrf(int n, int kl, int ku, cpx<R> *__restrict__ ab, int ldab,
             // U(j,j) when this thread has nothing of its own to write)
             {
                 const int cmax = min(kv, n - 1 - j);
-                const int ce = cl <= cmax ? cl : 0;
-                ab[(size_t)(j + ce) * ldab + kv - ce] = Urow[ce];
+#pragma unroll
+                for (int m = 0; m < NCL; ++m) {
+                    const int ce = cl[m] <= cmax ? cl[m] : 0;
+                    ab[(size_t)(j + ce) * ldab + kv - ce] = Urow[ce];
+                }
                 const int re = (kl > 0 && rl <= km) ? rl : 0;
                 ab[(size_t)j * ldab + kv + re] = re ? Lcol[re] : Urow[0];
                 ipiv[j] = j + s_p + 1;
@@ -208,18 +222,19 @@ void k_gbtrf(int n, int kl, int ku, cpx<R> *__restrict__ ab, int ldab,
             // row slot rs0 becomes row j+kl+1 (prefetched PF steps ago)
             for (int r = tid; r < WR; r += TRF_NT)
                 if (r != rs0) W[r * WC + cs0] = T{0, 0};
-            if (tid < WC) {
-                const bool in = (j + WR < n) && (j + 1 + cl < n);
-                int cs = cs0 + 1 + cl; if (cs >= WC) cs -= WC;
-                T qv;
-                qv.x = in ? q[u].x : R(0);
-                qv.y = in ? q[u].y : R(0);
-                W[rs0 * WC + cs] = qv;
-            }
-            {
-                const int i = j + TRF_PF + WR, k = j + TRF_PF + 1 + cl;
+#pragma unroll
+            for (int m = 0; m < NCL; ++m) {
+                if (tid + TRF_NT * m < WC) {
+                    const bool in = (j + WR < n) && (j + 1 + cl[m] < n);
+                    int cs = cs0 + 1 + cl[m]; if (cs >= WC) cs -= WC;
+                    T qv;
+                    qv.x = in ? q[u][m].x : R(0);
+                    qv.y = in ? q[u][m].y : R(0);
+                    W[rs0 * WC + cs] = qv;
+                }
+                const int i = j + TRF_PF + WR, k = j + TRF_PF + 1 + cl[m];
                 const bool ok = i < n && k < n;
-                q[u] = ab[ok ? (size_t)k * ldab + brow : 0];
+                q[u][m] = ab[ok ? (size_t)k * ldab + brow[m] : 0];
             }
             LDS_BARRIER();
             if (++rs0 == WR) rs0 = 0;
@@ -227,6 +242,316 @@ void k_gbtrf(int n, int kl, int ku, cpx<R> *__restrict__ ab, int ldab,
         }
     }
     if (tid == 0 && info) info[sys] = s_info;
+}
+
+// ---------------------------------------------------------------------------
+// Register-tiled factorization.  The same circular (kl+1) x (kl+ku+1) window
+// as k_gbtrf, but every element lives in a register of the thread that owns
+// its (row slot, column slot) tile for the whole factorization: thread
+// (tr, tc) owns row slots tr*RT .. tr*RT+RT-1 and column slots
+// tc*CT .. tc*CT+CT-1.  Only the row leaving the window (U row j), the
+// multipliers of column j, the pivot candidates and the incoming row pass
+// through LDS, double-buffered by step parity, so a step costs two barriers
+// and RT*CT complex FMAs per thread.  Rows and columns outside the active
+// ranges receive zero multipliers, which leaves them exactly unchanged.
+// Compile-time loops: the body sees std::integral_constant indices, so tile
+// arrays are only ever indexed by constants and stay in registers.
+template <typename F, int... I>
+__device__ __forceinline__ void sfor_impl(F &&f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+
+template <int N, typename F>
+__device__ __forceinline__ void sfor(F &&f)
+{
+    sfor_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+template <int RT, int CT>
+struct tile_sel {
+    // element (a, b) with a runtime row a (get_row) or column b (get_col)
+    template <int B, typename R>
+    static __device__ __forceinline__ R get_row(const R (&v)[RT * CT], int a)
+    {
+        R x = v[B];
+        sfor<RT>([&](auto aa) {
+            if constexpr (aa.value > 0) x = (aa.value == a) ? v[aa.value * CT + B] : x;
+        });
+        return x;
+    }
+    template <int A, typename R>
+    static __device__ __forceinline__ R get_col(const R (&v)[RT * CT], int b)
+    {
+        R x = v[A * CT];
+        sfor<CT>([&](auto bb) {
+            if constexpr (bb.value > 0) x = (bb.value == b) ? v[A * CT + bb.value] : x;
+        });
+        return x;
+    }
+    template <int B, typename R>
+    static __device__ __forceinline__ void set_row(R (&v)[RT * CT], int a, R x)
+    {
+        sfor<RT>([&](auto aa) {
+            v[aa.value * CT + B] = (aa.value == a) ? x : v[aa.value * CT + B];
+        });
+    }
+    template <int A, typename R>
+    static __device__ __forceinline__ void set_col(R (&v)[RT * CT], int b, R x)
+    {
+        sfor<CT>([&](auto bb) {
+            v[A * CT + bb.value] = (bb.value == b) ? x : v[A * CT + bb.value];
+        });
+    }
+};
+
+constexpr int REG_PF = 4;   // incoming rows prefetched ahead of their step
+
+template <typename R, int RT, int CT, int NT>
+__global__ __launch_bounds__(NT)
+void k_gbtrf_reg(int n, int kl, int ku, cpx<R> *__restrict__ ab, int ldab,
+                 long long sab, int *__restrict__ ipiv, long long sipiv,
+                 int *__restrict__ info, const int *__restrict__ skip, int nsys)
+{
+    using T = cpx<R>;
+    using S = tile_sel<RT, CT>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int sys = blockIdx.x;
+    if (sys >= nsys) return;
+    if (skip && skip[sys] != 0) return;
+    ab += (size_t)sys * (size_t)sab;
+    ipiv += (size_t)sys * (size_t)sipiv;
+    const int tid = threadIdx.x;
+    const int kv = kl + ku;
+    const int WR = kl + 1, WC = kv + 1;
+    const int NRG = (WR + RT - 1) / RT, NCG = (WC + CT - 1) / CT;
+    const int tr = tid / NCG, tc = tid - tr * NCG;
+    const bool tile = tr < NRG;
+
+    // LDS: per parity {Uold[WC], Unew[WC], newrow[WC], Lcol[WR], cval[NRG]},
+    // then cabs[2][NRG] and crow[2][NRG]
+    const int PB = 3 * WC + WR + NRG;
+    T *tb = reinterpret_cast<T *>(smem);
+    R *cabs_ = reinterpret_cast<R *>(tb + 2 * PB);
+    int *crow_ = reinterpret_cast<int *>(cabs_ + 2 * NRG);
+
+    R vx[RT * CT], vy[RT * CT];
+    sfor<RT * CT>([&](auto e) {
+        constexpr int a = e.value / CT, b = e.value % CT;
+        const int r = tr * RT + a, c = tc * CT + b;
+        const bool ok = tile && r < WR && c < WC && r < n && c < n &&
+                        r - c <= kl && c - r <= ku;
+        const T v = ab[ok ? (size_t)c * ldab + kv + r - c : 0];
+        vx[e.value] = ok ? v.x : R(0);
+        vy[e.value] = ok ? v.y : R(0);
+    });
+    // Every thread issues the same global loads and stores each step (see
+    // k_gbtrf); thread column cl brings in element cl of the entering row.
+    const int cl = tid % WC;
+    const int brow = kv + kl - cl;
+    const int rl = 1 + tid % (kl > 0 ? kl : 1);
+    T q[REG_PF];
+#pragma unroll
+    for (int u = 0; u < REG_PF; ++u) {
+        const int i = u + WR, k = u + 1 + cl;
+        const bool ok = i < n && k < n;
+        q[u] = ab[ok ? (size_t)k * ldab + brow : 0];
+    }
+
+    int ju = 0, first_zero = 0;
+    int rs0 = 0, cs0 = 0;
+    for (int j0 = 0; j0 < n; j0 += REG_PF) {
+#pragma unroll
+        for (int u = 0; u < REG_PF; ++u) {
+            const int j = j0 + u;
+            if (j >= n) break;
+            const int par = j & 1;
+            T *Uold = tb + par * PB, *Unew = Uold + WC, *newrow = Unew + WC;
+            T *Lcol = newrow + WC, *cval = Lcol + WR;
+            R *cabs = cabs_ + par * NRG;
+            int *crow = crow_ + par * NRG;
+            const int km = min(kl, n - 1 - j);
+            const int tr0 = rs0 / RT, a0i = rs0 - tr0 * RT;
+            const int tc0 = cs0 / CT, b0 = cs0 - tc0 * CT;
+
+            // ---- phase A: local pivot candidates, row j, incoming row
+            if (tile && tc == tc0) {
+                R best = R(-2);
+                int br = 0x7fffffff;
+                R bx = R(0), by = R(0);
+                sfor<RT>([&](auto ac) {
+                    constexpr int a = ac.value;
+                    const int rs = tr * RT + a;
+                    int r = rs - rs0; if (r < 0) r += WR;
+                    const R x = S::template get_col<a>(vx, b0);
+                    const R y = S::template get_col<a>(vy, b0);
+                    R ab1 = fabs(x) + fabs(y);
+                    if (ab1 != ab1) ab1 = R(-1);
+                    const bool cand = rs < WR && r <= km;
+                    const bool take = cand && (ab1 > best || (ab1 == best && r < br));
+                    best = take ? ab1 : best;
+                    br = take ? r : br;
+                    bx = take ? x : bx;
+                    by = take ? y : by;
+                });
+                cabs[tr] = best;
+                crow[tr] = br;
+                cval[tr] = T{bx, by};
+            }
+            if (tile && tr == tr0) {
+                sfor<CT>([&](auto bc) {
+                    constexpr int b = bc.value;
+                    const int cs = tc * CT + b;
+                    int c = cs - cs0; if (c < 0) c += WC;
+                    if (cs < WC)
+                        Uold[c] = T{S::template get_row<b>(vx, a0i),
+                                    S::template get_row<b>(vy, a0i)};
+                });
+            }
+            if (tid < WC) {
+                const bool in = (j + WR < n) && (j + 1 + tid < n);
+                T qv;
+                qv.x = in ? q[u].x : R(0);
+                qv.y = in ? q[u].y : R(0);
+                newrow[tid] = qv;
+            }
+            LDS_BARRIER();
+
+            // ---- phase B: pivot, interchange, multipliers
+            R best = R(-3);
+            int p = 0;
+            T piv = {0, 0};
+            for (int g = 0; g < NRG; ++g) {
+                const R a = cabs[g];
+                const int r = crow[g];
+                if (a > best || (a == best && r < p)) {
+                    best = a; p = r; piv = cval[g];
+                }
+            }
+            const T a0 = Uold[0];
+            const R a0abs = sz::cabs1(a0);
+            if (a0abs != a0abs) { p = 0; piv = a0; }
+            const bool nz = sz::cnonzero(piv);
+            if (nz) ju = max(ju, min(j + ku + p, n - 1));
+            else if (first_zero == 0) first_zero = j + 1;
+            const bool swap = nz && p != 0;
+            int prs = rs0 + p; if (prs >= WR) prs -= WR;
+            const int w = ju - j;
+            T rcp;
+            rcp.x = R(0);
+            rcp.y = R(0);
+            if (nz) rcp = sz::crcp(piv);
+            const int trp = prs / RT, ap = prs - trp * RT;
+            if (swap && tile && tr == trp) {
+                sfor<CT>([&](auto bc) {
+                    constexpr int b = bc.value;
+                    const int cs = tc * CT + b;
+                    int c = cs - cs0; if (c < 0) c += WC;
+                    if (cs < WC && c <= w) {
+                        Unew[c] = T{S::template get_row<b>(vx, ap),
+                                    S::template get_row<b>(vy, ap)};
+                        const T o = Uold[c];
+                        S::template set_row<b>(vx, ap, o.x);
+                        S::template set_row<b>(vy, ap, o.y);
+                    }
+                });
+            }
+            if (tile && tc == tc0) {
+                sfor<RT>([&](auto ac) {
+                    constexpr int a = ac.value;
+                    const int rs = tr * RT + a;
+                    int r = rs - rs0; if (r < 0) r += WR;
+                    if (rs < WR && r >= 1 && r <= km) {
+                        T v;
+                        v.x = S::template get_col<a>(vx, b0);
+                        v.y = S::template get_col<a>(vy, b0);
+                        const T l = sz::cmul(v, rcp);
+                        T o;
+                        o.x = nz ? l.x : v.x;
+                        o.y = nz ? l.y : v.y;
+                        Lcol[r] = o;
+                    }
+                });
+            }
+            LDS_BARRIER();
+
+            // ---- phase C: rank-1 update, outputs, window shift
+            {
+                R lx[RT], ly[RT], ux[CT], uy[CT];
+                sfor<RT>([&](auto ac) {
+                    constexpr int a = ac.value;
+                    const int rs = tr * RT + a;
+                    int r = rs - rs0; if (r < 0) r += WR;
+                    const bool lr = nz && rs < WR && r >= 1 && r <= km;
+                    const T l = Lcol[lr ? r : 0];
+                    lx[a] = lr ? l.x : R(0);
+                    ly[a] = lr ? l.y : R(0);
+                });
+                sfor<CT>([&](auto bc) {
+                    constexpr int b = bc.value;
+                    const int cs = tc * CT + b;
+                    int c = cs - cs0; if (c < 0) c += WC;
+                    const bool lc = cs < WC && c >= 1 && c <= w;
+                    const T uo = Uold[lc ? c : 0], un = Unew[lc ? c : 0];
+                    ux[b] = lc ? (swap ? un.x : uo.x) : R(0);
+                    uy[b] = lc ? (swap ? un.y : uo.y) : R(0);
+                });
+                sfor<RT * CT>([&](auto e) {
+                    constexpr int a = e.value / CT, b = e.value % CT;
+                    const T nv = sz::cfnms(T{vx[e.value], vy[e.value]},
+                                           T{lx[a], ly[a]}, T{ux[b], uy[b]});
+                    vx[e.value] = nv.x;
+                    vy[e.value] = nv.y;
+                });
+            }
+            {
+                const int cmax = min(kv, n - 1 - j);
+                const int ce = cl <= cmax ? cl : 0;
+                const T uo = Uold[ce], un = Unew[ce];
+                const bool sw = swap && ce <= w;
+                T ur;
+                ur.x = sw ? un.x : uo.x;
+                ur.y = sw ? un.y : uo.y;
+                ab[(size_t)(j + ce) * ldab + kv - ce] = ur;
+                const int re = (kl > 0 && rl <= km) ? rl : 0;
+                const T lv = Lcol[re > 0 ? re : 0];
+                const T u0 = swap ? Unew[0] : Uold[0];
+                T lo;
+                lo.x = re ? lv.x : u0.x;
+                lo.y = re ? lv.y : u0.y;
+                ab[(size_t)j * ldab + kv + re] = lo;
+                ipiv[j] = j + p + 1;
+            }
+            if (tile && tc == tc0) {
+                sfor<RT>([&](auto ac) {
+                    constexpr int a = ac.value;
+                    const int rs = tr * RT + a;
+                    if (rs != rs0) {
+                        S::template set_col<a>(vx, b0, R(0));
+                        S::template set_col<a>(vy, b0, R(0));
+                    }
+                });
+            }
+            if (tile && tr == tr0) {
+                sfor<CT>([&](auto bc) {
+                    constexpr int b = bc.value;
+                    const int cs = tc * CT + b;
+                    int c1 = cs - cs0 - 1; if (c1 < 0) c1 += WC;
+                    const T nv = newrow[cs < WC ? c1 : 0];
+                    S::template set_row<b>(vx, a0i, cs < WC ? nv.x : R(0));
+                    S::template set_row<b>(vy, a0i, cs < WC ? nv.y : R(0));
+                });
+            }
+            {
+                const int i = j + REG_PF + WR, k = j + REG_PF + 1 + cl;
+                const bool ok = i < n && k < n;
+                q[u] = ab[ok ? (size_t)k * ldab + brow : 0];
+            }
+            if (++rs0 == WR) rs0 = 0;
+            if (++cs0 == WC) cs0 = 0;
+        }
+    }
+    if (tid == 0 && info) info[sys] = first_zero;
 }
 
 // Columns of the factors stream through LDS in chunks; the next chunk is
@@ -433,10 +758,32 @@ void k_gbtrs(int trans, int n, int kl, int ku, int nrhs,
     }
 }
 
+template <typename R, int RT, int CT, int NT>
+bool try_reg(int n, int kl, int ku, void *ab, int ldab, long long sab,
+             int *ipiv, long long sipiv, int *info, const int *skip, int nsys,
+             hipStream_t stream, int *rc)
+{
+    const int WR = kl + 1, WC = kl + ku + 1;
+    const int NRG = (WR + RT - 1) / RT, NCG = (WC + CT - 1) / CT;
+    if (NRG * NCG > NT || WC > NT) return false;
+    const size_t lds = (size_t)2 * (3 * WC + WR + NRG) * sizeof(cpx<R>) +
+                       (size_t)2 * NRG * (sizeof(R) + sizeof(int));
+    hipLaunchKernelGGL((k_gbtrf_reg<R, RT, CT, NT>), dim3(nsys), dim3(NT), lds,
+                       stream, n, kl, ku, reinterpret_cast<cpx<R> *>(ab), ldab,
+                       sab, ipiv, sipiv, info, skip, nsys);
+    *rc = hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH;
+    return true;
+}
+
+// SZ_GBTRF_KERNEL selects the factorization variant.  Default: the LDS
+// window on 256 threads ("lds64"/"lds128" use fewer threads); "reg" and
+// "reg128" select the register-tiled kernel (5x2 tiles on 256 threads or 7x3
+// on 128), which measures slower on gfx950 because its per-step selects and
+// index arithmetic outweigh the saved LDS traffic.
 template <typename R>
 int launch_gbtrf(int n, int kl, int ku, void *ab, int ldab, long long sab,
-                 int *ipiv, long long sipiv, int *info, int nsys,
-                 hipStream_t stream)
+                 int *ipiv, long long sipiv, int *info, const int *skip,
+                 int nsys, hipStream_t stream)
 {
     if (n < 0) return -1;
     if (kl < 0) return -2;
@@ -446,12 +793,41 @@ int launch_gbtrf(int n, int kl, int ku, void *ab, int ldab, long long sab,
     if (sipiv < n) return -8;
     if (nsys < 0) return -10;
     if (nsys == 0 || n == 0) return 0;
+    static const char *sel = getenv("SZ_GBTRF_KERNEL");
+    const bool want_lds = !(sel && sel[0] == 'r');
+    const bool want_128 = sel && sel[0] == 'r' && sel[3] == '1';
+    int rc = 0;
+    if (!want_lds) {
+        if (want_128 && try_reg<R, 7, 3, 128>(n, kl, ku, ab, ldab, sab, ipiv,
+                                              sipiv, info, skip, nsys, stream, &rc))
+            return rc;
+        if (try_reg<R, 5, 2, 256>(n, kl, ku, ab, ldab, sab, ipiv, sipiv, info,
+                                  skip, nsys, stream, &rc))
+            return rc;
+        if (try_reg<R, 8, 4, 256>(n, kl, ku, ab, ldab, sab, ipiv, sipiv, info,
+                                  skip, nsys, stream, &rc))
+            return rc;
+    }
     const size_t lds = ((size_t)(kl + 1) * (kl + ku + 1) + (kl + ku + 1) + (kl + 1))
                        * sizeof(cpx<R>);
     if (lds > LDS_MAX) return SZ_ERR_LDS;
-    hipLaunchKernelGGL(k_gbtrf<R>, dim3(nsys), dim3(TRF_NT), lds, stream, n, kl,
-                       ku, reinterpret_cast<cpx<R> *>(ab), ldab, sab, ipiv,
-                       sipiv, info, nsys);
+    int nt = (sel && sel[0] == 'l' && sel[3] == '6') ? 64
+           : (sel && sel[0] == 'l' && sel[3] == '1') ? 128 : 256;
+    // every column of the window and every multiplier needs a thread
+    while (nt < 256 && (kl + ku + 1 > 2 * nt || kl > nt)) nt *= 2;
+    if (kl + ku + 1 > 256 || kl > 256) return SZ_ERR_LDS;
+    if (nt == 64)
+        hipLaunchKernelGGL((k_gbtrf<R, 64>), dim3(nsys), dim3(64), lds, stream, n,
+                           kl, ku, reinterpret_cast<cpx<R> *>(ab), ldab, sab,
+                           ipiv, sipiv, info, skip, nsys);
+    else if (nt == 128)
+        hipLaunchKernelGGL((k_gbtrf<R, 128>), dim3(nsys), dim3(128), lds, stream, n,
+                           kl, ku, reinterpret_cast<cpx<R> *>(ab), ldab, sab,
+                           ipiv, sipiv, info, skip, nsys);
+    else
+        hipLaunchKernelGGL((k_gbtrf<R, 256>), dim3(nsys), dim3(256), lds, stream, n,
+                           kl, ku, reinterpret_cast<cpx<R> *>(ab), ldab, sab,
+                           ipiv, sipiv, info, skip, nsys);
     return hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH;
 }
 
@@ -504,7 +880,7 @@ int sz_zgbtrf_batched(int n, int kl, int ku, void *ab, int ldab,
                       int *info, int nbatch, void *stream)
 {
     return launch_gbtrf<double>(n, kl, ku, ab, ldab, strideab, ipiv, strideipiv,
-                                info, nbatch, (hipStream_t)stream);
+                                info, nullptr, nbatch, (hipStream_t)stream);
 }
 
 int sz_cgbtrf_batched(int n, int kl, int ku, void *ab, int ldab,
@@ -512,7 +888,7 @@ int sz_cgbtrf_batched(int n, int kl, int ku, void *ab, int ldab,
                       int *info, int nbatch, void *stream)
 {
     return launch_gbtrf<float>(n, kl, ku, ab, ldab, strideab, ipiv, strideipiv,
-                               info, nbatch, (hipStream_t)stream);
+                               info, nullptr, nbatch, (hipStream_t)stream);
 }
 
 int sz_zgbtrs_batched(char trans, int n, int kl, int ku, int nrhs,
@@ -533,6 +909,35 @@ int sz_cgbtrs_batched(char trans, int n, int kl, int ku, int nrhs,
     return launch_gbtrs<float>(trans, n, kl, ku, nrhs, ab, ldab, strideab, ipiv,
                                strideipiv, b, ldb, strideb, nullptr, nbatch,
                                (hipStream_t)stream);
+}
+
+int sz_internal_gbtrf_masked(int single, int n, int kl, int ku, void *ab,
+                             int ldab, long long strideab, int *ipiv,
+                             long long strideipiv, int *info, const int *skip,
+                             int nbatch, void *stream)
+{
+    if (single)
+        return launch_gbtrf<float>(n, kl, ku, ab, ldab, strideab, ipiv,
+                                   strideipiv, info, skip, nbatch,
+                                   (hipStream_t)stream);
+    return launch_gbtrf<double>(n, kl, ku, ab, ldab, strideab, ipiv, strideipiv,
+                                info, skip, nbatch, (hipStream_t)stream);
+}
+
+int sz_internal_gbtrs_masked(int single, char trans, int n, int kl, int ku,
+                             int nrhs, const void *ab, int ldab,
+                             long long strideab, const int *ipiv,
+                             long long strideipiv, void *b, int ldb,
+                             long long strideb, const int *skip, int nbatch,
+                             void *stream)
+{
+    if (single)
+        return launch_gbtrs<float>(trans, n, kl, ku, nrhs, ab, ldab, strideab,
+                                   ipiv, strideipiv, b, ldb, strideb, skip,
+                                   nbatch, (hipStream_t)stream);
+    return launch_gbtrs<double>(trans, n, kl, ku, nrhs, ab, ldab, strideab, ipiv,
+                                strideipiv, b, ldb, strideb, skip, nbatch,
+                                (hipStream_t)stream);
 }
 
 int sz_zgbsv_batched(char trans, int n, int kl, int ku, int nrhs, void *ab,
