@@ -166,6 +166,31 @@ int sz_diffwave_accumulate(int dxcnt, int dzcnt, const double *alpha,
                            int dkez, void *stream);
 
 
+/* --------------------------------------------------- dealiased transforms
+ * Single-GPU replacement of pencil_grid::transform_wave_to_physical /
+ * transform_physical_to_wave (suzerain/pencil_grid.hpp:200-237; P3DFFT
+ * btran_c2r / ftran_r2c, lib/suzerain-p3dfft/module.F90:596,740) for nfields
+ * fields at element strides swave/sphys.  Wave: complex (Y, X, Z) column
+ * major, X extent dNx/2+1, Z extent dNz; physical: real (X, Z, Y), X
+ * fastest.  Unnormalized, FFTW signs.  Only modes with kx <= (Nx-1)/2 and
+ * |kz| <= (Nz-1)/2 are carried: wave_to_physical treats the others as zero
+ * (the caller has zeroed them, as suzerain's zero_dealiasing_modes does) and
+ * physical_to_wave writes zeros there.  mixed is an intermediate of
+ * sz_fft_mixed_elements(plan) complex elements per field (stride smixed). */
+typedef struct sz_fft_plan sz_fft_plan;
+sz_fft_plan *sz_fft_plan_create(int Ny, int Nx, int dNx, int Nz, int dNz,
+                                int *info);
+void sz_fft_plan_free(sz_fft_plan *p);
+long long sz_fft_mixed_elements(const sz_fft_plan *p);
+int sz_fft_wave_to_physical(const sz_fft_plan *p, int nfields,
+                            const void *wave, long long swave, void *mixed,
+                            long long smixed, double *phys, long long sphys,
+                            void *stream);
+int sz_fft_physical_to_wave(const sz_fft_plan *p, int nfields,
+                            const double *phys, long long sphys, void *mixed,
+                            long long smixed, void *wave, long long swave,
+                            void *stream);
+
 /* ------------------------------------------ mixed-precision refinement
  * Complex float LU + complex double iterative refinement for every system,
  * falling back to a complex double LU: suzerain_lapackext_zcgbsvx

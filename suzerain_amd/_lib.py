@@ -75,6 +75,11 @@ SIGNATURES = {
     "sz_diffwave_accumulate": (i32, [i32, i32, pd, vp, pd, vp, f64, f64, i32,
                                      i32, i32, i32, i32, i32, i32, i32, i32,
                                      vp]),
+    "sz_fft_plan_create": (vp, [i32, i32, i32, i32, i32, pi]),
+    "sz_fft_plan_free": (None, [vp]),
+    "sz_fft_mixed_elements": (i64, [vp]),
+    "sz_fft_wave_to_physical": (i32, [vp, i32, vp, i64, vp, i64, vp, i64, vp]),
+    "sz_fft_physical_to_wave": (i32, [vp, i32, vp, i64, vp, i64, vp, i64, vp]),
     "sz_zcgbsvx_workspace_bytes": (i64, [i32]),
     "sz_zcgbsvx_batched": (i32, [vp, vp, i32, ch, i32, i32, i32, vp, i32, i64,
                                  vp, vp, i32, i64, vp, i64, vp, i64, vp, i64,
