@@ -35,7 +35,25 @@ def parse():
     p.add_argument("--cpu-sample", type=int, default=1152,
                    help="systems in the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                   help="only run a plain device-to-device copy of MIB MiB "
+                        "three times (reference reading for HBM traffic "
+                        "counters) and exit")
     return p.parse_args()
+
+
+def copy_calibration(mib):
+    """A plain copy of known size: its FETCH_SIZE / WRITE_SIZE readings give
+    the factor between counter units and bytes moved."""
+    import torch
+    nbytes = mib << 20
+    src = torch.ones(nbytes, dtype=torch.uint8, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": nbytes, "launches": 3}),
+          flush=True)
 
 
 def cpu_baseline(inp, op, nsample):
@@ -95,6 +113,9 @@ def main():
               "whole-job throughput is the one-GPU value times the GPU count.",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     import torch
     import suzerain_amd as sa
     from suzerain_amd.workload import implicit_solve_inputs

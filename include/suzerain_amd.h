@@ -214,6 +214,92 @@ int sz_zcgbsvx_batched(char *fact, int *apprx, int aiter, char trans, int n,
                        void *r, long long strider, double *res, int *info,
                        void *work, int nbatch, void *stream);
 
+/* ------------------------------------------- compressible implicit operator
+ * M + phi L for the state (rho E, rho u, rho v, rho w, rho), L linearized
+ * about wall-normal reference profiles (suzerain/rholut_imexop.h:66-441). */
+
+/* suzerain_rholut_imexop_scenario (rholut_imexop.h:66-73). */
+typedef struct sz_imexop_scenario {
+    double Re, Pr, Ma, alpha, gamma;
+} sz_imexop_scenario;
+
+/* suzerain_rholut_imexop_ref / _refld (rholut_imexop.h:76-130): DEVICE
+ * pointers to the 26 reference profiles (n values each, stride ld). */
+typedef struct sz_imexop_ref {
+    const double *ux, *uy, *uz, *u2, *uxux, *uxuy, *uxuz, *uyuy, *uyuz, *uzuz,
+        *nu, *nuux, *nuuy, *nuuz, *nuu2, *nuuxux, *nuuxuy, *nuuxuz, *nuuyuy,
+        *nuuyuz, *nuuzuz, *ex_gradrho, *ey_gradrho, *ez_gradrho, *e_divm,
+        *e_deltarho;
+} sz_imexop_ref;
+typedef struct sz_imexop_refld {
+    int ux, uy, uz, u2, uxux, uxuy, uxuz, uyuy, uyuz, uzuz, nu, nuux, nuuy,
+        nuuz, nuu2, nuuxux, nuuxuy, nuuxuz, nuuyuy, nuuyuz, nuuzuz,
+        ex_gradrho, ey_gradrho, ez_gradrho, e_divm, e_deltarho;
+} sz_imexop_refld;
+
+/* specification_isothermal wall data used by IsothermalPATPTEnforcer
+ * (apps/perfect/operator_hybrid_isothermal.cpp:396-526). */
+typedef struct sz_isothermal_bc {
+    int enforce_lower, enforce_upper;
+    double lower_T, lower_u, lower_v, lower_w;
+    double upper_T, upper_u, upper_v, upper_w;
+} sz_isothermal_bc;
+
+/* Operator handle: the term table of rholut_imexop.c:121-487 for one
+ * scenario on the current device, bound to B-spline operators w (nderiv >= 2,
+ * borrowed; must outlive the handle).  info: 0, -1/-2 bad argument,
+ * SZ_ERR_ALLOC. */
+typedef struct sz_imexop sz_imexop;
+sz_imexop *sz_imexop_create(const sz_imexop_scenario *s, sz_bsplineop *w,
+                            int *info);
+void sz_imexop_free(sz_imexop *op);
+int sz_imexop_nterms(const sz_imexop *op);
+
+/* Batched suzerain_rholut_imexop_packc (rholut_imexop.h:381-401,
+ * rholut_imexop.def:41-500): for each system s, with wavenumbers km[s],
+ * kn[s] (device arrays), write P (M + phi L)^T P^T into every in-matrix band
+ * slot of patpt + s*stride, element (R, C) at C*ldpatpt + KU + R - C.
+ * rho_E..rho give each field's block position (negative omits it); A_T
+ * receives the geometry when non-NULL.  When bc is non-NULL the isothermal
+ * wall equations are written in place of the assembled ones, exactly as
+ * IsothermalPATPTEnforcer::op would rewrite them afterwards
+ * (operator_hybrid_isothermal.cpp:470-510).  phi is (re, im). */
+int sz_rholut_imexop_packc_batched(
+    const sz_imexop *op, const double *phi, const double *km, const double *kn,
+    const sz_imexop_ref *r, const sz_imexop_refld *ld, int rho_E, int rho_u,
+    int rho_v, int rho_w, int rho, sz_bsmbsm *A_T, void *patpt, int ldpatpt,
+    long long stride, int nsys, const sz_isothermal_bc *bc, void *stream);
+
+/* Batched suzerain_rholut_imexop_packf (rholut_imexop.h:403-441): as packc
+ * into LU storage (ld >= LD + KL), writing the band at lu + KL. */
+int sz_rholut_imexop_packf_batched(
+    const sz_imexop *op, const double *phi, const double *km, const double *kn,
+    const sz_imexop_ref *r, const sz_imexop_refld *ld, int rho_E, int rho_u,
+    int rho_v, int rho_w, int rho, sz_bsmbsm *A_T, void *lu, int ldlu,
+    long long stride, int nsys, const sz_isothermal_bc *bc, void *stream);
+
+/* Batched suzerain_rholut_imexop_accumulate (rholut_imexop.h:186-211,
+ * rholut_imexop.c:43-531) without the non-reflecting a, b, c matrices:
+ * out[f] + o(s) <- (M + phi L) x + beta (out[f] + o(s)) for each present
+ * field f in (rho E, rho u, rho v, rho w, rho) order, where x field j is
+ * in[j] + i(s); i(s) = inoff[s] when inoff is non-NULL else s*instride, o(s)
+ * likewise.  in[j] and out[j] are both NULL or both non-NULL (NULL omits the
+ * field).  beta == 0 never reads out.  Complex elements, stride 1 along y. */
+int sz_rholut_imexop_accumulate_batched(
+    const sz_imexop *op, const double *phi, const double *km, const double *kn,
+    const sz_imexop_ref *r, const sz_imexop_refld *ld,
+    const void *const *in, const long long *inoff, long long instride,
+    const double *beta, void *const *out, const long long *outoff,
+    long long outstride, int nsys, void *stream);
+
+/* IsothermalPATPTEnforcer::rhs (operator_hybrid_isothermal.cpp:513-524),
+ * batched: zero the permuted right hand side (pb + s*stride, length S*n) of
+ * the energy and momentum equations at the enforced walls. */
+int sz_isothermal_rhs_batched(int S, int n, int rho_E, int rho_u, int rho_v,
+                              int rho_w, int enforce_lower, int enforce_upper,
+                              void *pb, long long stride, int nsys,
+                              void *stream);
+
 /* ------------------------------------------------- solver handle (C view)
  * C handle over suzerain_amd::bsmbsm_solver (include/suzerain_amd_solver.hpp),
  * the batched counterpart of suzerain::bsmbsm_solver (suzerain/

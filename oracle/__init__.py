@@ -60,6 +60,16 @@ def lib():
             "sz_or_zcgbsvx": (i, [vp, vp, i, C.c_char, i, i, i, vp, vp, vp, vp,
                                   vp, vp, vp, vp, vp, vp, vp]),
             "sz_or_dgbmv_d_z": (i, [i, i, i, _Z, vp, i, i, vp, i, _Z, vp, i]),
+            "sz_or_imexop_terms": (i, [vp, vp, vp, vp, vp, vp, i]),
+            "sz_or_imexop_accumulate": (i, [_Z, C.c_double, C.c_double, vp, vp,
+                                            vp, i, vp, i, C.c_long, i, vp, vp,
+                                            vp, _Z, vp]),
+            "sz_or_imexop_pack": (i, [_Z, C.c_double, C.c_double, vp, vp, vp,
+                                      i, vp, i, C.c_long, i, i, vp, vp, vp, vp,
+                                      i]),
+            "sz_or_isothermal_op": (i, [i, i, i, i, vp, i, vp, vp, vp, vp, i,
+                                        i]),
+            "sz_or_isothermal_rhs": (i, [i, i, vp, vp, i, i]),
             "sz_or_diffwave_accumulate": (None, [i, i, _Z, vp, _Z, vp, C.c_double,
                                                  C.c_double, i, i, i, i, i, i,
                                                  i, i, i]),
@@ -164,3 +174,112 @@ def diffwave_accumulate(dx, dz, alpha, x, beta, y, Lx, Lz, Ny, Nx, dNx, Nz,
                                     Lx, Lz, Ny, Nx, dNx, 0, x.shape[1], Nz, dNz,
                                     0, x.shape[0])
     return y
+
+
+# ------------------------------------------------- compressible operator
+NREF = 26
+REF_NAMES = ("ux uy uz u2 uxux uxuy uxuz uyuy uyuz uzuz nu nuux nuuy nuuz nuu2 "
+             "nuuxux nuuxuy nuuxuz nuuyuy nuuyuz nuuzuz ex_gradrho ey_gradrho "
+             "ez_gradrho e_divm e_deltarho").split()
+
+
+class Ops:
+    """Greville collocation operators M, D1, D2 in the reference's transposed
+    band storage with one shared leading dimension (bsplineop.c:401-652):
+    row r of D_d sits at dt[d, r, max_ku + c - r] for r-ku[d] <= c <= r+kl[d]."""
+
+    def __init__(self, coll=None, bands=None):
+        """From oracle.bspline.collocation() output, or from bands =
+        (dt (3, n, ld), kl, ku, max_kl, max_ku) produced elsewhere (so a
+        comparison can run both sides on identical operator values)."""
+        if bands is not None:
+            dt, kl, ku, self.max_kl, self.max_ku = bands
+            self.dt = np.ascontiguousarray(dt[:3], dtype=np.float64)
+            self.n, self.ld = self.dt.shape[1], self.dt.shape[2]
+            self.kl = np.array(kl[:3], dtype=np.int32)
+            self.ku = np.array(ku[:3], dtype=np.int32)
+            self.D = None
+            return
+        D, kl, ku = coll["D"], coll["kl"], coll["ku"]
+        self.n = coll["n"]
+        self.kl = np.array(kl[:3], dtype=np.int32)
+        self.ku = np.array(ku[:3], dtype=np.int32)
+        self.max_kl, self.max_ku = int(max(kl)), int(max(ku))
+        self.ld = self.max_kl + 1 + self.max_ku
+        from .bspline import band_of
+        self.dt = np.ascontiguousarray(np.stack(
+            [band_of(np.ascontiguousarray(D[d].T), self.max_kl, self.max_ku)
+             for d in range(3)]))
+        self.D = D[:3]
+
+
+def _refs(refs):
+    refs = np.ascontiguousarray(refs, dtype=np.float64)
+    assert refs.shape[0] == NREF
+    ptrs = (vp * NREF)(*[refs[i].ctypes.data for i in range(NREF)])
+    ld = (C.c_int * NREF)(*([1] * NREF))
+    return refs, ptrs, ld
+
+
+def imexop_terms(scenario):
+    sc = np.ascontiguousarray(scenario, dtype=np.float64)
+    m = 256
+    eq, inp, op, ref = (np.zeros(m, np.int32) for _ in range(4))
+    k = np.zeros((m, 6))
+    cnt = lib().sz_or_imexop_terms(_p(sc), _p(eq), _p(inp), _p(op), _p(ref),
+                                   _p(k), m)
+    return eq[:cnt], inp[:cnt], op[:cnt], ref[:cnt], k[:cnt]
+
+
+def imexop_accumulate(phi, km, kn, scenario, refs, ops, x, beta=0.0, y=None,
+                      present=(True,) * 5):
+    """y <- (M + phi L) x + beta y for x, y shaped (5, n) complex128 in
+    (rho E, rho u, rho v, rho w, rho) order (rholut_imexop.c:43-531)."""
+    sc = np.ascontiguousarray(scenario, dtype=np.float64)
+    refs, rp, rld = _refs(refs)
+    x = np.ascontiguousarray(x, dtype=zc)
+    y = np.zeros_like(x) if y is None else y
+    assert y.flags.c_contiguous and y.dtype == zc
+    xin = (vp * 5)(*[x[i].ctypes.data if present[i] else None for i in range(5)])
+    yout = (vp * 5)(*[y[i].ctypes.data if present[i] else None for i in range(5)])
+    lib().sz_or_imexop_accumulate(_z(phi), km, kn, _p(sc), rp, rld, ops.n,
+                                  _p(ops.dt), ops.ld, ops.n * ops.ld,
+                                  ops.max_ku, _p(ops.kl), _p(ops.ku), xin,
+                                  _z(beta), yout)
+    return y
+
+
+def imexop_pack(phi, km, kn, scenario, refs, ops, papt, ldpapt,
+                idx=(0, 1, 2, 3, 4), offset=0):
+    """P (M + phi L)^T P^T into flat band storage papt starting at element
+    `offset` (packc: ldpapt = LD, offset 0; packf: ldpapt = LD + KL, offset
+    KL) following rholut_imexop.def:41-500."""
+    sc = np.ascontiguousarray(scenario, dtype=np.float64)
+    refs, rp, rld = _refs(refs)
+    ix = np.ascontiguousarray(idx, dtype=np.int32)
+    view = papt[offset:]
+    lib().sz_or_imexop_pack(_z(phi), km, kn, _p(sc), rp, rld, ops.n,
+                            _p(ops.dt), ops.ld, ops.n * ops.ld, ops.max_kl,
+                            ops.max_ku, _p(ops.kl), _p(ops.ku), _p(ix),
+                            _p(view), ldpapt)
+    return papt
+
+
+def isothermal_op(S, n, KL, KU, papt, ldpapt, scenario, lower, upper,
+                  idx=(0, 1, 2, 3, 4), enforce=(True, True), offset=0):
+    """IsothermalPATPTEnforcer::op; lower/upper = (T, u, v, w) at each wall."""
+    sc = np.ascontiguousarray(scenario, dtype=np.float64)
+    ix = np.ascontiguousarray(idx, dtype=np.int32)
+    lo = np.ascontiguousarray(lower, dtype=np.float64)
+    up = np.ascontiguousarray(upper, dtype=np.float64)
+    view = papt[offset:]
+    lib().sz_or_isothermal_op(S, n, KL, KU, _p(view), ldpapt, _p(ix), _p(sc),
+                              _p(lo), _p(up), int(enforce[0]), int(enforce[1]))
+    return papt
+
+
+def isothermal_rhs(S, n, pb, idx=(0, 1, 2, 3, 4), enforce=(True, True)):
+    ix = np.ascontiguousarray(idx, dtype=np.int32)
+    lib().sz_or_isothermal_rhs(S, n, _p(pb), _p(ix), int(enforce[0]),
+                               int(enforce[1]))
+    return pb

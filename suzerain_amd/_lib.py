@@ -24,6 +24,29 @@ class sz_bsmbsm(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class sz_imexop_scenario(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("Re", "Pr", "Ma", "alpha", "gamma")]
+
+
+REF_NAMES = ("ux uy uz u2 uxux uxuy uxuz uyuy uyuz uzuz nu nuux nuuy nuuz nuu2 "
+             "nuuxux nuuxuy nuuxuz nuuyuy nuuyuz nuuzuz ex_gradrho ey_gradrho "
+             "ez_gradrho e_divm e_deltarho").split()
+
+
+class sz_imexop_ref(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in REF_NAMES]
+
+
+class sz_imexop_refld(C.Structure):
+    _fields_ = [(k, C.c_int) for k in REF_NAMES]
+
+
+class sz_isothermal_bc(C.Structure):
+    _fields_ = [("enforce_lower", C.c_int), ("enforce_upper", C.c_int)] + [
+        (k, C.c_double) for k in ("lower_T", "lower_u", "lower_v", "lower_w",
+                                  "upper_T", "upper_u", "upper_v", "upper_w")]
+
+
 # name: (restype, [argtypes])
 SIGNATURES = {
     "sz_bsmbsm_construct": (sz_bsmbsm, [i32, i32, i32, i32]),
@@ -85,6 +108,20 @@ SIGNATURES = {
                                  vp, vp, i32, i64, vp, i64, vp, i64, vp, i64,
                                  i32, i32, f64, vp, vp, vp, vp, i64, vp, vp,
                                  vp, i32, vp]),
+    "sz_imexop_create": (vp, [vp, vp, pi]),
+    "sz_imexop_free": (None, [vp]),
+    "sz_imexop_nterms": (i32, [vp]),
+    "sz_rholut_imexop_packc_batched": (i32, [vp, pd, vp, vp, vp, vp, i32, i32,
+                                             i32, i32, i32, vp, vp, i32, i64,
+                                             i32, vp, vp]),
+    "sz_rholut_imexop_packf_batched": (i32, [vp, pd, vp, vp, vp, vp, i32, i32,
+                                             i32, i32, i32, vp, vp, i32, i64,
+                                             i32, vp, vp]),
+    "sz_rholut_imexop_accumulate_batched": (i32, [vp, pd, vp, vp, vp, vp, vp,
+                                                  vp, i64, pd, vp, vp, i64,
+                                                  i32, vp]),
+    "sz_isothermal_rhs_batched": (i32, [i32, i32, i32, i32, i32, i32, i32, i32,
+                                        vp, i64, i32, vp]),
     "sz_spec_normalize": (i32, [C.c_char_p, C.c_char_p, i32]),
     "sz_solver_create": (vp, [C.c_char_p, i32, i32, i32, i32, i32, i32, vp,
                               C.c_char_p, i32]),

@@ -1,0 +1,621 @@
+// The compressible implicit operator M + phi L of the hybrid implicit/explicit
+// channel solver on gfx950: batched assembly of P (M + phi L)^T P^T into
+// LU-ready band storage with the isothermal wall rows fused in, matrix-free
+// application y <- (M + phi L) x + beta y, and the matching right hand side
+// boundary treatment.
+//
+// L (suzerain/rholut_imexop.c:43-531, rholut_imexop.def:41-500) is a 5 x 5
+// block operator over (rho E, rho u, rho v, rho w, rho); block (i, j) is a
+// sum of groups, each group one derivative operator D in {M, D1, D2} scaled
+// on the left by sum_t alpha_t diag(ref_t) with ref_t one of 26 wall-normal
+// reference profiles (or 1) and alpha_t = phi kappa_t(km, kn).  The 96 terms
+// live in one device table built once per scenario; kernels read it through
+// uniform (scalar) loads.
+//
+// Arithmetic (contraction off; identical in the oracle's restatement):
+//   kappa = ((k0 + k1 km^2) + k2 kn^2) + k3 km kn + i (k4 km + k5 kn)
+//   alpha = phi * kappa                   (sz::cmul, fma form)
+//   W_g[r] = alpha_0 ref_0[r] then fma(alpha_t, ref_t[r], W) per component
+//   (D x)[r] = fma chain over D's band, ascending column
+//   accumulate: acc = sum_g W_g (D_g x_j) by sz::cfma, y = ((beta y) + acc)
+//               + (M x_i); beta == 0 never reads y
+//   assembly:   A_ij[r][c] = fma chain of W_g[r] D_g[r][c] over groups whose
+//               band holds c, then + M[r][c] on diagonal blocks
+#include "sz_common.hpp"
+#include "../../include/suzerain_amd.h"
+
+#include <cstring>
+#include <vector>
+
+namespace {
+
+using sz::zd;
+
+constexpr int NREF = 26;
+constexpr int MAXTERMS = 128;
+constexpr int MAXGROUPS = 64;
+enum { M = 0, D1 = 1, D2 = 2 };
+enum {
+    ux, uy, uz, u2, uxux, uxuy, uxuz, uyuy, uyuz, uzuz,
+    nu, nuux, nuuy, nuuz, nuu2, nuuxux, nuuxuy, nuuxuz, nuuyuy, nuuyuz,
+    nuuzuz, ex_gradrho, ey_gradrho, ez_gradrho, e_divm, e_deltarho,
+    ONE = -1
+};
+
+// Device-resident term table.
+struct table {
+    int bg0[25], bng[25];                  // block (i*5+j) -> groups
+    int gop[MAXGROUPS], gt0[MAXGROUPS], gnt[MAXGROUPS];
+    int tref[MAXTERMS];
+    double tk[MAXTERMS][6];
+    int nterms, ngroups;
+};
+
+struct refs {
+    const double *p[NREF];
+    int ld[NREF];
+};
+
+struct term_h { int ref; double k[6]; };
+struct group_h { int op; std::vector<term_h> t; };
+
+term_h T(int r, double a, double b, double c, double d, double e, double f)
+{
+    return term_h{r, {a, b, c, d, e, f}};
+}
+
+// The operator's terms, block by block in (rho E, rho u, rho v, rho w, rho)
+// order, with the wavenumber polynomials of rholut_imexop.c:121-487
+// expanded into k0 + k1 km^2 + k2 kn^2 + k3 km kn + i (k4 km + k5 kn).
+void build_table(const double *sc, table &tab)
+{
+    const double Re = sc[0], Pr = sc[1], Ma = sc[2], alpha = sc[3], gamma = sc[4];
+    const double gm1 = gamma - 1, gm3 = gamma - 3;
+    const double ap43 = alpha + 4.0 / 3.0, ap13 = alpha + 1.0 / 3.0;
+    const double Ma2 = Ma * Ma, invRe = 1 / Re, invMa2 = 1 / Ma2;
+    const double ginvPr = gamma / Pr, ginvRePr = gamma / (Re * Pr);
+    const double MiR = Ma2 * invRe, MiRa = Ma2 * invRe * ap13;
+    const double gRPg = ginvRePr / gm1, aiR = ap13 * invRe;
+
+    std::vector<group_h> B[5][5];
+    // energy
+    B[0][0] = {{M, {T(ux, 0, 0, 0, 0, -gamma, 0), T(uz, 0, 0, 0, 0, 0, -gamma),
+                    T(nu, 0, -ginvRePr, -ginvRePr, 0, 0, 0)}},
+               {D1, {T(uy, -gamma, 0, 0, 0, 0, 0)}},
+               {D2, {T(nu, ginvRePr, 0, 0, 0, 0, 0)}}};
+    B[0][1] = {{M, {T(nuux, 0, MiR * (ginvPr - ap43), MiR * (ginvPr - 1), 0, 0, 0),
+                    T(nuuz, 0, 0, 0, -MiRa, 0, 0), T(e_divm, 0, 0, 0, 0, -1, 0)}},
+               {D1, {T(nuuy, 0, 0, 0, 0, MiRa, 0)}},
+               {D2, {T(nuux, MiR * (1 - ginvPr), 0, 0, 0, 0, 0)}}};
+    B[0][2] = {{M, {T(nuuy, 0, MiR * (ginvPr - 1), MiR * (ginvPr - 1), 0, 0, 0)}},
+               {D1, {T(nuux, 0, 0, 0, 0, MiRa, 0), T(nuuz, 0, 0, 0, 0, 0, MiRa),
+                     T(e_divm, -1, 0, 0, 0, 0, 0)}},
+               {D2, {T(nuuy, MiR * (ap43 - ginvPr), 0, 0, 0, 0, 0)}}};
+    B[0][3] = {{M, {T(nuux, 0, 0, 0, -MiRa, 0, 0),
+                    T(nuuz, 0, MiR * (ginvPr - 1), MiR * (ginvPr - ap43), 0, 0, 0),
+                    T(e_divm, 0, 0, 0, 0, 0, -1)}},
+               {D1, {T(nuuy, 0, 0, 0, 0, 0, MiRa)}},
+               {D2, {T(nuuz, MiR * (1 - ginvPr), 0, 0, 0, 0, 0)}}};
+    B[0][4] = {{M, {T(nuu2, 0, MiR, MiR, 0, 0, 0), T(nuuxux, 0, MiRa, 0, 0, 0, 0),
+                    T(nuuxuz, 0, 0, 0, MiRa * 2, 0, 0), T(nuuzuz, 0, 0, MiRa, 0, 0, 0)}},
+               {M, {T(ex_gradrho, 0, 0, 0, 0, -1, 0), T(ez_gradrho, 0, 0, 0, 0, 0, -1),
+                    T(e_deltarho, 0, -gRPg, -gRPg, 0, 0, 0)}},
+               {D1, {T(nuuxuy, 0, 0, 0, 0, -MiRa * 2, 0),
+                     T(nuuyuz, 0, 0, 0, 0, 0, -MiRa * 2), T(ey_gradrho, -1, 0, 0, 0, 0, 0)}},
+               {D2, {T(nuu2, -MiR, 0, 0, 0, 0, 0), T(nuuyuy, -MiRa, 0, 0, 0, 0, 0),
+                     T(e_deltarho, gRPg, 0, 0, 0, 0, 0)}}};
+    // streamwise momentum
+    B[1][0] = {{M, {T(ONE, 0, 0, 0, 0, -gm1 * invMa2, 0)}}};
+    B[1][1] = {{M, {T(ux, 0, 0, 0, 0, gm3, 0), T(uz, 0, 0, 0, 0, 0, -1),
+                    T(nu, 0, -invRe * ap43, -invRe, 0, 0, 0)}},
+               {D1, {T(uy, -1, 0, 0, 0, 0, 0)}},
+               {D2, {T(nu, invRe, 0, 0, 0, 0, 0)}}};
+    B[1][2] = {{M, {T(uy, 0, 0, 0, 0, gm1, 0)}},
+               {D1, {T(ux, -1, 0, 0, 0, 0, 0), T(nu, 0, 0, 0, 0, aiR, 0)}}};
+    B[1][3] = {{M, {T(ux, 0, 0, 0, 0, 0, -1), T(uz, 0, 0, 0, 0, gm1, 0),
+                    T(nu, 0, 0, 0, -aiR, 0, 0)}}};
+    B[1][4] = {{M, {T(u2, 0, 0, 0, 0, -0.5 * gm1, 0), T(uxux, 0, 0, 0, 0, 1, 0),
+                    T(uxuz, 0, 0, 0, 0, 0, 1), T(nuux, 0, invRe * ap43, invRe, 0, 0, 0),
+                    T(nuuz, 0, 0, 0, aiR, 0, 0)}},
+               {D1, {T(uxuy, 1, 0, 0, 0, 0, 0), T(nuuy, 0, 0, 0, 0, -aiR, 0)}},
+               {D2, {T(nuux, -invRe, 0, 0, 0, 0, 0)}}};
+    // wall-normal momentum
+    B[2][0] = {{D1, {T(ONE, -gm1 * invMa2, 0, 0, 0, 0, 0)}}};
+    B[2][1] = {{M, {T(uy, 0, 0, 0, 0, -1, 0)}},
+               {D1, {T(ux, gm1, 0, 0, 0, 0, 0), T(nu, 0, 0, 0, 0, aiR, 0)}}};
+    B[2][2] = {{M, {T(ux, 0, 0, 0, 0, -1, 0), T(uz, 0, 0, 0, 0, 0, -1),
+                    T(nu, 0, -invRe, -invRe, 0, 0, 0)}},
+               {D1, {T(uy, gm3, 0, 0, 0, 0, 0)}},
+               {D2, {T(nu, ap43 * invRe, 0, 0, 0, 0, 0)}}};
+    B[2][3] = {{M, {T(uy, 0, 0, 0, 0, 0, -1)}},
+               {D1, {T(uz, gm1, 0, 0, 0, 0, 0), T(nu, 0, 0, 0, 0, 0, aiR)}}};
+    B[2][4] = {{M, {T(uxuy, 0, 0, 0, 0, 1, 0), T(uyuz, 0, 0, 0, 0, 0, 1),
+                    T(nuuy, 0, invRe, invRe, 0, 0, 0)}},
+               {D1, {T(u2, -0.5 * gm1, 0, 0, 0, 0, 0), T(uyuy, 1, 0, 0, 0, 0, 0),
+                     T(nuux, 0, 0, 0, 0, -aiR, 0), T(nuuz, 0, 0, 0, 0, 0, -aiR)}},
+               {D2, {T(nuuy, -ap43 * invRe, 0, 0, 0, 0, 0)}}};
+    // spanwise momentum
+    B[3][0] = {{M, {T(ONE, 0, 0, 0, 0, 0, -gm1 * invMa2)}}};
+    B[3][1] = {{M, {T(ux, 0, 0, 0, 0, 0, gm1), T(uz, 0, 0, 0, 0, -1, 0),
+                    T(nu, 0, 0, 0, -aiR, 0, 0)}}};
+    B[3][2] = {{M, {T(uy, 0, 0, 0, 0, 0, gm1)}},
+               {D1, {T(uz, -1, 0, 0, 0, 0, 0), T(nu, 0, 0, 0, 0, 0, aiR)}}};
+    B[3][3] = {{M, {T(ux, 0, 0, 0, 0, -1, 0), T(uz, 0, 0, 0, 0, 0, gm3),
+                    T(nu, 0, -invRe, -invRe * ap43, 0, 0, 0)}},
+               {D1, {T(uy, -1, 0, 0, 0, 0, 0)}},
+               {D2, {T(nu, invRe, 0, 0, 0, 0, 0)}}};
+    B[3][4] = {{M, {T(u2, 0, 0, 0, 0, 0, -0.5 * gm1), T(uxuz, 0, 0, 0, 0, 1, 0),
+                    T(uzuz, 0, 0, 0, 0, 0, 1), T(nuuz, 0, invRe, invRe * ap43, 0, 0, 0),
+                    T(nuux, 0, 0, 0, aiR, 0, 0)}},
+               {D1, {T(uyuz, 1, 0, 0, 0, 0, 0), T(nuuy, 0, 0, 0, 0, 0, -aiR)}},
+               {D2, {T(nuuz, -invRe, 0, 0, 0, 0, 0)}}};
+    // density
+    B[4][1] = {{M, {T(ONE, 0, 0, 0, 0, -1, 0)}}};
+    B[4][2] = {{D1, {T(ONE, -1, 0, 0, 0, 0, 0)}}};
+    B[4][3] = {{M, {T(ONE, 0, 0, 0, 0, 0, -1)}}};
+
+    std::memset(&tab, 0, sizeof(tab));
+    int ng = 0, nt = 0;
+    for (int i = 0; i < 5; ++i)
+        for (int j = 0; j < 5; ++j) {
+            tab.bg0[i * 5 + j] = ng;
+            tab.bng[i * 5 + j] = (int)B[i][j].size();
+            for (const group_h &g : B[i][j]) {
+                tab.gop[ng] = g.op;
+                tab.gt0[ng] = nt;
+                tab.gnt[ng] = (int)g.t.size();
+                for (const term_h &t : g.t) {
+                    tab.tref[nt] = t.ref;
+                    for (int q = 0; q < 6; ++q) tab.tk[nt][q] = t.k[q];
+                    ++nt;
+                }
+                ++ng;
+            }
+        }
+    tab.nterms = nt;
+    tab.ngroups = ng;
+}
+
+__device__ inline zd kappa(const double *k, double km, double kn)
+{
+    const double km2 = km * km, kn2 = kn * kn, kmkn = km * kn;
+    double re = k[0] + k[1] * km2;
+    re = re + k[2] * kn2;
+    re = re + k[3] * kmkn;
+    const double a = k[4] * km, b = k[5] * kn;
+    return zd{re, a + b};
+}
+
+__device__ inline double refv(const refs &rf, int t, int r)
+{
+    return t == ONE ? 1.0 : rf.p[t][(long long)r * rf.ld[t]];
+}
+
+// W_g[r] = sum_t alpha_t ref_t[r], first term a product, then fma.
+__device__ inline zd weight(const table *__restrict__ tab, const zd *alpha,
+                            const refs &rf, int g, int r)
+{
+    const int t0 = tab->gt0[g], nt = tab->gnt[g];
+    const double v0 = refv(rf, tab->tref[t0], r);
+    double wx = alpha[t0].x * v0, wy = alpha[t0].y * v0;
+    for (int t = t0 + 1; t < t0 + nt; ++t) {
+        const double v = refv(rf, tab->tref[t], r);
+        wx = fma(alpha[t].x, v, wx);
+        wy = fma(alpha[t].y, v, wy);
+    }
+    return zd{wx, wy};
+}
+
+struct bops {
+    const double *dt;     // derivative d's transposed band at dt + d * dstride
+    long long dstride;
+    int n, ld, max_kl, max_ku;
+    int kl[3], ku[3];
+};
+
+struct wallbc {
+    int enforce[2];       // lower, upper
+    double fac[2][4];     // E_factor, u, v, w per wall
+};
+
+constexpr int PK_NT = 256;
+constexpr int PK_RB = 8;  // wall-normal rows (5 columns each) per workgroup
+
+// Element (r, c) of block (fi, fj) of M + phi L; W holds this chunk's group
+// weights for row r, Dl the chunk's D_T rows.
+__device__ inline zd element(const table *__restrict__ tab, const bops &w,
+                             const zd *Wr, const double *Dl, int rl,
+                             int fi, int fj, int r, int c)
+{
+    double ex = 0, ey = 0;
+    if (c >= r - w.max_ku && c <= r + w.max_kl) {
+        const int b = fi * 5 + fj, g0 = tab->bg0[b], ng = tab->bng[b];
+        for (int g = g0; g < g0 + ng; ++g) {
+            const int op = tab->gop[g];
+            if (c < r - w.ku[op] || c > r + w.kl[op]) continue;
+            const double a = Dl[(op * PK_RB + rl) * w.ld + w.max_ku + c - r];
+            const zd W = Wr[g];
+            ex = fma(W.x, a, ex);
+            ey = fma(W.y, a, ey);
+        }
+        if (fi == fj && c >= r - w.ku[M] && c <= r + w.kl[M])
+            ex = ex + Dl[(M * PK_RB + rl) * w.ld + w.max_ku + c - r];
+    }
+    return zd{ex, ey};
+}
+
+// One workgroup assembles the 5*PK_RB columns of P (M + phi L)^T P^T whose
+// equation rows are r0 .. r0+PK_RB-1 (column C = r*S + p holds equation
+// field_of[p] at wall-normal row r) for one system.
+__global__ __launch_bounds__(PK_NT)
+void k_imexop_pack(const table *__restrict__ tab, bops w, refs rf, zd phi,
+                   const double *__restrict__ km, const double *__restrict__ kn,
+                   int S, int4 fof_lo, int fof_hi, int rho_pos, int KL, int KU,
+                   zd *__restrict__ papt, int ldpapt, long long stride,
+                   int nchunks, wallbc bc)
+{
+    extern __shared__ double smem[];
+    zd *alpha = reinterpret_cast<zd *>(smem);                   // MAXTERMS
+    zd *Wl = alpha + MAXTERMS;                                  // PK_RB x ngroups
+    double *Dl = reinterpret_cast<double *>(Wl + PK_RB * MAXGROUPS);  // 3 x PK_RB x ld
+
+    const int s = blockIdx.x / nchunks;
+    const int chunk = blockIdx.x - s * nchunks;
+    const int n = w.n, N = S * n;
+    const int r0 = chunk * PK_RB;
+    const int nr = min(PK_RB, n - r0);
+    const int tid = threadIdx.x;
+    const int field_of[5] = {fof_lo.x, fof_lo.y, fof_lo.z, fof_lo.w, fof_hi};
+
+    const double kms = km[s], kns = kn[s];
+    for (int t = tid; t < tab->nterms; t += PK_NT)
+        alpha[t] = sz::cmul(phi, kappa(tab->tk[t], kms, kns));
+    for (int e = tid; e < 3 * PK_RB * w.ld; e += PK_NT) {
+        const int op = e / (PK_RB * w.ld), rem = e - op * PK_RB * w.ld;
+        const int rl = rem / w.ld, k = rem - rl * w.ld;
+        Dl[e] = rl < nr ? w.dt[op * w.dstride + (long long)(r0 + rl) * w.ld + k] : 0.0;
+    }
+    __syncthreads();
+    const int ngroups = tab->ngroups;
+    for (int e = tid; e < nr * ngroups; e += PK_NT) {
+        const int rl = e / ngroups, g = e - rl * ngroups;
+        Wl[rl * MAXGROUPS + g] = weight(tab, alpha, rf, g, r0 + rl);
+    }
+    __syncthreads();
+
+    zd *P = papt + (size_t)s * stride;
+    const int LD = KL + 1 + KU;
+    const int ncols = S * nr;
+    for (int e = tid; e < ncols * LD; e += PK_NT) {
+        const int cl = e / LD, rr = e - cl * LD;
+        const int C = S * r0 + cl;
+        const int R = C - KU + rr;
+        if (R < 0 || R >= N) continue;
+        const int pi = C % S, r = C / S, rl = r - r0;
+        const int pj = R % S, c = R / S;
+        const int fi = field_of[pi], fj = field_of[pj];
+        const zd *Wr = Wl + rl * MAXGROUPS;
+        zd v = element(tab, w, Wr, Dl, rl, fi, fj, r, c);
+        // isothermal walls replace the energy and momentum equations
+        const int wall = r == 0 ? 0 : (r == n - 1 ? 1 : -1);
+        if (wall >= 0 && bc.enforce[wall] && fi < 4) {
+            zd sc = element(tab, w, Wr, Dl, rl, fi, fi, r, r);
+            if (sc.x == 0 && sc.y == 0) sc = zd{1, 0};
+            const int rhoR = r * S + rho_pos;
+            const double f = bc.fac[wall][fi];
+            v = R == C ? sc : (R == rhoR ? zd{-sc.x * f, -sc.y * f} : zd{0, 0});
+        }
+        P[(size_t)C * ldpapt + KU + R - C] = v;
+    }
+}
+
+constexpr int AC_NT = 256;
+
+// y_i <- (M + phi L) x + beta y_i for one system per workgroup.  Field f of
+// system s starts at in[f] + inoff(s) (complex elements, stride 1 in y).
+__global__ __launch_bounds__(AC_NT)
+void k_imexop_accumulate(const table *__restrict__ tab, bops w, refs rf, zd phi,
+                         const double *__restrict__ km,
+                         const double *__restrict__ kn, const zd *in0,
+                         const zd *in1, const zd *in2, const zd *in3,
+                         const zd *in4, const long long *__restrict__ inoff,
+                         long long instride, zd beta, int beta_zero, zd *out0,
+                         zd *out1, zd *out2, zd *out3, zd *out4,
+                         const long long *__restrict__ outoff,
+                         long long outstride)
+{
+    extern __shared__ double smem[];
+    const int n = w.n;
+    zd *alpha = reinterpret_cast<zd *>(smem);       // MAXTERMS
+    zd *xs = alpha + MAXTERMS;                      // 5 x n
+    zd *Dx = xs + 5 * n;                            // 3 x 5 x n
+    const int s = blockIdx.x, tid = threadIdx.x;
+    const zd *in[5] = {in0, in1, in2, in3, in4};
+    zd *out[5] = {out0, out1, out2, out3, out4};
+    const long long io = inoff ? inoff[s] : (long long)s * instride;
+    const long long oo = outoff ? outoff[s] : (long long)s * outstride;
+
+    const double kms = km[s], kns = kn[s];
+    for (int t = tid; t < tab->nterms; t += AC_NT)
+        alpha[t] = sz::cmul(phi, kappa(tab->tk[t], kms, kns));
+    for (int e = tid; e < 5 * n; e += AC_NT) {
+        const int f = e / n, y = e - f * n;
+        xs[e] = in[f] ? in[f][io + y] : zd{0, 0};
+    }
+    __syncthreads();
+    for (int e = tid; e < 15 * n; e += AC_NT) {
+        const int op = e / (5 * n), rem = e - op * 5 * n;
+        const int f = rem / n, r = rem - f * n;
+        const zd *x = xs + f * n;
+        const double *row = w.dt + op * w.dstride + (long long)r * w.ld + w.max_ku - r;
+        double tx = 0, ty = 0;
+        const int c1 = min(n - 1, r + w.kl[op]);
+        for (int c = max(0, r - w.ku[op]); c <= c1; ++c) {
+            const double a = row[c];
+            tx = fma(a, x[c].x, tx);
+            ty = fma(a, x[c].y, ty);
+        }
+        Dx[e] = zd{tx, ty};
+    }
+    __syncthreads();
+    for (int e = tid; e < 5 * n; e += AC_NT) {
+        const int i = e / n, r = e - i * n;
+        if (!out[i]) continue;
+        zd acc{0, 0};
+        for (int j = 0; j < 5; ++j) {
+            if (!in[j]) continue;
+            const int b = i * 5 + j, g0 = tab->bg0[b], ng = tab->bng[b];
+            for (int g = g0; g < g0 + ng; ++g) {
+                const zd W = weight(tab, alpha, rf, g, r);
+                acc = sz::cfma(acc, W, Dx[(tab->gop[g] * 5 + j) * n + r]);
+            }
+        }
+        zd *yp = out[i] + oo + r;
+        zd by = beta_zero ? zd{0, 0} : sz::cmul(beta, *yp);
+        const zd sm{by.x + acc.x, by.y + acc.y};
+        const zd m = Dx[(M * 5 + i) * n + r];
+        *yp = zd{sm.x + m.x, sm.y + m.y};
+    }
+}
+
+// Zero the permuted right hand side entries of the replaced wall equations.
+__global__ void k_isothermal_rhs(int S, int n, int4 idx, int lower, int upper,
+                                 zd *pb, long long stride, int nsys)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nsys * 8) return;
+    const int s = e / 8, k = e % 8, wall = k / 4, f = k % 4;
+    if (!(wall ? upper : lower)) return;
+    const int y = wall ? n - 1 : 0;
+    const int pos = f == 0 ? idx.x : f == 1 ? idx.y : f == 2 ? idx.z : idx.w;
+    const int a = pos * n + y;
+    pb[(size_t)s * stride + (a % n) * S + a / n] = zd{0, 0};
+}
+
+inline bool launch_ok() { return hipGetLastError() == hipSuccess; }
+
+}  // namespace
+
+struct sz_imexop {
+    double scen[5];
+    sz_bsplineop *w;
+    table host;
+    table *dev;
+    int dev_id;
+};
+
+extern "C" {
+
+sz_imexop *sz_imexop_create(const sz_imexop_scenario *s, sz_bsplineop *w,
+                            int *info)
+{
+    int dummy;
+    if (!info) info = &dummy;
+    *info = 0;
+    if (!s) { *info = -1; return nullptr; }
+    if (!w || sz_bsplineop_nderiv(w) < 2) { *info = -2; return nullptr; }
+    if (!sz_bsplineop_D_T_device(w)) { *info = SZ_ERR_ALLOC; return nullptr; }
+    sz_imexop *op = new sz_imexop();
+    op->scen[0] = s->Re; op->scen[1] = s->Pr; op->scen[2] = s->Ma;
+    op->scen[3] = s->alpha; op->scen[4] = s->gamma;
+    op->w = w;
+    build_table(op->scen, op->host);
+    hipGetDevice(&op->dev_id);
+    if (hipMalloc(&op->dev, sizeof(table)) != hipSuccess ||
+        hipMemcpy(op->dev, &op->host, sizeof(table), hipMemcpyHostToDevice)
+            != hipSuccess) {
+        hipFree(op->dev);
+        delete op;
+        *info = SZ_ERR_ALLOC;
+        return nullptr;
+    }
+    return op;
+}
+
+void sz_imexop_free(sz_imexop *op)
+{
+    if (!op) return;
+    hipFree(op->dev);
+    delete op;
+}
+
+int sz_imexop_nterms(const sz_imexop *op) { return op ? op->host.nterms : 0; }
+
+}  // extern "C"
+
+namespace {
+
+bops make_bops(sz_bsplineop *w)
+{
+    bops b;
+    b.dt = sz_bsplineop_D_T_device(w);
+    b.n = sz_bsplineop_n(w);
+    b.ld = sz_bsplineop_ld(w);
+    b.dstride = (long long)b.n * b.ld;
+    b.max_kl = sz_bsplineop_max_kl(w);
+    b.max_ku = sz_bsplineop_max_ku(w);
+    for (int d = 0; d < 3; ++d) {
+        b.kl[d] = sz_bsplineop_kl(w, d);
+        b.ku[d] = sz_bsplineop_ku(w, d);
+    }
+    return b;
+}
+
+refs make_refs(const sz_imexop_ref *r, const sz_imexop_refld *ld)
+{
+    refs rf;
+    std::memcpy(rf.p, r, sizeof(rf.p));
+    std::memcpy(rf.ld, ld, sizeof(rf.ld));
+    return rf;
+}
+
+// Positions of (rho E, rho u, rho v, rho w, rho); returns S or a negative
+// argument index.
+int positions(const int *pos, int field_of[5])
+{
+    int S = 0;
+    for (int f = 0; f < 5; ++f) if (pos[f] >= 0) ++S;
+    for (int p = 0; p < 5; ++p) field_of[p] = 0;
+    bool seen[5] = {false, false, false, false, false};
+    for (int f = 0; f < 5; ++f) {
+        if (pos[f] < 0) continue;
+        if (pos[f] >= S || seen[pos[f]]) return -(f + 1);
+        seen[pos[f]] = true;
+        field_of[pos[f]] = f;
+    }
+    return S;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sz_rholut_imexop_packc_batched(
+    const sz_imexop *op, const double *phi, const double *km, const double *kn,
+    const sz_imexop_ref *r, const sz_imexop_refld *ld, int rho_E, int rho_u,
+    int rho_v, int rho_w, int rho, sz_bsmbsm *A_T, void *patpt, int ldpatpt,
+    long long stride, int nsys, const sz_isothermal_bc *bc, void *stream)
+{
+    if (!op) return -1;
+    if (!phi) return -2;
+    if (!km) return -3;
+    if (!kn) return -4;
+    if (!r) return -5;
+    if (!ld) return -6;
+    const int pos[5] = {rho_E, rho_u, rho_v, rho_w, rho};
+    int field_of[5];
+    const int S = positions(pos, field_of);
+    if (S <= 0) return S == 0 ? -7 : -6 + S;
+    const bops w = make_bops(op->w);
+    const sz_bsmbsm g = sz_bsmbsm_construct(S, w.n, w.max_kl, w.max_ku);
+    if (A_T) *A_T = g;
+    if (ldpatpt < g.LD) return -14;
+    if (nsys < 0) return -16;
+    if (nsys == 0) return 0;
+    if (!patpt) return -13;
+    wallbc wb{};
+    if (bc) {
+        if (rho < 0 || rho_E < 0 || rho_u < 0 || rho_v < 0 || rho_w < 0) return -17;
+        const double Ma = op->scen[2], gamma = op->scen[4];
+        const double *spec[2] = {&bc->lower_T, &bc->upper_T};
+        wb.enforce[0] = bc->enforce_lower;
+        wb.enforce[1] = bc->enforce_upper;
+        for (int k = 0; k < 2; ++k) {
+            const double *p = spec[k];   // T, u, v, w
+            wb.fac[k][0] = p[0] / (gamma * (gamma - 1))
+                         + Ma * Ma / 2 * (p[1] * p[1] + p[2] * p[2] + p[3] * p[3]);
+            wb.fac[k][1] = p[1];
+            wb.fac[k][2] = p[2];
+            wb.fac[k][3] = p[3];
+        }
+    }
+    const int nchunks = (w.n + PK_RB - 1) / PK_RB;
+    const size_t lds = sizeof(zd) * (MAXTERMS + PK_RB * MAXGROUPS)
+                     + sizeof(double) * 3 * PK_RB * w.ld;
+    if (lds > 64 * 1024) return SZ_ERR_LDS;
+    hipLaunchKernelGGL(k_imexop_pack, dim3(nsys * nchunks), dim3(PK_NT), lds,
+                       (hipStream_t)stream, op->dev, w, make_refs(r, ld),
+                       zd{phi[0], phi[1]}, km, kn, S,
+                       make_int4(field_of[0], field_of[1], field_of[2], field_of[3]),
+                       field_of[4], rho < 0 ? 0 : rho, g.KL, g.KU, (zd *)patpt,
+                       ldpatpt, stride, nchunks, wb);
+    return launch_ok() ? 0 : SZ_ERR_LAUNCH;
+}
+
+int sz_rholut_imexop_packf_batched(
+    const sz_imexop *op, const double *phi, const double *km, const double *kn,
+    const sz_imexop_ref *r, const sz_imexop_refld *ld, int rho_E, int rho_u,
+    int rho_v, int rho_w, int rho, sz_bsmbsm *A_T, void *lu, int ldlu,
+    long long stride, int nsys, const sz_isothermal_bc *bc, void *stream)
+{
+    if (!op) return -1;
+    const int pos[5] = {rho_E, rho_u, rho_v, rho_w, rho};
+    int field_of[5];
+    const int S = positions(pos, field_of);
+    if (S <= 0) return S == 0 ? -7 : -6 + S;
+    const sz_bsmbsm g = sz_bsmbsm_construct(S, sz_bsplineop_n(op->w),
+                                            sz_bsplineop_max_kl(op->w),
+                                            sz_bsplineop_max_ku(op->w));
+    if (ldlu < g.LD + g.KL) return -14;
+    void *papt = lu ? (void *)((zd *)lu + g.KL) : nullptr;
+    return sz_rholut_imexop_packc_batched(op, phi, km, kn, r, ld, rho_E, rho_u,
+                                          rho_v, rho_w, rho, A_T, papt, ldlu,
+                                          stride, nsys, bc, stream);
+}
+
+int sz_rholut_imexop_accumulate_batched(
+    const sz_imexop *op, const double *phi, const double *km, const double *kn,
+    const sz_imexop_ref *r, const sz_imexop_refld *ld,
+    const void *const *in, const long long *inoff, long long instride,
+    const double *beta, void *const *out, const long long *outoff,
+    long long outstride, int nsys, void *stream)
+{
+    if (!op) return -1;
+    if (!phi) return -2;
+    if (!km) return -3;
+    if (!kn) return -4;
+    if (!r) return -5;
+    if (!ld) return -6;
+    if (!in) return -7;
+    if (!beta) return -10;
+    if (!out) return -11;
+    for (int f = 0; f < 5; ++f)
+        if ((in[f] == nullptr) != (out[f] == nullptr)) return -7;
+    if (nsys < 0) return -14;
+    if (nsys == 0) return 0;
+    const bops w = make_bops(op->w);
+    const size_t lds = sizeof(zd) * (MAXTERMS + 20 * (size_t)w.n);
+    if (lds > 160 * 1024) return SZ_ERR_LDS;
+    const int bz = beta[0] == 0 && beta[1] == 0;
+    hipLaunchKernelGGL(k_imexop_accumulate, dim3(nsys), dim3(AC_NT), lds,
+                       (hipStream_t)stream, op->dev, w, make_refs(r, ld),
+                       zd{phi[0], phi[1]}, km, kn, (const zd *)in[0],
+                       (const zd *)in[1], (const zd *)in[2], (const zd *)in[3],
+                       (const zd *)in[4], inoff, instride, zd{beta[0], beta[1]},
+                       bz, (zd *)out[0], (zd *)out[1], (zd *)out[2],
+                       (zd *)out[3], (zd *)out[4], outoff, outstride);
+    return launch_ok() ? 0 : SZ_ERR_LAUNCH;
+}
+
+int sz_isothermal_rhs_batched(int S, int n, int rho_E, int rho_u, int rho_v,
+                              int rho_w, int enforce_lower, int enforce_upper,
+                              void *pb, long long stride, int nsys,
+                              void *stream)
+{
+    if (S < 4 || S > 5) return -1;
+    if (n < 1) return -2;
+    const int pos[4] = {rho_E, rho_u, rho_v, rho_w};
+    for (int k = 0; k < 4; ++k)
+        if (pos[k] < 0 || pos[k] >= S) return -(3 + k);
+    if (nsys < 0) return -11;
+    if (nsys == 0 || !(enforce_lower || enforce_upper)) return 0;
+    if (!pb) return -9;
+    const int work = nsys * 8;
+    hipLaunchKernelGGL(k_isothermal_rhs, dim3((work + 255) / 256), dim3(256), 0,
+                       (hipStream_t)stream, S, n,
+                       make_int4(rho_E, rho_u, rho_v, rho_w), enforce_lower,
+                       enforce_upper, (zd *)pb, stride, nsys);
+    return launch_ok() ? 0 : SZ_ERR_LAUNCH;
+}
+
+}  // extern "C"
