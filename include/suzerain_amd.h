@@ -300,6 +300,99 @@ int sz_isothermal_rhs_batched(int S, int n, int rho_E, int rho_u, int rho_v,
                               void *pb, long long stride, int nsys,
                               void *stream);
 
+/* -------------------------------------------- pointwise Navier-Stokes flux
+ * The nonlinear right hand side of apps/perfect/navier_stokes.hpp:338-892 on
+ * the dealiased physical grid.  Physical fields are real (Ny, dNz, dNx)
+ * arrays, x fastest; a field set is base + f*stride.  State fields come in
+ * (rho E, rho u, rho v, rho w, rho) order; the 41 derivative fields in the
+ * order of navier_stokes.hpp:159-166 (e_y e_yy div_grad_e e_x e_z, then for
+ * each of mx, my, mz: _y _yy _x _xx _xz _z _zz _xy _yz, then the same nine
+ * for rho). */
+typedef struct sz_ns_scenario {   /* definition_scenario subset */
+    double alpha, beta, gamma, Ma, Pr, Re;
+} sz_ns_scenario;
+#define SZ_LINEARIZE_NONE      0   /* linearize::none      */
+#define SZ_LINEARIZE_RHOME_XYZ 1   /* linearize::rhome_xyz */
+#define SZ_NS_NREF 42              /* references::q::count (references.hpp:59-102) */
+
+/* Device workspace bytes for sz_ns_references / sz_ns_rhs at this grid. */
+long long sz_ns_workspace_bytes(int Ny, int dNz, int dNx);
+
+/* collect_references (apps/perfect/perfect.cpp:1266-1401) without MPI:
+ * refs[q*Ny + j] <- mean over the (x, z) plane j of quantity q, q in the
+ * order of references::q.  Two passes with a fixed summation order. */
+int sz_ns_references(const sz_ns_scenario *s, int Ny, int dNz, int dNx,
+                     const double *P, long long fstride, double *refs,
+                     void *work, void *stream);
+
+/* The explicit right hand side (navier_stokes.hpp:475-752) for linearize =
+ * SZ_LINEARIZE_RHOME_XYZ (uses refs) or SZ_LINEARIZE_NONE, written to the 5
+ * output fields (out may alias P).  When zeroth is nonzero also reduces the
+ * 12 stable step candidates of navier_stokes.hpp:766-892 into candidates
+ * (device, 12 doubles) given per-y lambda1_y/lambda2_y and lxz = {lambda1_x,
+ * lambda1_z, evmaxmag_real, evmaxmag_imag} (device, 4 doubles). */
+int sz_ns_rhs(const sz_ns_scenario *s, int linearize, int zeroth, int Ny,
+              int dNz, int dNx, const double *P, long long fstride,
+              const double *aux, long long astride, double *out,
+              long long ostride, const double *refs, const double *lambda1_y,
+              const double *lambda2_y, const double *lxz, double *candidates,
+              void *work, void *stream);
+
+/* ------------------------------------------------------ channel time step
+ * The channel solver on one GPU (apps/perfect: operator_hybrid_isothermal,
+ * navier_stokes.hpp, perfect.cpp collect_references; suzerain/lowstorage.hpp
+ * step/substep with SMR91).  State and scratch live on the device; the
+ * wave-space state is 5 complex fields (rho E, rho u, rho v, rho w, rho),
+ * each (dNz, dNx/2+1, Ny) in C order with y fastest holding B-spline
+ * coefficients on [0, Ly], field stride from sz_channel_geometry.  State 0
+ * is "a" (the solution), state 1 is "b" (scratch). */
+typedef struct sz_channel_spec {
+    int Nx, Ny, Nz;          /* logical extents; Ny = B-spline DOF         */
+    int dNx, dNz;            /* dealiased transform extents (3/2 rule)     */
+    int k;                   /* B-spline order (6 or 8)                    */
+    double htdelta;          /* two-sided tanh stretching (>= 0)           */
+    double Lx, Ly, Lz;       /* domain lengths                             */
+    sz_ns_scenario scenario; /* alpha, beta, gamma, Ma, Pr, Re             */
+    sz_isothermal_bc bc;     /* isothermal wall data                        */
+    const char *solver;      /* specification_zgbsv string; NULL = "zgbsv" */
+    int linearize;           /* SZ_LINEARIZE_RHOME_XYZ                     */
+} sz_channel_spec;
+typedef struct sz_channel sz_channel;
+
+sz_channel *sz_channel_create(const sz_channel_spec *spec, void *stream,
+                              char *err, int errlen);
+void sz_channel_free(sz_channel *h);
+int sz_channel_geometry(const sz_channel *h, int *Xw, int *nsys,
+                        long long *field_stride, long long *phys_stride);
+/* Base of state 0 or 1 (5 fields), swapped by sz_channel_exchange. */
+void *sz_channel_state(sz_channel *h, int which);
+/* Internal buffers for inspection: 0 refs (42 x Ny), 1 physical fields
+ * (46), 2 wave derivative fields (41), 3 wave values (5), 4 km, 5 kn,
+ * 6 per-system pencil offsets, 7 lambda1_y then lambda2_y, 8 the
+ * sz_bsplineop, 9 the suzerain_amd::bsmbsm_solver. */
+void *sz_channel_buffer(sz_channel *h, int which);
+/* operator_nonlinear::apply_operator (navier_stokes.hpp:107-986): state
+ * `which` <- N(state) in place; on zeroth also the references and the 12
+ * step candidates (copied to host `candidates` when non-NULL). */
+int sz_channel_nonlinear(sz_channel *h, int which, int zeroth,
+                         double *candidates);
+/* apply/accumulate_mass_plus_scaled_operator (operator_hybrid_isothermal.cpp
+ * :103-374): out <- (M + phi L) in + beta out on retained wavenumbers. */
+int sz_channel_accumulate_operator(sz_channel *h, double phi, int in,
+                                   double beta, int out);
+int sz_channel_apply_operator(sz_channel *h, double phi, int which);
+/* invert_mass_plus_scaled_operator (operator_hybrid_isothermal.cpp:528-689):
+ * state <- (M + phi L)^{-1} state with isothermal walls; asynchronous. */
+int sz_channel_invert_operator(sz_channel *h, double phi, int which);
+/* Wait; return the first nonzero per-system solver info, else 0. */
+int sz_channel_check(sz_channel *h);
+int sz_channel_add_scaled(sz_channel *h, double alpha, int src, int dst);
+int sz_channel_copy(sz_channel *h, int src, int dst);
+void sz_channel_exchange(sz_channel *h);
+/* lowstorage::step (lowstorage.hpp:1461-1520) with SMR91 and the
+ * delta_t_reducer; dt_out receives the step taken. */
+int sz_channel_step(sz_channel *h, double max_dt, double *dt_out);
+
 /* ------------------------------------------------- solver handle (C view)
  * C handle over suzerain_amd::bsmbsm_solver (include/suzerain_amd_solver.hpp),
  * the batched counterpart of suzerain::bsmbsm_solver (suzerain/

@@ -198,7 +198,15 @@ class Ops:
             self.n, self.ld = self.dt.shape[1], self.dt.shape[2]
             self.kl = np.array(kl[:3], dtype=np.int32)
             self.ku = np.array(ku[:3], dtype=np.int32)
-            self.D = None
+            n = self.n
+            self.D = []
+            for d in range(3):
+                Dd = np.zeros((n, n))
+                for r in range(n):
+                    for c in range(max(0, r - self.ku[d]),
+                                   min(n, r + self.kl[d] + 1)):
+                        Dd[r, c] = self.dt[d, r, self.max_ku + c - r]
+                self.D.append(Dd)
             return
         D, kl, ku = coll["D"], coll["kl"], coll["ku"]
         self.n = coll["n"]

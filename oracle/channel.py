@@ -153,15 +153,30 @@ def apply_ns(g, swave, scenario, linearize="rhome_xyz", refs=None,
                                            SMR91_EVMAXMAG_IMAG)):
     """Returns (N wave (5, dNz, Xw, Ny), refs dict, delta_t candidates or
     None).  refs: plane means to use (collected here when zeroth)."""
-    alpha, beta, gamma, Ma, Pr, Re = (scenario[k] for k in (
-        "alpha", "beta", "gamma", "Ma", "Pr", "Re"))
     s, aux = aux_fields(g, swave)
     P = g.to_physical(s)                                  # (5, Ny, dNz, dNx)
     A = {k: g.to_physical(v) for k, v in aux.items()}
     if zeroth or refs is None:
         refs = references(g, scenario, P)
-    Ny = g.Ny
-    sh = (Ny, 1, 1)
+    out, cand = rhs_physical(scenario, P, A, refs, linearize,
+                             lam if zeroth else None, evmax)
+    W = g.to_wave(out)
+    for f in range(5):
+        g.zero_dealiasing(W[f])
+    if linearize == "rhome_xyz":
+        W[RHO] = 0
+    return W, refs, cand
+
+
+def rhs_physical(scenario, P, A, refs, linearize="rhome_xyz", lam=None,
+                 evmax=(SMR91_EVMAXMAG_REAL, SMR91_EVMAXMAG_IMAG)):
+    """The pointwise right hand side (navier_stokes.hpp:475-752) on physical
+    fields P (5, Ny, ...) and derivative fields A (dict by AUX name), refs a
+    dict of per-y plane means; returns (out (5, Ny, ...), candidates)."""
+    alpha, beta, gamma, Ma, Pr, Re = (scenario[k] for k in (
+        "alpha", "beta", "gamma", "Ma", "Pr", "Re"))
+    Ny = P.shape[1]
+    sh = (Ny,) + (1,) * (P.ndim - 2)
     R = {k: v.reshape(sh) for k, v in refs.items()}
 
     e, rho = P[E], P[RHO]
@@ -262,16 +277,10 @@ def apply_ns(g, swave, scenario, linearize="rhome_xyz", refs=None,
     out[MX], out[MY], out[MZ] = mom[..., 0], mom[..., 1], mom[..., 2]
 
     cand = None
-    if zeroth and lam is not None:
+    if lam is not None:
         cand = dt_candidates(scenario, linearize, u, T, mu, rho, ref_u,
                              ref_nu, lam, evmax)
-
-    W = g.to_wave(out)
-    for f in range(5):
-        g.zero_dealiasing(W[f])
-    if linearize == "rhome_xyz":
-        W[RHO] = 0
-    return W, refs, cand
+    return out, cand
 
 
 def dt_candidates(scenario, linearize, u, T, mu, rho, ref_u, ref_nu, lam,
@@ -281,8 +290,8 @@ def dt_candidates(scenario, linearize, u, T, mu, rho, ref_u, ref_nu, lam,
                                                       "Pr", "Re"))
     l1x, l2x, l1z, l2z, l1y, l2y = lam
     evr, evi = evmax
-    l1y = l1y.reshape(-1, 1, 1)
-    l2y = l2y.reshape(-1, 1, 1)
+    l1y = np.asarray(l1y).reshape((-1,) + (1,) * (T.ndim - 1))
+    l2y = np.asarray(l2y).reshape((-1,) + (1,) * (T.ndim - 1))
     maxdiffconst = (1 / Re) * max(gamma / Pr, max(1.0, alpha))
     a = np.sqrt(T) / Ma
     ux, uy, uz = u[..., 0], u[..., 1], u[..., 2]
@@ -317,3 +326,84 @@ def dt_candidates(scenario, linearize, u, T, mu, rho, ref_u, ref_nu, lam,
             c[8] = min(c[8], np.min(evr / (dd * md2y[ok])))
             c[11] = min(c[11], np.min(evr / (dd * md2z)))
     return np.minimum(c, np.finfo(float).max)
+
+
+# ------------------------------------------------------------- SMR91 step
+def _systems(g):
+    for iz in range(g.dNz):
+        for ix in range(g.Xw):
+            if g.keep[iz, ix]:
+                yield iz, ix
+
+
+def apply_operator(g, phi, scen5, R, x, beta=0.0, y=None):
+    """y <- (M + phi L) x + beta y on retained wavenumbers (oracle imexop);
+    x, y (5, dNz, Xw, Ny); non-retained entries of y are left as they are."""
+    from . import imexop_accumulate
+    out = np.array(x if y is None else y, dtype=complex, copy=True)
+    for iz, ix in _systems(g):
+        xs = np.ascontiguousarray(x[:, iz, ix, :])
+        ys = np.ascontiguousarray(out[:, iz, ix, :])
+        imexop_accumulate(phi, g.kx[ix], g.kz[iz], scen5, R, g.ops, xs,
+                          beta, ys)
+        out[:, iz, ix, :] = ys
+    return out
+
+
+def invert_operator(g, phi, scen5, R, x, bc):
+    """(M + phi L)^{-1} x with isothermal walls, per retained wavenumber
+    (operator_hybrid_isothermal.cpp:612-689); bc = (lower, upper) (T,u,v,w)."""
+    from . import (imexop_pack, isothermal_op, isothermal_rhs, zaPxpby,
+                   zgbtrf, zgbtrs)
+    S, n = 5, g.Ny
+    KL = S * (g.ops.max_kl + 1) - 1
+    KU = S * (g.ops.max_ku + 1) - 1
+    LD = KL + 1 + KU
+    N = S * n
+    ld = LD + KL
+    out = np.zeros_like(x)
+    for iz, ix in _systems(g):
+        ab = np.zeros(N * ld, dtype=complex)
+        imexop_pack(phi, g.kx[ix], g.kz[iz], scen5, R, g.ops, ab, ld, offset=KL)
+        isothermal_op(S, n, KL, KU, ab, ld, scen5, bc[0], bc[1], offset=KL)
+        p = np.zeros(N, dtype=complex)
+        zaPxpby("N", S, n, 1, np.ascontiguousarray(x[:, iz, ix, :]).reshape(-1),
+                0, p)
+        isothermal_rhs(S, n, p)
+        info, ipiv = zgbtrf(N, KL, KU, ab, ld)
+        if info:
+            raise ArithmeticError(f"zero pivot {info} at ({iz}, {ix})")
+        zgbtrs("T", N, KL, KU, ab, ld, ipiv, p)
+        v = np.zeros(N, dtype=complex)
+        zaPxpby("T", S, n, 1, p, 0, v)
+        out[:, iz, ix, :] = v.reshape(S, n)
+    return out
+
+
+def smr91_step(g, a, scenario, bc, lam, max_dt=0.0):
+    """lowstorage::step (lowstorage.hpp:1461-1520) for the hybrid operator:
+    returns (new a, dt, refs)."""
+    scen5 = [scenario[k] for k in ("Re", "Pr", "Ma", "alpha", "gamma")]
+    chi = 1.0 / (g.dNx * g.dNz)
+    b = a.copy()
+    b, refs, cand = apply_ns(g, b, scenario, "rhome_xyz", zeroth=True, lam=lam)
+    dt = cand[0]
+    for c in cand[1:]:
+        if dt != dt:
+            break
+        if c != c or c < dt:
+            dt = c
+    if max_dt > 0 and dt == dt:
+        dt = min(max_dt, dt)
+    R = ns.imexop_refs(refs)
+    a = apply_operator(g, dt * SMR91_ALPHA[0], scen5, R, a)
+    a = a + chi * dt * SMR91_GAMMA[0] * b
+    a = invert_operator(g, -dt * SMR91_BETA[0], scen5, R, a, bc)
+    for i in (1, 2):
+        b = apply_operator(g, dt * SMR91_ALPHA[i], scen5, R, a,
+                           beta=chi * dt * SMR91_ZETA[i], y=b)
+        a, b = b, a
+        b, _, _ = apply_ns(g, b, scenario, "rhome_xyz", refs=refs, zeroth=False)
+        a = a + chi * dt * SMR91_GAMMA[i] * b
+        a = invert_operator(g, -dt * SMR91_BETA[i], scen5, R, a, bc)
+    return a, dt, refs

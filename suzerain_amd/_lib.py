@@ -122,6 +122,10 @@ SIGNATURES = {
                                                   i32, vp]),
     "sz_isothermal_rhs_batched": (i32, [i32, i32, i32, i32, i32, i32, i32, i32,
                                         vp, i64, i32, vp]),
+    "sz_ns_workspace_bytes": (i64, [i32, i32, i32]),
+    "sz_ns_references": (i32, [vp, i32, i32, i32, vp, i64, vp, vp, vp]),
+    "sz_ns_rhs": (i32, [vp, i32, i32, i32, i32, i32, vp, i64, vp, i64, vp,
+                        i64, vp, vp, vp, vp, vp, vp, vp]),
     "sz_spec_normalize": (i32, [C.c_char_p, C.c_char_p, i32]),
     "sz_solver_create": (vp, [C.c_char_p, i32, i32, i32, i32, i32, i32, vp,
                               C.c_char_p, i32]),

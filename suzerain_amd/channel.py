@@ -1,0 +1,268 @@
+"""Channel solver on one GPU: Python view of the sz_channel C handle.
+
+    ch = Channel(Nx=96, Ny=96, Nz=96, dNx=144, dNz=144, k=8, htdelta=3.0)
+    ch.set_state(laminar_state(ch, seed=...))
+    dt = ch.step()
+
+The step is suzerain's SMR91 low-storage hybrid implicit/explicit step
+(suzerain/lowstorage.hpp:1461-1520) with the rhome_xyz linearization of
+apps/perfect (navier_stokes.hpp, operator_hybrid_isothermal.cpp).  State
+arrays are complex128 (5, dNz, dNx/2+1, Ny): (rho E, rho u, rho v, rho w,
+rho), y holding B-spline coefficients.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from ._lib import load, check, sz_isothermal_bc
+
+__all__ = ["Channel", "laminar_state", "DEFAULT_SCENARIO"]
+
+# apps/perfect/main_initial.cpp:124-134 defaults with the channel case values
+DEFAULT_SCENARIO = {"alpha": 0.0, "beta": 2.0 / 3.0, "gamma": 1.4, "Ma": 1.5,
+                    "Pr": 0.7, "Re": 3000.0}
+
+
+class sz_ns_scenario(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("alpha", "beta", "gamma", "Ma",
+                                          "Pr", "Re")]
+
+
+class sz_channel_spec(C.Structure):
+    _fields_ = [("Nx", C.c_int), ("Ny", C.c_int), ("Nz", C.c_int),
+                ("dNx", C.c_int), ("dNz", C.c_int), ("k", C.c_int),
+                ("htdelta", C.c_double), ("Lx", C.c_double),
+                ("Ly", C.c_double), ("Lz", C.c_double),
+                ("scenario", sz_ns_scenario), ("bc", sz_isothermal_bc),
+                ("solver", C.c_char_p), ("linearize", C.c_int)]
+
+
+_SIGS = {
+    "sz_channel_create": (C.c_void_p, [C.c_void_p, C.c_void_p, C.c_char_p,
+                                       C.c_int]),
+    "sz_channel_free": (None, [C.c_void_p]),
+    "sz_channel_geometry": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p]),
+    "sz_channel_state": (C.c_void_p, [C.c_void_p, C.c_int]),
+    "sz_channel_buffer": (C.c_void_p, [C.c_void_p, C.c_int]),
+    "sz_channel_nonlinear": (C.c_int, [C.c_void_p, C.c_int, C.c_int,
+                                       C.c_void_p]),
+    "sz_channel_accumulate_operator": (C.c_int, [C.c_void_p, C.c_double,
+                                                 C.c_int, C.c_double,
+                                                 C.c_int]),
+    "sz_channel_apply_operator": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
+    "sz_channel_invert_operator": (C.c_int, [C.c_void_p, C.c_double,
+                                             C.c_int]),
+    "sz_channel_check": (C.c_int, [C.c_void_p]),
+    "sz_channel_add_scaled": (C.c_int, [C.c_void_p, C.c_double, C.c_int,
+                                        C.c_int]),
+    "sz_channel_copy": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "sz_channel_exchange": (None, [C.c_void_p]),
+    "sz_channel_step": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p]),
+}
+
+
+def _lib():
+    lib = load()
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype, fn.argtypes = res, args
+    return lib
+
+
+def _hip():
+    return C.CDLL("libamdhip64.so")
+
+
+class Channel:
+    def __init__(self, Nx=96, Ny=96, Nz=96, dNx=None, dNz=None, k=8,
+                 htdelta=3.0, Lx=4 * math.pi, Ly=2.0, Lz=4 * math.pi / 3,
+                 scenario=None, lower=(1.0, 0.0, 0.0, 0.0),
+                 upper=(1.0, 0.0, 0.0, 0.0), solver="zgbsv", stream=None):
+        import torch
+        sc = dict(DEFAULT_SCENARIO, **(scenario or {}))
+        self.scenario = sc
+        dNx = dNx or 3 * Nx // 2
+        dNz = dNz or 3 * Nz // 2
+        self.Nx, self.Ny, self.Nz, self.dNx, self.dNz, self.k = (
+            Nx, Ny, Nz, dNx, dNz, k)
+        self.htdelta, self.Lx, self.Ly, self.Lz = htdelta, Lx, Ly, Lz
+        self.lower, self.upper = lower, upper
+        spec = sz_channel_spec(
+            Nx, Ny, Nz, dNx, dNz, k, htdelta, Lx, Ly, Lz,
+            sz_ns_scenario(*[sc[q] for q in ("alpha", "beta", "gamma", "Ma",
+                                             "Pr", "Re")]),
+            sz_isothermal_bc(1, 1, *lower, *upper), solver.encode(), 1)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        self._stream = stream
+        err = C.create_string_buffer(512)
+        self._h = _lib().sz_channel_create(C.byref(spec), C.c_void_p(stream),
+                                           err, 512)
+        if not self._h:
+            raise ValueError(err.value.decode())
+        Xw, nsys = C.c_int(), C.c_int()
+        F, Fp = C.c_longlong(), C.c_longlong()
+        _lib().sz_channel_geometry(self._h, C.byref(Xw), C.byref(nsys),
+                                   C.byref(F), C.byref(Fp))
+        self.Xw, self.nsys, self.F, self.Fp = Xw.value, nsys.value, F.value, Fp.value
+        self.shape = (5, dNz, self.Xw, Ny)
+
+    # -------------------------------------------------------------- access
+    def _copy(self, dst, src, nbytes, kind):
+        rc = _hip().hipMemcpyAsync(C.c_void_p(dst), C.c_void_p(src),
+                                   C.c_size_t(nbytes), kind,
+                                   C.c_void_p(self._stream))
+        if rc:
+            raise RuntimeError(f"hipMemcpyAsync failed: {rc}")
+        _hip().hipStreamSynchronize(C.c_void_p(self._stream))
+
+    def state_ptr(self, which=0):
+        return _lib().sz_channel_state(self._h, which)
+
+    def set_state(self, t, which=0):
+        """Copy a device tensor (5, dNz, Xw, Ny) complex128 into a state."""
+        import torch
+        t = t.to(dtype=torch.complex128).contiguous()
+        assert tuple(t.shape) == self.shape
+        self._copy(self.state_ptr(which), t.data_ptr(), t.numel() * 16, 3)
+
+    def get_state(self, which=0):
+        import torch
+        out = torch.empty(self.shape, dtype=torch.complex128, device="cuda:0")
+        self._copy(out.data_ptr(), self.state_ptr(which), out.numel() * 16, 3)
+        return out
+
+    def refs(self):
+        """Plane means (42, Ny) of the last zeroth-substep collection."""
+        out = np.empty((42, self.Ny))
+        p = _lib().sz_channel_buffer(self._h, 0)
+        self._copy(out.ctypes.data, p, out.nbytes, 2)
+        return out
+
+    def collocation_points(self):
+        w = _lib().sz_channel_buffer(self._h, 8)
+        f = load().sz_bsplineop_collocation_point
+        return np.array([f(w, i) for i in range(self.Ny)])
+
+    def operators(self):
+        """Host copies (3, Ny, ld) of the transposed bands of M, D1, D2 and
+        (kl, ku, max_kl, max_ku)."""
+        lib = load()
+        w = _lib().sz_channel_buffer(self._h, 8)
+        ld, n = lib.sz_bsplineop_ld(w), self.Ny
+        dt = np.stack([np.ctypeslib.as_array(lib.sz_bsplineop_D_T(w, d),
+                                             shape=(n * ld,)).copy().reshape(n, ld)
+                       for d in range(3)])
+        kl = [lib.sz_bsplineop_kl(w, d) for d in range(3)]
+        ku = [lib.sz_bsplineop_ku(w, d) for d in range(3)]
+        return dt, kl, ku, lib.sz_bsplineop_max_kl(w), lib.sz_bsplineop_max_ku(w)
+
+    def lambdas(self):
+        out = np.empty(2 * self.Ny)
+        p = _lib().sz_channel_buffer(self._h, 7)
+        self._copy(out.ctypes.data, p, out.nbytes, 2)
+        return out[:self.Ny], out[self.Ny:]
+
+    # ----------------------------------------------------------- operators
+    def nonlinear(self, which=1, zeroth=True):
+        cand = np.zeros(12)
+        rc = _lib().sz_channel_nonlinear(self._h, which, int(zeroth),
+                                         cand.ctypes.data if zeroth else None)
+        check(rc, "sz_channel_nonlinear")
+        return cand if zeroth else None
+
+    def accumulate_operator(self, phi, src, beta, dst):
+        check(_lib().sz_channel_accumulate_operator(self._h, phi, src, beta, dst),
+              "sz_channel_accumulate_operator")
+
+    def apply_operator(self, phi, which=0):
+        check(_lib().sz_channel_apply_operator(self._h, phi, which),
+              "sz_channel_apply_operator")
+
+    def invert_operator(self, phi, which=0):
+        check(_lib().sz_channel_invert_operator(self._h, phi, which),
+              "sz_channel_invert_operator")
+        rc = _lib().sz_channel_check(self._h)
+        if rc:
+            raise ArithmeticError(f"implicit solve: zero pivot, info {rc}")
+
+    def add_scaled(self, alpha, src, dst):
+        check(_lib().sz_channel_add_scaled(self._h, alpha, src, dst),
+              "sz_channel_add_scaled")
+
+    def exchange(self):
+        _lib().sz_channel_exchange(self._h)
+
+    def step(self, max_dt=0.0):
+        dt = C.c_double(0)
+        rc = _lib().sz_channel_step(self._h, max_dt, C.byref(dt))
+        if rc > 0:
+            raise ArithmeticError(f"implicit solve: zero pivot, info {rc}")
+        check(rc, "sz_channel_step")
+        return dt.value
+
+    def step_async(self, max_dt):
+        """Enqueue one step with a fixed dt = max_dt without host syncs beyond
+        the candidate read; returns the dt taken."""
+        return self.step(max_dt)
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                _lib().sz_channel_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+def laminar_state(ch, seed=20261202, amp=1e-2, kmax=8):
+    """Isothermal laminar channel (apps/perfect/main_initial.cpp:404-452:
+    T = 1 walls, parabolic streamwise velocity with unit bulk velocity,
+    constant density 1, p = rho T / gamma) plus seeded Fourier perturbations
+    of the momentum of amplitude amp |k|^-2 for 0 < |k| <= kmax with a
+    y(Ly - y) wall-normal shape, as B-spline coefficients on the host;
+    returns a complex128 numpy array shaped ch.shape."""
+    rng = np.random.default_rng(seed)
+    sc = ch.scenario
+    gamma, Ma = sc["gamma"], sc["Ma"]
+    y = ch.collocation_points()
+    Ly = ch.Ly
+    dt, kl, ku, max_kl, max_ku = ch.operators()
+    n = ch.Ny
+    Mdense = np.zeros((n, n))
+    for r in range(n):
+        for c in range(max(0, r - ku[0]), min(n, r + kl[0] + 1)):
+            Mdense[r, c] = dt[0, r, max_ku + c - r]
+    u = 6 / Ly ** 2 * y * (Ly - y)               # bulk velocity 1
+    T = np.ones(n)
+    rho = np.ones(n)
+    p = rho * T / gamma
+    e = p / (gamma - 1) + Ma * Ma / 2 * rho * u * u
+    vals = np.zeros(ch.shape, dtype=complex)
+    vals[0, 0, 0] = e
+    vals[1, 0, 0] = rho * u
+    vals[4, 0, 0] = rho
+    shape = y * (Ly - y)
+    zabs = (ch.Nz - 1) // 2
+    for iz in range(ch.dNz):
+        wz = iz if iz < (ch.dNz + 1) // 2 else iz - ch.dNz
+        if abs(wz) > zabs:
+            continue
+        for ix in range(ch.Xw):
+            if ix > (ch.Nx - 1) // 2:
+                continue
+            kk = math.hypot(ix, wz)
+            if kk == 0 or kk > kmax:
+                continue
+            for f in (1, 2, 3):
+                c = (rng.standard_normal() + 1j * rng.standard_normal()) / math.sqrt(2)
+                vals[f, iz, ix] = amp / kk ** 2 * c * shape
+    # Hermitian symmetry on the kx = 0 plane keeps the fields real
+    for iz in range(1, ch.dNz):
+        jz = ch.dNz - iz
+        if iz < jz:
+            vals[:, jz, 0] = np.conj(vals[:, iz, 0])
+    coeff = np.linalg.solve(Mdense, vals.reshape(-1, n).T).T.reshape(ch.shape)
+    return coeff

@@ -1,0 +1,613 @@
+// Channel solver driver on one GPU: state storage, the explicit operator
+// (derivatives, dealiased transforms, pointwise flux, references, stable step
+// candidates), the hybrid implicit operator (apply, accumulate, invert with
+// isothermal walls) and the SMR91 low-storage step.
+//
+//   nonlinear()   apps/perfect/navier_stokes.hpp:107-986 (rhome_xyz, none)
+//   apply/accumulate_operator()
+//                 apps/perfect/operator_hybrid_isothermal.cpp:103-374
+//   invert_operator()
+//                 apps/perfect/operator_hybrid_isothermal.cpp:528-689
+//   step()        suzerain/lowstorage.hpp:1461-1520 with the smr91 constants
+//                 of lowstorage.hpp:1241-1305 and delta_t_reducer (:593-600)
+//
+// Wave-space state: 5 complex fields (rho E, rho u, rho v, rho w, rho), each
+// (dNz, dNx/2+1, Ny) in C order (y fastest: B-spline coefficients), field
+// stride F.  Only wavenumbers with kx <= (Nx-1)/2 and |kz| <= (Nz-1)/2 are
+// nonzero; each is one implicit system whose five pencils start at
+// field base + f*F + sysoff[s].
+#include "sz_common.hpp"
+#include "../../include/suzerain_amd.h"
+#include "../../include/suzerain_amd_solver.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+using sz::zd;
+constexpr int NT = 256;
+
+inline unsigned nblocks(long long work) { return (unsigned)((work + NT - 1) / NT); }
+
+__global__ __launch_bounds__(NT)
+void k_zero_dealias(zd *w, long long F, int nf, int Ny, int Xw, int dNz,
+                    int xkeep, int zabs)
+{
+    const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+    if (e >= F * nf) return;
+    const long long rem = e % F;
+    const int pencil = (int)(rem / Ny);
+    const int x = pencil % Xw, z = pencil / Xw;
+    const int wz = z < (dNz + 1) / 2 ? z : z - dNz;
+    const bool keep = x < xkeep && (wz < 0 ? -wz : wz) <= zabs;
+    if (!keep) w[e] = zd{0, 0};
+}
+
+__global__ __launch_bounds__(NT)
+void k_axpy(zd *y, const zd *x, double a, long long n)
+{
+    const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+    if (e >= n) return;
+    const zd xv = x[e], yv = y[e];
+    const double px = a * xv.x, py = a * xv.y;
+    y[e] = zd{yv.x + px, yv.y + py};
+}
+
+__global__ __launch_bounds__(NT)
+void k_copy(zd *dst, const zd *src, long long n)
+{
+    const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+    if (e < n) dst[e] = src[e];
+}
+
+__global__ __launch_bounds__(NT)
+void k_fill0(zd *dst, long long n)
+{
+    const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+    if (e < n) dst[e] = zd{0, 0};
+}
+
+// PB[s][i] = state[q(i)]: the permuted right hand side of system s gathered
+// from the five field pencils (bsmbsm.h q(i) = (i mod S) n + i / S).
+__global__ __launch_bounds__(NT)
+void k_gather(const zd *__restrict__ st, long long F,
+              const long long *__restrict__ sysoff, int S, int n,
+              zd *__restrict__ PB, long long strideB, int nsys)
+{
+    const int N = S * n;
+    const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+    if (e >= (long long)N * nsys) return;
+    const int s = (int)(e / N), i = (int)(e - (long long)s * N);
+    const int qi = (i % S) * n + i / S;
+    const int f = qi / n, y = qi - f * n;
+    PB[(size_t)s * strideB + i] = st[f * F + sysoff[s] + y];
+}
+
+// state[a] = PX[s][q^{-1}(a)]: the inverse permutation, writes coalesced.
+__global__ __launch_bounds__(NT)
+void k_scatter(zd *__restrict__ st, long long F,
+               const long long *__restrict__ sysoff, int S, int n,
+               const zd *__restrict__ PX, long long strideX, int nsys)
+{
+    const int N = S * n;
+    const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+    if (e >= (long long)N * nsys) return;
+    const int s = (int)(e / N), a = (int)(e - (long long)s * N);
+    const int f = a / n, y = a - f * n;
+    const int i = y * S + f;
+    st[f * F + sysoff[s] + y] = PX[(size_t)s * strideX + i];
+}
+
+bool ok() { return hipGetLastError() == hipSuccess; }
+
+// Empirical eigenvalue scale fits of suzerain/bspline.c:355-470
+// (suzerain_bspline_htstretch2_evdeltascale_greville_abscissae) for the
+// B-spline orders used here; columns a..i, low error bound (percent).
+struct evfit { double a, b, c, d, e, f, g, h, i, errl; };
+bool evfit_for(int nderiv, int k, evfit &r)
+{
+    if (k == 6 && nderiv == 1)
+        r = {8193. / 7183., 74. / 7487., 10787. / 6960., -12529. / 6209., 3196. / 4771.,
+             4441318. / 12819., 932. / 12135., 3915. / 17923., 870. / 2123., -1.39};
+    else if (k == 6 && nderiv == 2)
+        r = {2635. / 8541., -299. / 15112., 174058. / 24237., -17342. / 13001., 2894. / 10471.,
+             291982. / 9057., 68. / 17203., 5579. / 13080., 6911. / 9355., -1.51};
+    else if (k == 8 && nderiv == 1)
+        r = {20956. / 18111., 73. / 3043., 3316. / 4207., -17945. / 10877., 6659. / 11166.,
+             2564433. / 9169., 658. / 10813., 2725. / 12594., 1181. / 2847., -1.48};
+    else if (k == 8 && nderiv == 2)
+        r = {4751. / 5346., 2143. / 18802., -46259. / 19493., -4506. / 9193., 4. / 6951.,
+             49249. / 1482., -24. / 8359., 959. / 3662., 9327. / 7114., -4.50};
+    else
+        return false;
+    return true;
+}
+
+double evclow(int nderiv, int k, double htdelta, int N)
+{
+    evfit p;
+    if (!evfit_for(nderiv, k, p)) return NAN;
+    const double deltahat = std::tanh(htdelta) * std::pow(1. + htdelta, p.i);
+    double C = std::pow((double)k, p.d + p.e * deltahat);
+    C *= 1.0 + std::pow((p.f * k) / ((double)N - k + 1.0), p.g * k + p.h * deltahat);
+    C += p.a * k;
+    C += p.b * deltahat;
+    C += p.c / std::sqrt((double)k);
+    return C / (1 - p.errl / 100);
+}
+
+// SMR91 (lowstorage.hpp:1241-1305): numerators over 480.
+constexpr double smr91_alpha[3] = {145. / 480, -36. / 480, 80. / 480};
+constexpr double smr91_beta[3] = {111. / 480, 100. / 480, 80. / 480};
+constexpr double smr91_gamma[3] = {256. / 480, 200. / 480, 360. / 480};
+constexpr double smr91_zeta[3] = {0. / 480, -136. / 480, -200. / 480};
+constexpr double smr91_evmaxmag_real = 2.51274532661832862402373;
+
+}  // namespace
+
+struct sz_channel {
+    sz_channel_spec spec{};
+    std::string solver_spec;
+    hipStream_t stream = nullptr;
+    int Ny = 0, Xw = 0, xkeep = 0, zabs = 0, nsys = 0;
+    long long F = 0, Fp = 0;   // wave and physical field strides
+    sz_bsplineop *bop = nullptr;
+    sz_fft_plan *fft = nullptr;
+    sz_imexop *imx = nullptr;
+    suzerain_amd::bsmbsm_solver *solver = nullptr;
+    zd *buf[2] = {nullptr, nullptr};   // state a = buf[ia], b = buf[1-ia]
+    int ia = 0;
+    zd *vals = nullptr, *auxw = nullptr, *mixed = nullptr;
+    double *phys = nullptr, *refs = nullptr, *work = nullptr, *dbuf = nullptr;
+    double *km = nullptr, *kn = nullptr;
+    long long *sysoff = nullptr;
+    sz_imexop_ref iref{};
+    sz_imexop_refld ild{};
+    std::vector<double> lambda1_y, lambda2_y;
+    double lxz[4] = {0, 0, 0, 0};
+    std::vector<void *> owned;
+
+    template <typename T>
+    T *dalloc(size_t count)
+    {
+        void *p = nullptr;
+        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) throw std::bad_alloc();
+        owned.push_back(p);
+        return static_cast<T *>(p);
+    }
+    ~sz_channel()
+    {
+        for (void *p : owned) hipFree(p);
+        delete solver;
+        sz_imexop_free(imx);
+        sz_fft_plan_free(fft);
+        sz_bsplineop_free(bop);
+    }
+    zd *a() { return buf[ia]; }
+    zd *b() { return buf[1 - ia]; }
+    zd *state(int which) { return which == 0 ? a() : b(); }
+    int zero_dealias(zd *w, int nf)
+    {
+        hipLaunchKernelGGL(k_zero_dealias, dim3(nblocks(F * nf)), dim3(NT), 0,
+                           stream, w, F, nf, Ny, Xw, spec.dNz, xkeep, zabs);
+        return ok() ? 0 : SZ_ERR_LAUNCH;
+    }
+};
+
+namespace {
+
+int set_err(char *err, int errlen, const std::string &m)
+{
+    if (err && errlen > 0) std::snprintf(err, (size_t)errlen, "%s", m.c_str());
+    return -1;
+}
+
+const double TWO_PI = 6.283185307179586476925286766559005768394338798750211642;
+
+}  // namespace
+
+extern "C" {
+
+sz_channel *sz_channel_create(const sz_channel_spec *spec, void *stream,
+                              char *err, int errlen)
+{
+    if (!spec) { set_err(err, errlen, "null spec"); return nullptr; }
+    const sz_channel_spec &sp = *spec;
+    if (sp.Nx < 1 || sp.Nz < 1 || sp.dNx < sp.Nx || sp.dNz < sp.Nz ||
+        sp.Ny < sp.k || sp.k < 2) {
+        set_err(err, errlen, "inconsistent grid extents");
+        return nullptr;
+    }
+    if (sp.htdelta < 0) {
+        set_err(err, errlen, "one-sided (htdelta < 0) grids are not provided");
+        return nullptr;
+    }
+    if (!(sp.k == 6 || sp.k == 8)) {
+        set_err(err, errlen, "B-spline order must be 6 or 8 (eigenvalue fits)");
+        return nullptr;
+    }
+    if (sp.linearize != SZ_LINEARIZE_RHOME_XYZ) {
+        set_err(err, errlen, "only the rhome_xyz linearization is provided");
+        return nullptr;
+    }
+    sz_channel *h = new sz_channel();
+    try {
+        h->spec = sp;
+        h->solver_spec = sp.solver ? sp.solver : "zgbsv";
+        h->spec.solver = nullptr;
+        h->stream = (hipStream_t)stream;
+        h->Ny = sp.Ny;
+        h->Xw = sp.dNx / 2 + 1;
+        h->xkeep = (sp.Nx - 1) / 2 + 1;
+        h->zabs = (sp.Nz - 1) / 2;
+        h->F = (long long)sp.dNz * h->Xw * sp.Ny;
+        h->Fp = (long long)sp.Ny * sp.dNz * sp.dNx;
+
+        // B-spline operators (support.cpp:245-311) on [0, Ly]
+        const int nbreak = sp.Ny - sp.k + 2;
+        std::vector<double> bp((size_t)nbreak);
+        if (sz_bspline_breakpoints(sp.Ny, sp.k, 0.0, sp.Ly, sp.htdelta, bp.data())
+                != nbreak)
+            throw std::runtime_error("breakpoint construction failed");
+        int info = 0;
+        h->bop = sz_bsplineop_alloc(sp.k, nbreak, bp.data(), 2, &info);
+        if (!h->bop) throw std::runtime_error("sz_bsplineop_alloc failed");
+        h->fft = sz_fft_plan_create(sp.Ny, sp.Nx, sp.dNx, sp.Nz, sp.dNz, &info);
+        if (!h->fft) throw std::runtime_error("sz_fft_plan_create failed");
+        sz_imexop_scenario is{sp.scenario.Re, sp.scenario.Pr, sp.scenario.Ma,
+                              sp.scenario.alpha, sp.scenario.gamma};
+        h->imx = sz_imexop_create(&is, h->bop, &info);
+        if (!h->imx) throw std::runtime_error("sz_imexop_create failed");
+
+        // retained wavenumbers = implicit systems
+        std::vector<double> km, kn;
+        std::vector<long long> off;
+        const double tpx = TWO_PI / sp.Lx, tpz = TWO_PI / sp.Lz;
+        for (int z = 0; z < sp.dNz; ++z) {
+            const int wz = z < (sp.dNz + 1) / 2 ? z : z - sp.dNz;
+            if ((wz < 0 ? -wz : wz) > h->zabs) continue;
+            for (int x = 0; x < h->xkeep; ++x) {
+                km.push_back(tpx * x);
+                kn.push_back(tpz * wz);
+                off.push_back(((long long)z * h->Xw + x) * sp.Ny);
+            }
+        }
+        h->nsys = (int)km.size();
+        h->km = h->dalloc<double>(km.size());
+        h->kn = h->dalloc<double>(kn.size());
+        h->sysoff = h->dalloc<long long>(off.size());
+        hipMemcpy(h->km, km.data(), km.size() * 8, hipMemcpyHostToDevice);
+        hipMemcpy(h->kn, kn.data(), kn.size() * 8, hipMemcpyHostToDevice);
+        hipMemcpy(h->sysoff, off.data(), off.size() * 8, hipMemcpyHostToDevice);
+
+        const sz_bsmbsm g = sz_bsmbsm_construct(5, sp.Ny, sz_bsplineop_max_kl(h->bop),
+                                                sz_bsplineop_max_ku(h->bop));
+        h->solver = suzerain_amd::bsmbsm_solver::build(
+            g, suzerain_amd::specification_zgbsv(h->solver_spec), 1, h->nsys,
+            stream);
+
+        // device storage
+        h->buf[0] = h->dalloc<zd>((size_t)h->F * 5);
+        h->buf[1] = h->dalloc<zd>((size_t)h->F * 5);
+        h->vals = h->dalloc<zd>((size_t)h->F * 5);
+        h->auxw = h->dalloc<zd>((size_t)h->F * 41);
+        h->phys = h->dalloc<double>((size_t)h->Fp * 46);
+        h->mixed = h->dalloc<zd>((size_t)sz_fft_mixed_elements(h->fft) * 46);
+        h->refs = h->dalloc<double>((size_t)SZ_NS_NREF * sp.Ny);
+        h->work = h->dalloc<double>(
+            (size_t)(sz_ns_workspace_bytes(sp.Ny, sp.dNz, sp.dNx) / 8 + 1));
+        h->dbuf = h->dalloc<double>(12 + 4 + 2 * (size_t)sp.Ny);
+        hipMemsetAsync(h->buf[0], 0, (size_t)h->F * 5 * sizeof(zd), h->stream);
+        hipMemsetAsync(h->buf[1], 0, (size_t)h->F * 5 * sizeof(zd), h->stream);
+        hipMemsetAsync(h->refs, 0, (size_t)SZ_NS_NREF * sp.Ny * 8, h->stream);
+
+        // imexop views of the reference rows (references.cpp:51-108)
+        const int qmap[26] = {5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17,
+                              18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30};
+        const double **rp = reinterpret_cast<const double **>(&h->iref);
+        int *lp = reinterpret_cast<int *>(&h->ild);
+        for (int t = 0; t < 26; ++t) {
+            rp[t] = h->refs + (size_t)qmap[t] * sp.Ny;
+            lp[t] = 1;
+        }
+
+        // stable step scales (operator_base.cpp:36-88)
+        const double pi = 3.1415926535897932384626433832795028841971693993751;
+        const double C1low = std::pow(evclow(1, sp.k, sp.htdelta, sp.Ny), 33 / 64.);
+        const double C2low = std::pow(evclow(2, sp.k, sp.htdelta, sp.Ny), 27 / 64.);
+        h->lambda1_y.resize((size_t)sp.Ny);
+        h->lambda2_y.resize((size_t)sp.Ny);
+        for (int j = 0; j < sp.Ny; ++j) {
+            const int jm = j == 0 ? 1 : j - 1, jp = j == sp.Ny - 1 ? sp.Ny - 2 : j + 1;
+            const double x = sz_bsplineop_collocation_point(h->bop, j);
+            const double xm = sz_bsplineop_collocation_point(h->bop, jm);
+            const double xp = sz_bsplineop_collocation_point(h->bop, jp);
+            const double dxm = std::fabs(x - xm), dxp = std::fabs(xp - x);
+            const double inv = 1 / (dxm < dxp ? dxm : dxp);
+            h->lambda1_y[j] = pi * inv / C1low;
+            const double l2 = pi * inv / C2low;
+            h->lambda2_y[j] = l2 * l2;
+        }
+        h->lxz[0] = pi * (sp.Nx / sp.Lx);
+        h->lxz[1] = pi * (sp.Nz / sp.Lz);
+        h->lxz[2] = smr91_evmaxmag_real;
+        h->lxz[3] = std::sqrt(3.0);
+        std::vector<double> hb(12 + 4 + 2 * (size_t)sp.Ny, 0.0);
+        std::memcpy(hb.data() + 12, h->lxz, 4 * sizeof(double));
+        std::memcpy(hb.data() + 16, h->lambda1_y.data(), sp.Ny * sizeof(double));
+        std::memcpy(hb.data() + 16 + sp.Ny, h->lambda2_y.data(), sp.Ny * sizeof(double));
+        hipMemcpy(h->dbuf, hb.data(), hb.size() * 8, hipMemcpyHostToDevice);
+        if (hipStreamSynchronize(h->stream) != hipSuccess)
+            throw std::runtime_error("device failure during setup");
+    } catch (const std::exception &ex) {
+        set_err(err, errlen, std::string("sz_channel_create: ") + ex.what());
+        delete h;
+        return nullptr;
+    }
+    return h;
+}
+
+void sz_channel_free(sz_channel *h) { delete h; }
+
+int sz_channel_geometry(const sz_channel *h, int *Xw, int *nsys,
+                        long long *field_stride, long long *phys_stride)
+{
+    if (!h) return -1;
+    if (Xw) *Xw = h->Xw;
+    if (nsys) *nsys = h->nsys;
+    if (field_stride) *field_stride = h->F;
+    if (phys_stride) *phys_stride = h->Fp;
+    return 0;
+}
+
+void *sz_channel_state(sz_channel *h, int which)
+{
+    return h && (which == 0 || which == 1) ? (void *)h->state(which) : nullptr;
+}
+
+void *sz_channel_buffer(sz_channel *h, int which)
+{
+    if (!h) return nullptr;
+    switch (which) {
+    case 0: return h->refs;
+    case 1: return h->phys;
+    case 2: return h->auxw;
+    case 3: return h->vals;
+    case 4: return h->km;
+    case 5: return h->kn;
+    case 6: return h->sysoff;
+    case 7: return h->dbuf + 16;              // lambda1_y then lambda2_y
+    case 8: return h->bop;
+    case 9: return h->solver;
+    default: return nullptr;
+    }
+}
+
+// N(u) in place on state `which` (navier_stokes.hpp:107-986).
+int sz_channel_nonlinear(sz_channel *h, int which, int zeroth,
+                         double *candidates)
+{
+    if (!h) return -1;
+    if (which != 0 && which != 1) return -2;
+    const sz_channel_spec &sp = h->spec;
+    zd *s = h->state(which);
+    const long long F = h->F;
+    hipStream_t st = h->stream;
+    int rc = h->zero_dealias(s, 5);
+    if (rc) return rc;
+    // wave-space derivatives (navier_stokes.hpp:225-311)
+    auto A = [&](int idx) { return h->auxw + (size_t)idx * F; };
+    const double one[2] = {1, 0}, zero[2] = {0, 0};
+    const int npen = sp.dNz * h->Xw;
+    auto bop = [&](int d, const zd *x, zd *y) {
+        return sz_bsplineop_accumulate_complex_batched(h->bop, d, npen, one, x, 1,
+                                                       sp.Ny, zero, y, 1, sp.Ny, st);
+    };
+    auto dw = [&](int dx, int dz, const zd *x, double beta, zd *y) {
+        const double b2[2] = {beta, 0};
+        return sz_diffwave_accumulate(dx, dz, one, x, b2, y, sp.Lx, sp.Lz, sp.Ny,
+                                      sp.Nx, sp.dNx, 0, h->Xw, sp.Nz, sp.dNz, 0,
+                                      sp.dNz, st);
+    };
+    // aux indices: e 0..4, then mx 5.., my 14.., mz 23.., rho 32..
+    for (int f = 0; f < 5 && !rc; ++f) {
+        const zd *c = s + (size_t)f * F;
+        zd *v = h->vals + (size_t)f * F;
+        if (f == 0) {
+            rc = bop(1, c, A(0));
+            if (!rc) rc = bop(2, c, A(1));
+            if (!rc) rc = bop(2, c, A(2));
+            if (!rc) rc = bop(0, c, v);
+            if (!rc) rc = dw(1, 0, v, 0, A(3));
+            if (!rc) rc = dw(2, 0, v, 1, A(2));
+            if (!rc) rc = dw(0, 1, v, 0, A(4));
+            if (!rc) rc = dw(0, 2, v, 1, A(2));
+        } else {
+            const int b0 = 5 + 9 * (f - 1);  // _y _yy _x _xx _xz _z _zz _xy _yz
+            rc = bop(1, c, A(b0 + 0));
+            if (!rc) rc = bop(2, c, A(b0 + 1));
+            if (!rc) rc = bop(0, c, v);
+            if (!rc) rc = dw(1, 0, v, 0, A(b0 + 2));
+            if (!rc) rc = dw(2, 0, v, 0, A(b0 + 3));
+            if (!rc) rc = dw(1, 1, v, 0, A(b0 + 4));
+            if (!rc) rc = dw(0, 1, v, 0, A(b0 + 5));
+            if (!rc) rc = dw(0, 2, v, 0, A(b0 + 6));
+            if (!rc) rc = dw(1, 0, A(b0 + 0), 0, A(b0 + 7));
+            if (!rc) rc = dw(0, 1, A(b0 + 0), 0, A(b0 + 8));
+        }
+    }
+    if (rc) return rc;
+    // to physical space: 5 state values then 41 derivatives
+    const long long smix = sz_fft_mixed_elements(h->fft);
+    rc = sz_fft_wave_to_physical(h->fft, 5, h->vals, F, h->mixed, smix, h->phys,
+                                 h->Fp, st);
+    if (!rc)
+        rc = sz_fft_wave_to_physical(h->fft, 41, h->auxw, F, h->mixed + 5 * smix,
+                                     smix, h->phys + 5 * h->Fp, h->Fp, st);
+    if (rc) return rc;
+    if (zeroth) {
+        rc = sz_ns_references(&sp.scenario, sp.Ny, sp.dNz, sp.dNx, h->phys, h->Fp,
+                              h->refs, h->work, st);
+        if (rc) return rc;
+    }
+    double *cand = h->dbuf;
+    rc = sz_ns_rhs(&sp.scenario, SZ_LINEARIZE_RHOME_XYZ, zeroth, sp.Ny, sp.dNz,
+                   sp.dNx, h->phys, h->Fp, h->phys + 5 * h->Fp, h->Fp, h->phys,
+                   h->Fp, h->refs, h->dbuf + 16, h->dbuf + 16 + sp.Ny,
+                   h->dbuf + 12, cand, h->work, st);
+    if (rc) return rc;
+    // back to wave space; rhome_xyz has no explicit density equation
+    rc = sz_fft_physical_to_wave(h->fft, 4, h->phys, h->Fp, h->mixed, smix, s, F, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_fill0, dim3(nblocks(F)), dim3(NT), 0, st,
+                       s + 4 * (size_t)F, F);
+    if (!ok()) return SZ_ERR_LAUNCH;
+    if (zeroth && candidates) {
+        if (hipMemcpyAsync(candidates, cand, 12 * sizeof(double),
+                           hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return SZ_ERR_LAUNCH;
+    }
+    return 0;
+}
+
+// out <- (M + phi L) in + beta out on every retained wavenumber
+// (operator_hybrid_isothermal.cpp:103-374); in == out allowed with beta = 0.
+int sz_channel_accumulate_operator(sz_channel *h, double phi, int in,
+                                   double beta, int out)
+{
+    if (!h) return -1;
+    if (in != 0 && in != 1) return -3;
+    if (out != 0 && out != 1) return -5;
+    const zd *x = h->state(in);
+    zd *y = h->state(out);
+    const void *xs[5];
+    void *ys[5];
+    for (int f = 0; f < 5; ++f) {
+        xs[f] = x + (size_t)f * h->F;
+        ys[f] = y + (size_t)f * h->F;
+    }
+    const double p[2] = {phi, 0}, b[2] = {beta, 0};
+    return sz_rholut_imexop_accumulate_batched(h->imx, p, h->km, h->kn, &h->iref,
+                                               &h->ild, xs, h->sysoff, 0, b, ys,
+                                               h->sysoff, 0, h->nsys, h->stream);
+}
+
+int sz_channel_apply_operator(sz_channel *h, double phi, int which)
+{
+    return sz_channel_accumulate_operator(h, phi, which, 0.0, which);
+}
+
+// Solve (M + phi L) x = state with isothermal walls for every retained
+// wavenumber, in place; other wavenumbers are zeroed
+// (operator_hybrid_isothermal.cpp:528-689).  Asynchronous: per-system info
+// stays in the solver's info array (checked by sz_channel_check).
+int sz_channel_invert_operator(sz_channel *h, double phi, int which)
+{
+    if (!h) return -1;
+    if (which != 0 && which != 1) return -3;
+    suzerain_amd::bsmbsm_solver &sv = *h->solver;
+    zd *s = h->state(which);
+    const double p[2] = {phi, 0};
+    int rc = sz_rholut_imexop_packc_batched(
+        h->imx, p, h->km, h->kn, &h->iref, &h->ild, 0, 1, 2, 3, 4, nullptr,
+        sv.PAPT, sv.ld_PAPT, sv.stride_PAPT, h->nsys, &h->spec.bc, h->stream);
+    if (rc) return rc;
+    sv.supplied_PAPT();
+    const int N = sv.N;
+    hipLaunchKernelGGL(k_gather, dim3(nblocks((long long)N * h->nsys)), dim3(NT), 0,
+                       h->stream, s, h->F, h->sysoff, 5, h->Ny,
+                       reinterpret_cast<zd *>(sv.PB), sv.stride_PB, h->nsys);
+    if (!ok()) return SZ_ERR_LAUNCH;
+    rc = sz_isothermal_rhs_batched(5, h->Ny, 0, 1, 2, 3, h->spec.bc.enforce_lower,
+                                   h->spec.bc.enforce_upper, sv.PB, sv.stride_PB,
+                                   h->nsys, h->stream);
+    if (rc) return rc;
+    rc = sv.solve_async('T', 1);
+    if (rc) return rc;
+    rc = h->zero_dealias(s, 5);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_scatter, dim3(nblocks((long long)N * h->nsys)), dim3(NT), 0,
+                       h->stream, s, h->F, h->sysoff, 5, h->Ny,
+                       reinterpret_cast<const zd *>(sv.PX), sv.stride_PB, h->nsys);
+    return ok() ? 0 : SZ_ERR_LAUNCH;
+}
+
+// Wait for the stream and report the first system whose factorization met
+// an exact zero pivot (its info), or 0.
+int sz_channel_check(sz_channel *h)
+{
+    if (!h) return -1;
+    std::vector<int> info((size_t)h->nsys);
+    if (hipMemcpyAsync(info.data(), h->solver->info, sizeof(int) * h->nsys,
+                       hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+        return SZ_ERR_LAUNCH;
+    for (int s = 0; s < h->nsys; ++s)
+        if (info[(size_t)s]) return info[(size_t)s];
+    return 0;
+}
+
+int sz_channel_add_scaled(sz_channel *h, double alpha, int src, int dst)
+{
+    if (!h) return -1;
+    const long long n = h->F * 5;
+    hipLaunchKernelGGL(k_axpy, dim3(nblocks(n)), dim3(NT), 0, h->stream,
+                       h->state(dst), h->state(src), alpha, n);
+    return ok() ? 0 : SZ_ERR_LAUNCH;
+}
+
+int sz_channel_copy(sz_channel *h, int src, int dst)
+{
+    if (!h) return -1;
+    const long long n = h->F * 5;
+    hipLaunchKernelGGL(k_copy, dim3(nblocks(n)), dim3(NT), 0, h->stream,
+                       h->state(dst), h->state(src), n);
+    return ok() ? 0 : SZ_ERR_LAUNCH;
+}
+
+void sz_channel_exchange(sz_channel *h)
+{
+    if (h) h->ia = 1 - h->ia;
+}
+
+// One SMR91 step of lowstorage.hpp:1461-1520 advancing state a; b is scratch.
+// The step is the minimum over the 12 candidates (NaN counts as minimum),
+// capped by max_dt > 0.  Returns 0, a solver info (> 0), or an error.
+int sz_channel_step(sz_channel *h, double max_dt, double *dt_out)
+{
+    if (!h) return -1;
+    const double chi = 1.0 / ((double)h->spec.dNx * h->spec.dNz);
+    double cand[12];
+    int rc = sz_channel_copy(h, 0, 1);
+    if (!rc) rc = sz_channel_nonlinear(h, 1, 1, cand);
+    if (rc) return rc;
+    double dt = cand[0];
+    for (int k = 1; k < 12; ++k) {
+        if (dt != dt) break;
+        if (cand[k] != cand[k] || cand[k] < dt) dt = cand[k];
+    }
+    if (max_dt > 0) dt = (dt != dt) ? dt : (max_dt < dt ? max_dt : dt);
+    if (dt_out) *dt_out = dt;
+    if (!(dt > 0)) return -2;
+    rc = sz_channel_apply_operator(h, dt * smr91_alpha[0], 0);
+    if (!rc) rc = sz_channel_add_scaled(h, chi * dt * smr91_gamma[0], 1, 0);
+    if (!rc) rc = sz_channel_invert_operator(h, -dt * smr91_beta[0], 0);
+    for (int i = 1; i < 3 && !rc; ++i) {
+        rc = sz_channel_accumulate_operator(h, dt * smr91_alpha[i], 0,
+                                            chi * dt * smr91_zeta[i], 1);
+        if (rc) break;
+        sz_channel_exchange(h);
+        rc = sz_channel_nonlinear(h, 1, 0, nullptr);
+        if (!rc) rc = sz_channel_add_scaled(h, chi * dt * smr91_gamma[i], 1, 0);
+        if (!rc) rc = sz_channel_invert_operator(h, -dt * smr91_beta[i], 0);
+    }
+    if (rc) return rc;
+    return sz_channel_check(h);
+}
+
+}  // extern "C"

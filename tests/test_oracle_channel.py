@@ -117,3 +117,39 @@ def test_dt_candidates(grid):
     assert c[0] <= min(c[3], c[6], c[9]) + 1e-15
     _, _, c2 = channel.apply_ns(grid, 2 * W, SCEN, "rhome_xyz", lam=lam)
     assert c2[0] > 0
+
+
+def test_smr91_step_keeps_rest_state(grid):
+    """A uniform state at rest with T = 1 and isothermal T = 1 walls is a
+    steady solution; one SMR91 step must leave it unchanged."""
+    g = grid
+    gamma = SCEN["gamma"]
+    M = g.ops.D[0]
+    ones = np.linalg.solve(M, np.ones(g.Ny))
+    W = np.zeros((5, g.dNz, g.Xw, g.Ny), dtype=complex)
+    W[0, 0, 0] = ones / (gamma * (gamma - 1))       # rho E at rho = T = 1
+    W[4, 0, 0] = ones
+    y = np.linspace(0, 2, g.Ny)
+    lam = channel.lambdas(g, y, 6, 2.0, 1.0, 1.0)
+    bc = ((1.0, 0, 0, 0), (1.0, 0, 0, 0))
+    a, dt, _ = channel.smr91_step(g, W, SCEN, bc, lam, max_dt=1e-3)
+    assert dt == 1e-3
+    np.testing.assert_allclose(a, W, rtol=0, atol=1e-13)
+
+
+def test_smr91_step_advances_perturbed_state(grid):
+    g = grid
+    W = random_state(g)
+    y = np.linspace(0, 2, g.Ny)
+    lam = channel.lambdas(g, y, 6, 2.0, 1.0, 1.0)
+    bc = ((1.0, 0, 0, 0), (1.0, 0, 0, 0))
+    a, dt, refs = channel.smr91_step(g, W, SCEN, bc, lam, max_dt=1e-3)
+    assert np.isfinite(a).all()
+    # every wavenumber meets the isothermal wall constraints afterwards:
+    # rho E = E_factor rho and m = 0 at both walls (end coefficients)
+    gamma = SCEN["gamma"]
+    Ef = 1.0 / (gamma * (gamma - 1))
+    scale = np.abs(a).max()
+    for y in (0, g.Ny - 1):
+        assert np.abs(a[0, :, :, y] - Ef * a[4, :, :, y]).max() < 1e-12 * scale
+        assert np.abs(a[1:4, :, :, y]).max() < 1e-12 * scale
