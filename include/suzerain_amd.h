@@ -389,6 +389,11 @@ int sz_channel_check(sz_channel *h);
 int sz_channel_add_scaled(sz_channel *h, double alpha, int src, int dst);
 int sz_channel_copy(sz_channel *h, int src, int dst);
 void sz_channel_exchange(sz_channel *h);
+/* Return the summed device time of the batched factorization launches
+ * bracketed since the last call (and their count), then (enable >= 0)
+ * switch the bracketing on or off.  HIP events on the solver's stream. */
+int sz_channel_factor_timing(sz_channel *h, int enable, double *total_ms,
+                             int *count);
 /* lowstorage::step (lowstorage.hpp:1461-1520) with SMR91 and the
  * delta_t_reducer; dt_out receives the step taken. */
 int sz_channel_step(sz_channel *h, double max_dt, double *dt_out);

@@ -116,6 +116,12 @@ public:
     virtual std::vector<std::string> summarize_statistics() const;
     const std::string &last_error() const { return last_error_; }
 
+    // When set, each batched factorization launch is bracketed by HIP events
+    // on the solver's stream; factor_time_ms waits for them and returns the
+    // summed milliseconds (and launch count), then forgets them.
+    bool time_factor = false;
+    double factor_time_ms(int *count);
+
 protected:
     bsmbsm_solver(const sz_bsmbsm &bsmbsm, const specification_zgbsv &spec,
                   int nrhs, int nbatch, void *stream);
@@ -123,6 +129,8 @@ protected:
     char fact_;
     int apprx_;
     std::string last_error_;
+    std::vector<void *> factor_events_;   // hipEvent_t pairs
+    void factor_mark();                    // record one event
 };
 
 class bsmbsm_solver_zgbsv : public bsmbsm_solver {

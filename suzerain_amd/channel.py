@@ -60,6 +60,8 @@ _SIGS = {
     "sz_channel_copy": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "sz_channel_exchange": (None, [C.c_void_p]),
     "sz_channel_step": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p]),
+    "sz_channel_factor_timing": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p,
+                                           C.c_void_p]),
 }
 
 
@@ -203,10 +205,12 @@ class Channel:
         check(rc, "sz_channel_step")
         return dt.value
 
-    def step_async(self, max_dt):
-        """Enqueue one step with a fixed dt = max_dt without host syncs beyond
-        the candidate read; returns the dt taken."""
-        return self.step(max_dt)
+    def factor_timing(self, enable=-1):
+        """(summed ms, count) of the factorization launches bracketed by HIP
+        events since the last call; enable 1/0 switches bracketing on/off."""
+        ms, n = C.c_double(0), C.c_int(0)
+        _lib().sz_channel_factor_timing(self._h, enable, C.byref(ms), C.byref(n))
+        return ms.value, n.value
 
     def __del__(self):
         try:

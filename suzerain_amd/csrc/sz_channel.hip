@@ -570,6 +570,18 @@ int sz_channel_copy(sz_channel *h, int src, int dst)
     return ok() ? 0 : SZ_ERR_LAUNCH;
 }
 
+int sz_channel_factor_timing(sz_channel *h, int enable, double *total_ms,
+                             int *count)
+{
+    if (!h) return -1;
+    int n = 0;
+    const double ms = h->solver->factor_time_ms(&n);
+    if (total_ms) *total_ms = ms;
+    if (count) *count = n;
+    if (enable >= 0) h->solver->time_factor = enable != 0;
+    return 0;
+}
+
 void sz_channel_exchange(sz_channel *h)
 {
     if (h) h->ia = 1 - h->ia;

@@ -775,6 +775,273 @@ bool try_reg(int n, int kl, int ku, void *ab, int ldab, long long sab,
     return true;
 }
 
+// ---------------------------------------------------------------- w64
+// One wavefront per system, no workgroup barriers.  Window column j+c
+// (0 <= c < 64) lives in the registers of lane (j+c) mod 64 as kl+1 row
+// slots (row i in slot i mod (kl+1)), kept in short ext_vector chunks so a
+// uniform slot index becomes one gpr-indexed move; columns j+64 .. j+kl+ku
+// (only when kl+ku >= 64) live in a small LDS tail indexed by column mod
+// the tail width.  Per step the pivot column is transposed through LDS so the
+// IZAMAX search and the scaling run across lanes, the multipliers are
+// broadcast from LDS, and each lane applies the rank-1 update to its own
+// column with the same fnms per element, in the same order, as the LDS
+// kernel above and the unblocked LAPACK loop -- results are bit-identical.
+
+#define WAVE_SYNC()                                                           \
+    do {                                                                      \
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");                \
+        __builtin_amdgcn_wave_barrier();                                      \
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");                \
+    } while (0)
+
+// The window column of one lane: 40 row slots in five 8-wide chunks per
+// component, held as separate named vectors (arrays of vectors are demoted to
+// scratch).  A uniform slot index selects a chunk with independent uniform
+// branches and the element with a gpr-indexed move.
+#define W_GET(S, V)                                                           \
+    do {                                                                      \
+        const int q_ = (S) >> 3, e_ = (S) & 7;                                \
+        if (q_ == 0) { V.x = r0[e_]; V.y = i0[e_]; }                          \
+        if (q_ == 1) { V.x = r1[e_]; V.y = i1[e_]; }                          \
+        if (q_ == 2) { V.x = r2[e_]; V.y = i2[e_]; }                          \
+        if (q_ == 3) { V.x = r3[e_]; V.y = i3[e_]; }                          \
+        if (q_ == 4) { V.x = r4[e_]; V.y = i4[e_]; }                          \
+    } while (0)
+#define W_SET(S, V)                                                           \
+    do {                                                                      \
+        const int q_ = (S) >> 3, e_ = (S) & 7;                                \
+        if (q_ == 0) { r0[e_] = V.x; i0[e_] = V.y; }                          \
+        if (q_ == 1) { r1[e_] = V.x; i1[e_] = V.y; }                          \
+        if (q_ == 2) { r2[e_] = V.x; i2[e_] = V.y; }                          \
+        if (q_ == 3) { r3[e_] = V.x; i3[e_] = V.y; }                          \
+        if (q_ == 4) { r4[e_] = V.x; i4[e_] = V.y; }                          \
+    } while (0)
+
+constexpr int W64_PF = 4;   // entering rows prefetched ahead of their step
+
+template <typename R, int WR>
+__global__ __launch_bounds__(64)
+void k_gbtrf_w64(int n, int ku, cpx<R> *__restrict__ ab, int ldab,
+                 long long sab, int *__restrict__ ipiv, long long sipiv,
+                 int *__restrict__ info, const int *__restrict__ skip,
+                 int nsys)
+{
+    using T = cpx<R>;
+    constexpr int kl = WR - 1;
+    static_assert(WR <= 40, "five 8-slot chunks per component");
+    __shared__ T colbuf[64];
+    __shared__ T lbuf[64];
+    extern __shared__ __align__(16) unsigned char smem[];
+    T *tail = reinterpret_cast<T *>(smem);
+
+    const int sys = blockIdx.x;
+    if (sys >= nsys) return;
+    if (skip && skip[sys] != 0) return;
+    ab += (size_t)sys * (size_t)sab;
+    ipiv += (size_t)sys * (size_t)sipiv;
+    const int lane = threadIdx.x;
+    const int kv = kl + ku;
+    const int ntail = kv >= 64 ? kv - 63 : 0;
+    auto tix = [&](int k) { return (k % (ntail > 0 ? ntail : 1)) * WR; };
+    auto band = [&](int i, int k) -> T {   // A(i, k) on entry, 0 outside
+        // unconditional load from a clamped (always valid) address, then a
+        // select, so outstanding-load counting stays exact
+        const bool ok = i < n && k < n && k - i <= ku && i - k <= kl;
+        const T *p = ab + (ok ? (size_t)k * ldab + kv + i - k : 0);
+        const R x = p->x, y = p->y;
+        return T{ok ? x : R(0), ok ? y : R(0)};
+    };
+
+    // initial window: rows 0..kl, columns 0..kv
+    typedef R v8 __attribute__((ext_vector_type(8)));
+    v8 r0 = R(0), r1 = R(0), r2 = R(0), r3 = R(0), r4 = R(0);
+    v8 i0 = R(0), i1 = R(0), i2 = R(0), i3 = R(0), i4 = R(0);
+    int colk = lane;
+#pragma unroll
+    for (int s = 0; s < WR; ++s) {
+        const T v = band(s, colk);
+        W_SET(s, v);
+    }
+    for (int e = lane; e < ntail * WR; e += 64) {
+        const int t = e / WR, s = e - t * WR;
+        const int k = 64 + t;
+        tail[tix(k) + s] = band(s, k);
+    }
+    WAVE_SYNC();
+
+    // prefetch queue of entering-row elements: step j brings row j+WR into
+    // slot j mod WR of the lane's column at that time (and tail columns)
+    T q[W64_PF], qt[W64_PF];
+    auto lane_col_at = [&](int j) {      // lane's column during step j+1
+        const int b = j + 1;
+        return b + (((lane - b) % 64) + 64) % 64;
+    };
+#pragma unroll
+    for (int u = 0; u < W64_PF; ++u) {
+        const int i = u + WR;
+        q[u] = band(i, lane_col_at(u));
+        qt[u] = lane < ntail ? band(i, u + 65 + lane) : T{R(0), R(0)};
+    }
+
+    int ju = 0, first_zero = 0, j0 = 0;
+    for (int jb = 0; jb < n; jb += W64_PF) {
+#pragma unroll
+        for (int u = 0; u < W64_PF; ++u) {
+            const int j = jb + u;
+            if (j < n) {
+            const int km = min(kl, n - 1 - j);
+            const int pl = j & 63;
+            // ---- pivot column to LDS, by logical row
+            if (lane == pl) {
+#pragma unroll
+                for (int s = 0; s < WR; ++s) {
+                    int r = s - j0;
+                    if (r < 0) r += WR;
+                    T v;
+                    W_GET(s, v);
+                    colbuf[r] = v;
+                }
+            }
+            WAVE_SYNC();
+            T v = lane <= km ? colbuf[lane] : T{R(0), R(0)};
+            R key = R(-2);
+            if (lane <= km) {
+                const R a = fabs(v.x) + fabs(v.y);
+                key = (a != a) ? (lane == 0 ? R(__builtin_inff()) : R(-1)) : a;
+            }
+            int bidx = lane;
+            argmax_cabs1(key, bidx);
+            const int jp = __builtin_amdgcn_readfirstlane(bidx);
+            const T piv = colbuf[jp];
+            if (lane == 0) ipiv[j] = j + jp + 1;
+            const bool nz = sz::cnonzero(piv);
+            if (nz) {
+                ju = max(ju, min(j + ku + jp, n - 1));
+                const bool inr = colk >= j && colk <= ju;
+                if (jp != 0) {
+                    const int sp = j0 + jp >= WR ? j0 + jp - WR : j0 + jp;
+                    T a0, a1;
+                    W_GET(j0, a0);
+                    W_GET(sp, a1);
+                    if (inr) { W_SET(j0, a1); W_SET(sp, a0); }
+                    if (ntail > 0 && ju >= j + 64) {
+                        if (lane < ntail) {
+                            const int k = j + 64 + lane;
+                            if (k <= ju) {
+                                T *tc = tail + tix(k);
+                                const T t0 = tc[j0], t1 = tc[sp];
+                                tc[j0] = t1;
+                                tc[sp] = t0;
+                            }
+                        }
+                        WAVE_SYNC();
+                    }
+                    const T c0 = colbuf[0];
+                    if (lane == 0) v = piv;
+                    else if (lane == jp) v = c0;
+                }
+                if (km > 0) {
+                    const T rcp = sz::crcp(piv);
+                    const T l = sz::cmul(v, rcp);
+                    if (lane >= 1 && lane <= km) {
+                        lbuf[lane] = l;
+                        ab[(size_t)j * ldab + kv + lane] = l;
+                    }
+                    WAVE_SYNC();
+                    const bool upd = colk > j && colk <= ju;
+                    T uc;
+                    W_GET(j0, uc);
+#pragma unroll
+                    for (int s = 0; s < WR; ++s) {
+                        int r = s - j0;
+                        if (r < 0) r += WR;
+                        if (r >= 1 && r <= km) {
+                            const T lr = lbuf[r];
+                            if (upd) {
+                                T a;
+                                W_GET(s, a);
+                                a = sz::cfnms(a, lr, uc);
+                                W_SET(s, a);
+                            }
+                        }
+                    }
+                    if (ntail > 0 && ju >= j + 64) {
+                        for (int e = lane; e < ntail * WR; e += 64) {
+                            const int t = e / WR, s = e - t * WR;
+                            const int k = j + 64 + t;
+                            int r = s - j0;
+                            if (r < 0) r += WR;
+                            if (k <= ju && r >= 1 && r <= km) {
+                                T *tc = tail + tix(k);
+                                tc[s] = sz::cfnms(tc[s], lbuf[r], tc[j0]);
+                            }
+                        }
+                        WAVE_SYNC();
+                    }
+                }
+            } else {
+                if (first_zero == 0) first_zero = j + 1;
+                if (lane >= 1 && lane <= km) ab[(size_t)j * ldab + kv + lane] = v;
+            }
+            // ---- U row j: slot j0 of every column j..min(j+kv, n-1)
+            {
+                T uj;
+                W_GET(j0, uj);
+                if (colk >= j && colk <= j + kv && colk < n)
+                    ab[(size_t)colk * ldab + kv + j - colk] = uj;
+                if (lane < ntail) {
+                    const int k = j + 64 + lane;
+                    if (k <= j + kv && k < n)
+                        ab[(size_t)k * ldab + kv + j - k] = tail[tix(k) + j0];
+                }
+            }
+            // ---- retire column j: lane pl takes column j+64
+            if (lane == pl) {
+                colk += 64;
+                if (ntail > 0) {
+                    const T *tc = tail + tix(j + 64);
+#pragma unroll
+                    for (int s = 0; s < WR; ++s) {
+                        const T t = tc[s];
+                        W_SET(s, t);
+                    }
+                } else {
+                    r0 = R(0); r1 = R(0); r2 = R(0); r3 = R(0); r4 = R(0);
+                    i0 = R(0); i1 = R(0); i2 = R(0); i3 = R(0); i4 = R(0);
+                }
+            }
+            if (ntail > 0) {
+                WAVE_SYNC();
+                // tail slot of column j+64 now belongs to column j+kv+1
+                if (lane < WR) tail[tix(j + 64) + lane] = T{R(0), R(0)};
+                WAVE_SYNC();
+            }
+            // ---- entering row j+WR into slot j0 (prefetched W64_PF ago)
+            {
+                const int i = j + WR;
+                const bool irow = i < n;
+                T qv = q[u];
+                if (!(irow && colk < n && colk - i <= ku)) qv = T{R(0), R(0)};
+                W_SET(j0, qv);
+                if (lane < ntail) {
+                    const int k = j + 65 + lane;
+                    T tv = qt[u];
+                    if (!(irow && k < n && k - i <= ku)) tv = T{R(0), R(0)};
+                    tail[tix(k) + j0] = tv;
+                }
+                // refill this queue slot for step j + W64_PF
+                const int jn = j + W64_PF;
+                q[u] = band(jn + WR, lane_col_at(jn));
+                qt[u] = lane < ntail ? band(jn + WR, jn + 65 + lane) : T{R(0), R(0)};
+            }
+            if (ntail > 0) WAVE_SYNC();
+            j0 = j0 + 1 == WR ? 0 : j0 + 1;
+            }
+        }
+    }
+    if (lane == 0 && info) info[sys] = first_zero;
+}
+
 // SZ_GBTRF_KERNEL selects the factorization variant.  Default: the LDS
 // window on 256 threads ("lds64"/"lds128" use fewer threads); "reg" and
 // "reg128" select the register-tiled kernel (5x2 tiles on 256 threads or 7x3
@@ -794,6 +1061,19 @@ int launch_gbtrf(int n, int kl, int ku, void *ab, int ldab, long long sab,
     if (nsys < 0) return -10;
     if (nsys == 0 || n == 0) return 0;
     static const char *sel = getenv("SZ_GBTRF_KERNEL");
+    if (sel && sel[0] == 'w' && (kl == 34 || kl == 24) && kl + ku + 1 <= 64 + 64) {
+        const int ntail = kl + ku >= 64 ? kl + ku - 63 : 0;
+        const size_t lds = (size_t)ntail * (kl + 1) * sizeof(cpx<R>);
+        if (kl == 34)
+            hipLaunchKernelGGL((k_gbtrf_w64<R, 35>), dim3(nsys), dim3(64), lds,
+                               stream, n, ku, reinterpret_cast<cpx<R> *>(ab), ldab,
+                               sab, ipiv, sipiv, info, skip, nsys);
+        else
+            hipLaunchKernelGGL((k_gbtrf_w64<R, 25>), dim3(nsys), dim3(64), lds,
+                               stream, n, ku, reinterpret_cast<cpx<R> *>(ab), ldab,
+                               sab, ipiv, sipiv, info, skip, nsys);
+        return hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH;
+    }
     const bool want_lds = !(sel && sel[0] == 'r');
     const bool want_128 = sel && sel[0] == 'r' && sel[3] == '1';
     int rc = 0;

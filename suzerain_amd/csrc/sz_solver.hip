@@ -379,12 +379,42 @@ bsmbsm_solver_zgbsv::bsmbsm_solver_zgbsv(const sz_bsmbsm &b,
         throw std::invalid_argument("Invalid spec in bsmbsm_solver_zgbsv");
 }
 
+void bsmbsm_solver::factor_mark()
+{
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    hipEventRecord(e, (hipStream_t)stream);
+    factor_events_.push_back((void *)e);
+}
+
+double bsmbsm_solver::factor_time_ms(int *count)
+{
+    double total = 0;
+    int n = 0;
+    for (size_t i = 0; i + 1 < factor_events_.size(); i += 2) {
+        hipEvent_t a = (hipEvent_t)factor_events_[i];
+        hipEvent_t b = (hipEvent_t)factor_events_[i + 1];
+        float ms = 0;
+        if (hipEventSynchronize(b) == hipSuccess &&
+            hipEventElapsedTime(&ms, a, b) == hipSuccess) {
+            total += ms;
+            ++n;
+        }
+    }
+    for (void *e : factor_events_) hipEventDestroy((hipEvent_t)e);
+    factor_events_.clear();
+    if (count) *count = n;
+    return total;
+}
+
 int bsmbsm_solver_zgbsv::solve_hook(char trans, int nrhs_)
 {
     int rc = 0;
     if (std::toupper((unsigned char)fact_) == 'N') {
+        if (time_factor) factor_mark();
         rc = sz_zgbtrf_batched(N, KL, KU, LU, ld_LU, stride_LU, ipiv, N, info,
                                nbatch, stream);
+        if (time_factor) factor_mark();
         fact_ = 'F';
         if (rc) return rc;
     }

@@ -36,7 +36,21 @@ class PencilFFT:
         import torch
         return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
+    def _check(self, wave, phys):
+        """Every field must be one dense C-ordered block: the kernels address
+        wave (dNz, dNx/2+1, Ny) and physical (Ny, dNz, dNx) fields from the
+        field base pointer with those extents."""
+        wshape = (self.dNz, self.dNx // 2 + 1, self.Ny)
+        pshape = (self.Ny, self.dNz, self.dNx)
+        if tuple(wave.shape[1:]) != wshape or not wave[0].is_contiguous():
+            raise ValueError(f"wave fields must be contiguous {wshape}")
+        if tuple(phys.shape[1:]) != pshape or not phys[0].is_contiguous():
+            raise ValueError(f"physical fields must be contiguous {pshape}")
+        if wave.shape[0] != phys.shape[0]:
+            raise ValueError("field counts differ")
+
     def wave_to_physical(self, wave, phys):
+        self._check(wave, phys)
         nf = wave.shape[0]
         mix = self._work(nf, wave.device)
         rc = load().sz_fft_wave_to_physical(
@@ -46,6 +60,7 @@ class PencilFFT:
         return check(rc, "sz_fft_wave_to_physical")
 
     def physical_to_wave(self, phys, wave):
+        self._check(wave, phys)
         nf = phys.shape[0]
         mix = self._work(nf, phys.device)
         rc = load().sz_fft_physical_to_wave(

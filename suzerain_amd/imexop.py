@@ -68,6 +68,10 @@ class Imexop:
         (None omits a field) with optional per-system offsets."""
         r, ld = self._refs(refs)
         n = self.op.n
+        for t in (list(x) if not hasattr(x, "dim") else [x]) + \
+                (list(y) if not hasattr(y, "dim") else [y]):
+            if t is not None and t.stride(-1) != 1:
+                raise ValueError("pencils must be contiguous along y")
         if hasattr(x, "dim"):
             nsys = x.shape[1] if nsys is None else nsys
             xs = [x[f] for f in range(5)]

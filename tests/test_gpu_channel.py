@@ -77,6 +77,9 @@ def test_references_and_rhs_match_oracle(gpu):
     l1y = rng.uniform(1, 5, Ny)
     l2y = rng.uniform(1, 25, Ny)
     lxz = np.array([3.0, 4.0, och.SMR91_EVMAXMAG_REAL, och.SMR91_EVMAXMAG_IMAG])
+    # device copies held for the whole call (a temporary's memory would be
+    # recycled by the allocator as soon as its pointer is taken)
+    l1y_d, l2y_d, lxz_d = _t(l1y, gpu), _t(l2y, gpu), _t(lxz, gpu)
     for lin, code in (("rhome_xyz", 1), ("none", 0)):
         out_d = torch.zeros_like(Pd)
         cand_d = torch.zeros(12, dtype=torch.float64, device=gpu)
@@ -85,9 +88,9 @@ def test_references_and_rhs_match_oracle(gpu):
                            C.c_void_p(Ad.data_ptr()), Fp,
                            C.c_void_p(out_d.data_ptr()), Fp,
                            C.c_void_p(refs_d.data_ptr()),
-                           C.c_void_p(_t(l1y, gpu).data_ptr()),
-                           C.c_void_p(_t(l2y, gpu).data_ptr()),
-                           C.c_void_p(_t(lxz, gpu).data_ptr()),
+                           C.c_void_p(l1y_d.data_ptr()),
+                           C.c_void_p(l2y_d.data_ptr()),
+                           C.c_void_p(lxz_d.data_ptr()),
                            C.c_void_p(cand_d.data_ptr()),
                            C.c_void_p(work.data_ptr()), stream)
         assert rc == 0

@@ -81,7 +81,8 @@ def _random_bands(rng, nb, N, kl, ku, dom=0.0):
 
 
 @pytest.mark.parametrize("N,kl,ku", [(1, 0, 0), (9, 2, 1), (60, 9, 4),
-                                     (130, 5, 20), (480, 34, 34)])
+                                     (130, 5, 20), (480, 34, 34),
+                                     (300, 24, 24), (40, 34, 34), (97, 34, 20)])
 def test_gbtrf_gbtrs_match_oracle(gpu, N, kl, ku):
     import suzerain_amd as sa
     rng = np.random.default_rng(N * 7 + kl)
@@ -129,6 +130,29 @@ def test_gbtrf_zero_pivot_info_and_gbsv_skip(gpu):
     got = bd.cpu().numpy()
     np.testing.assert_array_equal(got[1], b[1])      # skipped, untouched
     np.testing.assert_array_equal(got[0], b[0])      # identity solve
+
+
+@pytest.mark.parametrize("N,kl,ku", [(100, 34, 34), (60, 24, 24)])
+def test_gbtrf_zero_pivots_continue_like_lapack(gpu, N, kl, ku):
+    """Exactly zero pivot columns set info to the first one and the
+    factorization continues without scaling or update (ZGBTF2)."""
+    import suzerain_amd as sa
+    rng = np.random.default_rng(N + kl)
+    nb = 3
+    ab = _random_bands(rng, nb, N, kl, ku)
+    ldab = ab.shape[2]
+    ab[1, 10, :] = 0.0                # column 10 of system 1 entirely zero
+    ab[2, N - 1, :] = 0.0             # last column of system 2
+    abd = _t(ab, gpu)
+    ipiv, info = sa.gbtrf(abd, kl, ku)
+    lu, ipiv, info = abd.cpu().numpy(), ipiv.cpu().numpy(), info.cpu().numpy()
+    for s in range(nb):
+        ref = ab[s].reshape(-1).copy()
+        rinfo, rpiv = oracle.zgbtrf(N, kl, ku, ref, ldab)
+        assert info[s] == rinfo
+        np.testing.assert_array_equal(ipiv[s], rpiv)
+        np.testing.assert_array_equal(lu[s], ref.reshape(N, ldab))
+    assert info[1] > 0 and info[2] > 0
 
 
 def test_single_precision_factor_matches_oracle(gpu):
