@@ -128,6 +128,11 @@ def lu_roofline(ms, nsys, N, KL, KU, kernel="k_gbtrf<double>"):
             "fp64_tflops": round(flops_sys * nsys / (ms * 1e-3) / 1e12, 3)}
 
 
+# HBM bytes per LU launch measured with FETCH_SIZE/WRITE_SIZE PMC passes and
+# scaled by a plain-copy calibration (profiles/gbtrf_4560_hbm_traffic_pmc.md),
+# keyed by (kernel, systems per launch).
+PMC_TRAFFIC = {("k_gbtrf<double, 256>", 4560): 10.15e9}
+
 RK3_CONFIG = dict(Nx=96, Ny=96, Nz=96, dNx=144, dNz=144, k=8, htdelta=3.0)
 
 
@@ -179,6 +184,43 @@ def cpu_baseline_rk3(ch, W0, threads):
     return 1.0 / el, el
 
 
+def stage_breakdown(ch):
+    """Device time of each stage of one more (untimed) step, replaying
+    lowstorage::step call by call with HIP events on the channel's stream."""
+    import torch
+    from suzerain_amd.channel import SMR91
+    st = torch.cuda.current_stream()
+    acc = {"explicit_operator": 0.0, "operator_apply": 0.0,
+           "implicit_solve": 0.0, "state_update": 0.0}
+
+    def timed(key, fn):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        r = fn()
+        e1.record(st)
+        e1.synchronize()
+        acc[key] += e0.elapsed_time(e1)
+        return r
+
+    chi = 1.0 / (ch.dNx * ch.dNz)
+    timed("state_update", lambda: ch.copy(0, 1))
+    cand = timed("explicit_operator", lambda: ch.nonlinear(1, True))
+    dt = float(np.nanmin(cand))
+    a, b, g, z = SMR91
+    timed("operator_apply", lambda: ch.apply_operator(dt * a[0], 0))
+    timed("state_update", lambda: ch.add_scaled(chi * dt * g[0], 1, 0))
+    timed("implicit_solve", lambda: ch.invert_operator(-dt * b[0], 0))
+    for i in (1, 2):
+        timed("operator_apply",
+              lambda: ch.accumulate_operator(dt * a[i], 0, chi * dt * z[i], 1))
+        ch.exchange()
+        timed("explicit_operator", lambda: ch.nonlinear(1, False))
+        timed("state_update", lambda: ch.add_scaled(chi * dt * g[i], 1, 0))
+        timed("implicit_solve", lambda: ch.invert_operator(-dt * b[i], 0))
+    return {k: round(v, 3) for k, v in acc.items()}
+
+
 def main_rk3(args):
     import torch
     from suzerain_amd.channel import Channel, laminar_state
@@ -196,6 +238,7 @@ def main_rk3(args):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     fms, fcount = ch.factor_timing(enable=0)
+    stages = stage_breakdown(ch)
     substeps = 3 * args.steps
     N = 5 * ch.Ny
     KL = KU = 5 * (7) - 1
@@ -221,9 +264,14 @@ def main_rk3(args):
                    "dt_last": dts[-1],
                    "parallelism": "single GPU"},
         "implicit_solves_per_s": round(ch.nsys * substeps / elapsed, 1),
+        "stages_ms_per_step": stages,
         "roofline": lu_roofline(fms / max(fcount, 1), ch.nsys, N, KL, KU),
     }
     out["roofline"]["launches_timed"] = fcount
+    kern = os.environ.get("SZ_GBTRF_KERNEL", "")
+    kname = "k_gbtrf<double, 256>" if kern in ("", "lds", "lds256") else None
+    out["roofline"]["kernel"] = kname or f"k_gbtrf ({kern})"
+    out["roofline"]["traffic"] = PMC_TRAFFIC.get((kname, ch.nsys))
     if not args.no_cpu:
         threads = max(1, min(16, os.cpu_count() or 1))
         rate, el = cpu_baseline_rk3(ch, W0, threads)

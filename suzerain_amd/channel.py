@@ -17,7 +17,11 @@ import numpy as np
 
 from ._lib import load, check, sz_isothermal_bc
 
-__all__ = ["Channel", "laminar_state", "DEFAULT_SCENARIO"]
+__all__ = ["Channel", "laminar_state", "DEFAULT_SCENARIO", "SMR91"]
+
+# SMR91 alpha, beta, gamma, zeta (suzerain/lowstorage.hpp:1241-1305)
+SMR91 = ((29 / 96, -3 / 40, 1 / 6), (37 / 160, 5 / 24, 1 / 6),
+         (8 / 15, 5 / 12, 3 / 4), (0.0, -17 / 60, -5 / 12))
 
 # apps/perfect/main_initial.cpp:124-134 defaults with the channel case values
 DEFAULT_SCENARIO = {"alpha": 0.0, "beta": 2.0 / 3.0, "gamma": 1.4, "Ma": 1.5,
@@ -196,6 +200,9 @@ class Channel:
 
     def exchange(self):
         _lib().sz_channel_exchange(self._h)
+
+    def copy(self, src, dst):
+        check(_lib().sz_channel_copy(self._h, src, dst), "sz_channel_copy")
 
     def step(self, max_dt=0.0):
         dt = C.c_double(0)

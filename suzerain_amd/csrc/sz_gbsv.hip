@@ -951,13 +951,15 @@ void k_gbtrf_w64(int n, int ku, cpx<R> *__restrict__ ab, int ldab,
                     const bool upd = colk > j && colk <= ju;
                     T uc;
                     W_GET(j0, uc);
+                    // lanes outside (j, ju] skip the whole loop (exec mask);
+                    // inside, the per-slot row test is uniform
+                    if (upd) {
 #pragma unroll
-                    for (int s = 0; s < WR; ++s) {
-                        int r = s - j0;
-                        if (r < 0) r += WR;
-                        if (r >= 1 && r <= km) {
-                            const T lr = lbuf[r];
-                            if (upd) {
+                        for (int s = 0; s < WR; ++s) {
+                            int r = s - j0;
+                            if (r < 0) r += WR;
+                            if (r >= 1 && r <= km) {
+                                const T lr = lbuf[r];
                                 T a;
                                 W_GET(s, a);
                                 a = sz::cfnms(a, lr, uc);

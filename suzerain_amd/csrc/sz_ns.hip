@@ -406,18 +406,29 @@ void k_ns_rhs(scen s, const double *__restrict__ Pin, long long fstride,
     }
 }
 
-__global__ void k_cand_final(const double *__restrict__ partial, int nblocks,
-                             double *__restrict__ cand)
+// One workgroup per candidate: strided partial minima, then a fixed tree.
+__device__ inline double cmin(double v, double x, bool xyz)
 {
-    const int k = threadIdx.x;
-    if (k >= 12) return;
+    return xyz ? ((v != v || x != x) ? (v != v ? v : x) : fmin(v, x)) : fmin(v, x);
+}
+
+__global__ __launch_bounds__(256)
+void k_cand_final(const double *__restrict__ partial, int nblocks,
+                  double *__restrict__ cand)
+{
+    __shared__ double red[256];
+    const int k = blockIdx.x, tid = threadIdx.x;
     const bool xyz = k < 3;
     double v = 1.7976931348623157e308;
-    for (int b = 0; b < nblocks; ++b) {
-        const double x = partial[(long long)b * 12 + k];
-        v = xyz ? ((v != v || x != x) ? (v != v ? v : x) : fmin(v, x)) : fmin(v, x);
+    for (int b = tid; b < nblocks; b += 256)
+        v = cmin(v, partial[(long long)b * 12 + k], xyz);
+    red[tid] = v;
+    __syncthreads();
+    for (int off = 128; off >= 1; off >>= 1) {
+        if (tid < off) red[tid] = cmin(red[tid], red[tid + off], xyz);
+        __syncthreads();
     }
-    cand[k] = v;
+    if (tid == 0) cand[k] = red[0];
 }
 
 inline bool launch_ok() { return hipGetLastError() == hipSuccess; }
@@ -500,7 +511,7 @@ int sz_ns_rhs(const sz_ns_scenario *sc, int linearize, int zeroth, int Ny,
     else       { if (zeroth) SZ_LAUNCH(false, true); else SZ_LAUNCH(false, false); }
 #undef SZ_LAUNCH
     if (zeroth)
-        hipLaunchKernelGGL(k_cand_final, dim3(1), dim3(64), 0,
+        hipLaunchKernelGGL(k_cand_final, dim3(12), dim3(256), 0,
                            (hipStream_t)stream, (const double *)cp, nblk * Ny,
                            candidates);
     return launch_ok() ? 0 : SZ_ERR_LAUNCH;

@@ -61,7 +61,7 @@ def test_transforms_match_numpy(gpu, Ny, Nx, Nz):
                                atol=1e-13 * np.abs(Wref).max())
     # round trip on a dealiased spectrum of a real field (the kx = 0 plane
     # of an arbitrary W is not Hermitian, so only such spectra survive)
-    Wh = Wref / (dNx * dNz)
+    Wh = np.ascontiguousarray(Wref / (dNx * dNz))
     Pd2 = torch.empty_like(Pd)
     fft.wave_to_physical(torch.from_numpy(Wh).to(gpu), Pd2)
     W2 = torch.zeros_like(Wd)
