@@ -1044,6 +1044,502 @@ void k_gbtrf_w64(int n, int ku, cpx<R> *__restrict__ ab, int ldab,
     if (lane == 0 && info) info[sys] = first_zero;
 }
 
+// ---------------------------------------------------------------- col4
+// The window in registers, four threads per column slot: thread (cs, k)
+// owns row slots k, k+4, k+8, ... of column slot cs (row i sits in slot
+// i mod (kl+1), column c in slot c mod (kl+ku+1)), so every register index is
+// a compile-time constant.  Per step only the pivot column (to wave 0), the
+// multipliers, the new pivot row and the displaced row j cross LDS; wave 0
+// searches the pivot and forms all multipliers at once, one per lane.  Each
+// element receives exactly the fnms sequence of the unblocked loop.
+constexpr int COL4_TPC = 4;
+
+template <typename R, int MR, int NT>
+__global__ __launch_bounds__(NT)
+void k_gbtrf_col4(int n, int kl, int ku, cpx<R> *__restrict__ ab, int ldab,
+                  long long sab, int *__restrict__ ipiv, long long sipiv,
+                  int *__restrict__ info, const int *__restrict__ skip,
+                  int nsys)
+{
+    using T = cpx<R>;
+    constexpr int TPC = COL4_TPC;
+    __shared__ T colbuf[TPC * MR];
+    __shared__ T lbuf[TPC * MR];
+    __shared__ T ubuf[NT / TPC];
+    __shared__ T swp[NT / TPC];
+    __shared__ int s_jp, s_nz;
+
+    const int sys = blockIdx.x;
+    if (sys >= nsys) return;
+    if (skip && skip[sys] != 0) return;
+    ab += (size_t)sys * (size_t)sab;
+    ipiv += (size_t)sys * (size_t)sipiv;
+    const int tid = threadIdx.x;
+    const int cs = tid / TPC, k = tid - cs * TPC;
+    const int kv = kl + ku;
+    const int WR = kl + 1, WC = kv + 1;
+    const bool active = cs < WC;
+    auto band = [&](int i, int c) -> T {
+        const bool ok = i < n && c < n && c - i <= ku && i - c <= kl;
+        const T *p = ab + (ok ? (size_t)c * ldab + kv + i - c : 0);
+        const R x = p->x, y = p->y;
+        return T{ok ? x : R(0), ok ? y : R(0)};
+    };
+
+    T a[MR];
+#pragma unroll
+    for (int m = 0; m < MR; ++m) {
+        const int s = k + TPC * m;
+        a[m] = (active && s < WR) ? band(s, cs) : T{R(0), R(0)};
+    }
+
+    // entering rows: at the end of step j slot j mod WR receives row j+WR;
+    // this thread receives when (j mod WR) mod TPC == k
+    auto colend = [&](int j) {          // this slot's column after step j
+        const int d = (cs - j % WC + WC) % WC;
+        return d == 0 ? j + WC : j + d;
+    };
+    auto next_recv = [&](int j) {       // first step > j this thread receives
+        int t = 1;
+        while ((((j + t) % WR) % TPC) != k) ++t;
+        return j + t;
+    };
+    int nrecv = k < WR ? next_recv(k - 1) : n + WR;
+    T nval = (active && nrecv < n) ? band(nrecv + WR, colend(nrecv)) : T{R(0), R(0)};
+
+    int c0 = 0, j0 = 0, ju = 0, first_zero = 0;
+    for (int j = 0; j < n; ++j) {
+        const int km = min(kl, n - 1 - j);
+        const int d = active ? (cs - c0 + WC) % WC : WC;
+        const int m0 = j0 / TPC, k0 = j0 - m0 * TPC;
+        // (1) pivot column to LDS by logical row
+        if (d == 0) {
+#pragma unroll
+            for (int m = 0; m < MR; ++m) {
+                const int s = k + TPC * m;
+                if (s < WR) {
+                    int r = s - j0;
+                    if (r < 0) r += WR;
+                    colbuf[r] = a[m];
+                }
+            }
+        }
+        LDS_BARRIER();
+        // (2) wave 0: IZAMAX, multipliers, pivots
+        if (tid < 64) {
+            T v = tid <= km ? colbuf[tid] : T{R(0), R(0)};
+            R key = R(-2);
+            if (tid <= km) {
+                const R av = fabs(v.x) + fabs(v.y);
+                key = (av != av) ? (tid == 0 ? R(__builtin_inff()) : R(-1)) : av;
+            }
+            int bidx = tid;
+            argmax_cabs1(key, bidx);
+            const int jp = __builtin_amdgcn_readfirstlane(bidx);
+            const T piv = colbuf[jp];
+            const bool nz = sz::cnonzero(piv);
+            if (tid == 0) {
+                s_jp = jp;
+                s_nz = nz;
+                ipiv[j] = j + jp + 1;
+            }
+            if (nz) {
+                if (tid == jp) v = colbuf[0];
+                const T l = sz::cmul(v, sz::crcp(piv));
+                if (tid >= 1 && tid <= km) {
+                    lbuf[tid] = l;
+                    ab[(size_t)j * ldab + kv + tid] = l;
+                }
+            } else if (tid >= 1 && tid <= km) {
+                ab[(size_t)j * ldab + kv + tid] = v;
+            }
+        }
+        LDS_BARRIER();
+        const int jp = s_jp;
+        const bool nz = s_nz != 0;
+        if (nz) ju = max(ju, min(j + ku + jp, n - 1));
+        else if (first_zero == 0) first_zero = j + 1;
+        const int sp = j0 + jp >= WR ? j0 + jp - WR : j0 + jp;
+        const int mp = sp / TPC, kp = sp - mp * TPC;
+        const bool swap = nz && jp != 0 && d <= ju - j;
+        // (3) new pivot row (old row jp) and displaced row j through LDS
+        if (active && k == kp) {
+#pragma unroll
+            for (int m = 0; m < MR; ++m)
+                if (m == mp) ubuf[cs] = a[m];
+        }
+        if (swap && k == k0) {
+#pragma unroll
+            for (int m = 0; m < MR; ++m)
+                if (m == m0) swp[cs] = a[m];
+        }
+        LDS_BARRIER();
+        if (swap) {
+            if (k == k0) {
+                const T t = ubuf[cs];
+#pragma unroll
+                for (int m = 0; m < MR; ++m)
+                    if (m == m0) a[m] = t;
+            }
+            if (k == kp) {
+                const T t = swp[cs];
+#pragma unroll
+                for (int m = 0; m < MR; ++m)
+                    if (m == mp) a[m] = t;
+            }
+        }
+        // (4) U row j of columns j .. j+kv, then the rank-1 update
+        if (active && k == k0) {
+            const int cg = j + d;
+            if (cg <= j + kv && cg < n) {
+#pragma unroll
+                for (int m = 0; m < MR; ++m)
+                    if (m == m0) ab[(size_t)cg * ldab + kv + j - cg] = a[m];
+            }
+        }
+        if (nz && km > 0 && d >= 1 && d <= ju - j) {
+            const T u = ubuf[cs];
+#pragma unroll
+            for (int m = 0; m < MR; ++m) {
+                const int s = k + TPC * m;
+                if (s < WR) {
+                    int r = s - j0;
+                    if (r < 0) r += WR;
+                    if (r >= 1 && r <= km) a[m] = sz::cfnms(a[m], lbuf[r], u);
+                }
+            }
+        }
+        // (5) column j leaves (its slot becomes column j+kv+1, zero fill);
+        // row j leaves (its slot receives row j+kl+1)
+        if (d == 0) {
+#pragma unroll
+            for (int m = 0; m < MR; ++m) a[m] = T{R(0), R(0)};
+        }
+        if (j == nrecv) {
+            if (active) {
+#pragma unroll
+                for (int m = 0; m < MR; ++m)
+                    if (m == m0) a[m] = nval;
+            }
+            nrecv = next_recv(j);
+            nval = (active && nrecv < n) ? band(nrecv + WR, colend(nrecv))
+                                         : T{R(0), R(0)};
+        }
+        c0 = c0 + 1 == WC ? 0 : c0 + 1;
+        j0 = j0 + 1 == WR ? 0 : j0 + 1;
+    }
+    if (tid == 0 && info) info[sys] = first_zero;
+}
+
+// ---------------------------------------------------------------- quad
+// One barrier per elimination step.  The window sits in registers as in
+// k_gbtrf_col4 (quad cs = threads 4cs..4cs+3 owns column slot cs; lane k of
+// the quad owns row slots k, k+4, ...), slots 0..7 of each component in a
+// named 8-vector and slot 8 in a scalar so that a uniform slot index selects
+// with a gpr-indexed move.  Step j:
+//   A  the quad holding column j (one quad, no other thread involved) finds
+//      the pivot with a local search and two quad DPP rounds, forms every
+//      multiplier, publishes them with the pivot index and value in LDS
+//      (double-buffered by step parity) and clears its column slot, which
+//      becomes the fill-in column j+kl+ku+1;
+//   -  barrier;
+//   B  every quad takes the pivot-row and row-j values of its column by DPP
+//      broadcast inside the quad, swaps, stores its U(j, .) element, one
+//      multiplier and the pivot index (unconditional stores; threads with
+//      nothing to write store U(j,j) again), applies the rank-1 update and
+//      receives the entering row, prefetched QD_PF steps ahead by every
+//      thread of the quad.
+// Every element sees the fnms sequence of the unblocked loop.  Rows past n
+// get zero multipliers and padding slots are never read as window rows, so
+// the update itself needs no per-slot predicate; slot j of lane k0 is
+// updated with a meaningless multiplier and then overwritten by the
+// entering row.
+constexpr int QD_PF = 4;
+
+template <int CTRL, typename R>
+__device__ __forceinline__ R dpp_r(R v)
+{
+    if constexpr (sizeof(R) == 8) {
+        const long long b = __builtin_bit_cast(long long, v);
+        const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF,
+                                                   0xF, false);
+        return __builtin_bit_cast(R, (long long)(((unsigned long long)(unsigned)hi << 32)
+                                                 | (unsigned)lo));
+    } else {
+        return __builtin_bit_cast(
+            R, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF,
+                                           0xF, false));
+    }
+}
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+
+template <typename R>
+__device__ __forceinline__ R readlane_r(R v, int lane)
+{
+    if constexpr (sizeof(R) == 8) {
+        const long long b = __builtin_bit_cast(long long, v);
+        const int lo = __builtin_amdgcn_readlane((int)b, lane);
+        const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+        return __builtin_bit_cast(R, (long long)(((unsigned long long)(unsigned)hi << 32)
+                                                 | (unsigned)lo));
+    } else {
+        return __builtin_bit_cast(R, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v),
+                                                               lane));
+    }
+}
+
+// value held by lane kq (uniform) of the caller's quad
+template <typename R>
+__device__ __forceinline__ cpx<R> quad_bcast(cpx<R> v, int kq)
+{
+    cpx<R> o;
+    switch (kq) {
+    case 0: o.x = dpp_r<0x00>(v.x); o.y = dpp_r<0x00>(v.y); break;
+    case 1: o.x = dpp_r<0x55>(v.x); o.y = dpp_r<0x55>(v.y); break;
+    case 2: o.x = dpp_r<0xAA>(v.x); o.y = dpp_r<0xAA>(v.y); break;
+    default: o.x = dpp_r<0xFF>(v.x); o.y = dpp_r<0xFF>(v.y); break;
+    }
+    return o;
+}
+
+#define Q_GET(M, V)                                                           \
+    do {                                                                      \
+        const int m_ = (M);                                                   \
+        if (MR <= 8 || m_ < 8) { V.x = xr[m_ & 7]; V.y = xi[m_ & 7]; }        \
+        else { V.x = xr8; V.y = xi8; }                                        \
+    } while (0)
+#define Q_SET(M, V)                                                           \
+    do {                                                                      \
+        const int m_ = (M);                                                   \
+        if (MR <= 8 || m_ < 8) { xr[m_ & 7] = V.x; xi[m_ & 7] = V.y; }        \
+        else { xr8 = V.x; xi8 = V.y; }                                        \
+    } while (0)
+
+template <typename R, int MR, int NT>
+__global__ __launch_bounds__(NT)
+void k_gbtrf_quad(int n, int kl, int ku, cpx<R> *__restrict__ ab, int ldab,
+                  long long sab, int *__restrict__ ipiv, long long sipiv,
+                  int *__restrict__ info, const int *__restrict__ skip,
+                  int nsys)
+{
+    using T = cpx<R>;
+    static_assert(MR >= 1 && MR <= 9, "at most 36 row slots");
+    __shared__ T lbuf[2][64];
+    __shared__ T s_piv[2];
+    __shared__ int s_jp[2], s_nz[2];
+
+    const int sys = blockIdx.x;
+    if (sys >= nsys) return;
+    if (skip && skip[sys] != 0) return;
+    ab += (size_t)sys * (size_t)sab;
+    ipiv += (size_t)sys * (size_t)sipiv;
+    const int tid = threadIdx.x;
+    const int cs = tid >> 2, k = tid & 3;
+    const int kv = kl + ku;
+    const int WR = kl + 1, WC = kv + 1;
+    const bool active = cs < WC;
+
+    typedef R v8 __attribute__((ext_vector_type(8)));
+    v8 xr = R(0), xi = R(0);
+    R xr8 = R(0), xi8 = R(0);
+#pragma unroll
+    for (int m = 0; m < MR; ++m) {
+        const int s = k + 4 * m;
+        const bool ok = active && s < WR && s < n && cs < n && cs - s <= ku;
+        const T *p = ab + (ok ? (size_t)cs * ldab + kv + s - cs : 0);
+        const R x = p->x, y = p->y;
+        T v;
+        v.x = ok ? x : R(0);
+        v.y = ok ? y : R(0);
+        Q_SET(m, v);
+    }
+    // raw entering-row elements (clamped address; zeroed when consumed):
+    // at the end of step jj the slot of row jj receives row jj+WR, whose
+    // element in this column slot is column jj + (d ? d : WC), d the slot's
+    // offset from column jj
+    auto enter_raw = [&](int jj, int dd) -> T {
+        const int c = jj + (dd == 0 ? WC : dd), i = jj + WR;
+        const bool ok = active && i < n && c < n;
+        return ab[ok ? (size_t)c * ldab + kv + i - c : 0];
+    };
+    T qv[QD_PF];
+#pragma unroll
+    for (int u = 0; u < QD_PF; ++u) {
+        int dd = cs - u % WC;
+        if (dd < 0) dd += WC;
+        qv[u] = enter_raw(u, dd);
+    }
+    const int rl = kl > 0 ? 1 + tid % kl : 0;   // multiplier row this thread stores
+
+    int c0 = 0, j0 = 0, ju = 0, first_zero = 0;
+    for (int jb = 0; jb < n; jb += QD_PF) {
+#pragma unroll
+        for (int u = 0; u < QD_PF; ++u) {
+            const int j = jb + u;
+            if (j >= n) break;
+            const int p = u & 1;                    // QD_PF is even
+            const int km = min(kl, n - 1 - j);
+            const int m0 = j0 >> 2, k0 = j0 & 3;
+            int d = cs - c0;
+            if (d < 0) d += WC;
+            if (!active) d = WC;
+
+            // ---- A: pivot column
+            if (d == 0) {
+                R key[MR];
+                R best = R(-1);
+#pragma unroll
+                for (int m = 0; m < MR; ++m) {
+                    const int s = k + 4 * m;
+                    int r = s - j0;
+                    if (r < 0) r += WR;
+                    T v;
+                    Q_GET(m, v);
+                    const R a = fabs(v.x) + fabs(v.y);
+                    key[m] = (s < WR && r <= km) ? a : R(-1);
+                    best = fmax(best, key[m]);        // NaN keys never win here
+                }
+                best = fmax(best, dpp_r<0xB1>(best));
+                best = fmax(best, dpp_r<0x4E>(best));
+                int rb = 64;
+#pragma unroll
+                for (int m = 0; m < MR; ++m) {
+                    const int s = k + 4 * m;
+                    int r = s - j0;
+                    if (r < 0) r += WR;
+                    rb = key[m] == best ? min(rb, r) : rb;
+                }
+                rb = min(rb, dpp_i<0xB1>(rb));
+                rb = min(rb, dpp_i<0x4E>(rb));
+                const int L0 = (4 * c0) & 63;
+                T rj, t;
+                Q_GET(m0, t);
+                rj.x = readlane_r(t.x, L0 + k0);
+                rj.y = readlane_r(t.y, L0 + k0);
+                int jp = __builtin_amdgcn_readfirstlane(rb);
+                const R a0 = fabs(rj.x) + fabs(rj.y);
+                if (a0 != a0) jp = 0;                 // a NaN in row j stays
+                const int sp = j0 + jp >= WR ? j0 + jp - WR : j0 + jp;
+                T piv;
+                Q_GET(sp >> 2, t);
+                piv.x = readlane_r(t.x, L0 + (sp & 3));
+                piv.y = readlane_r(t.y, L0 + (sp & 3));
+                const bool nz = sz::cnonzero(piv);
+                if (nz) {
+                    const T rcp = sz::crcp(piv);
+#pragma unroll
+                    for (int m = 0; m < MR; ++m) {
+                        const int s = k + 4 * m;
+                        int r = s - j0;
+                        if (r < 0) r += WR;
+                        T v;
+                        Q_GET(m, v);
+                        const T l = sz::cmul(v, rcp);
+                        if (s < WR) lbuf[p][r] = r <= km ? l : T{R(0), R(0)};
+                    }
+                    // row jp now holds old row j: same wave, written last
+                    if (jp != 0 && k == 0) lbuf[p][jp] = sz::cmul(rj, rcp);
+                } else {
+#pragma unroll
+                    for (int m = 0; m < MR; ++m) {
+                        const int s = k + 4 * m;
+                        int r = s - j0;
+                        if (r < 0) r += WR;
+                        T v;
+                        Q_GET(m, v);
+                        if (s < WR) lbuf[p][r] = r <= km ? v : T{R(0), R(0)};
+                    }
+                }
+                if (k == 0) {
+                    s_jp[p] = jp;
+                    s_nz[p] = nz ? 1 : 0;
+                    s_piv[p] = piv;
+                }
+                xr = R(0);
+                xi = R(0);
+                xr8 = R(0);
+                xi8 = R(0);
+            }
+            LDS_BARRIER();
+
+            // ---- B: swap, stores, update, entering row
+            const int jp = __builtin_amdgcn_readfirstlane(s_jp[p]);
+            const bool nz = __builtin_amdgcn_readfirstlane(s_nz[p]) != 0;
+            const T piv = s_piv[p];
+            if (nz) ju = max(ju, min(j + ku + jp, n - 1));
+            else if (first_zero == 0) first_zero = j + 1;
+            const int sp = j0 + jp >= WR ? j0 + jp - WR : j0 + jp;
+            const int mp = sp >> 2, kp = sp & 3;
+            T t;
+            Q_GET(mp, t);
+            const T pv = quad_bcast(t, kp);           // pivot row, this column
+            T uval = pv;
+            const bool inr = d <= ju - j;
+            if (jp != 0) {
+                Q_GET(m0, t);
+                const T rj = quad_bcast(t, k0);       // row j, this column
+                if (inr) {
+                    if (k == k0) Q_SET(m0, pv);
+                    if (k == kp) Q_SET(mp, rj);
+                }
+                uval.x = inr ? pv.x : rj.x;
+                uval.y = inr ? pv.y : rj.y;
+            }
+            {
+                const int cg = j + d;
+                const bool ok = d > 0 && active && cg < n;
+                const size_t ujj = (size_t)j * ldab + kv;
+                T v;
+                v.x = ok ? uval.x : piv.x;
+                v.y = ok ? uval.y : piv.y;
+                ab[ok ? (size_t)cg * ldab + kv - d : ujj] = v;
+                const bool okl = rl >= 1 && rl <= km;
+                const T lv = lbuf[p][okl ? rl : 0];
+                T w;
+                w.x = okl ? lv.x : piv.x;
+                w.y = okl ? lv.y : piv.y;
+                ab[okl ? ujj + rl : ujj] = w;
+                ipiv[j] = j + jp + 1;
+            }
+            if (nz && km > 0 && d >= 1 && d <= ju - j) {
+#pragma unroll
+                for (int m = 0; m < MR; ++m) {
+                    const int s = k + 4 * m;
+                    int r = s - j0;
+                    if (r < 0) r += WR;
+                    const T l = lbuf[p][r];
+                    T a;
+                    Q_GET(m, a);
+                    a = sz::cfnms(a, l, pv);
+                    Q_SET(m, a);
+                }
+            }
+            {
+                const int ce = j + (d == 0 ? WC : d);
+                const bool ok = active && j + WR < n && ce < n;
+                T e = qv[u];
+                e.x = ok ? e.x : R(0);
+                e.y = ok ? e.y : R(0);
+                if (k == k0) Q_SET(m0, e);
+                int dd = d - QD_PF;
+                while (dd < 0) dd += WC;
+                qv[u] = enter_raw(j + QD_PF, dd);
+            }
+            c0 = c0 + 1 == WC ? 0 : c0 + 1;
+            j0 = j0 + 1 == WR ? 0 : j0 + 1;
+        }
+    }
+    if (tid == 0 && info) info[sys] = first_zero;
+}
+#undef Q_GET
+#undef Q_SET
+
 // SZ_GBTRF_KERNEL selects the factorization variant.  Default: the LDS
 // window on 256 threads ("lds64"/"lds128" use fewer threads); "reg" and
 // "reg128" select the register-tiled kernel (5x2 tiles on 256 threads or 7x3
@@ -1063,6 +1559,38 @@ int launch_gbtrf(int n, int kl, int ku, void *ab, int ldab, long long sab,
     if (nsys < 0) return -10;
     if (nsys == 0 || n == 0) return 0;
     static const char *sel = getenv("SZ_GBTRF_KERNEL");
+    if (sel && sel[0] == 'q' && kl + 1 <= 36) {
+        const int WC = kl + ku + 1;
+        const int nt = ((4 * WC + 63) / 64) * 64;
+#define SZ_QUAD(MR, NT_)                                                        \
+        hipLaunchKernelGGL((k_gbtrf_quad<R, MR, NT_>), dim3(nsys), dim3(NT_), 0,  \
+                           stream, n, kl, ku, reinterpret_cast<cpx<R> *>(ab),     \
+                           ldab, sab, ipiv, sipiv, info, skip, nsys)
+        if (nt <= 64) { if (kl + 1 <= 16) SZ_QUAD(4, 64); else SZ_QUAD(9, 64); }
+        else if (nt <= 128) { if (kl + 1 <= 28) SZ_QUAD(7, 128); else SZ_QUAD(9, 128); }
+        else if (nt <= 192) { if (kl + 1 <= 28) SZ_QUAD(7, 192); else SZ_QUAD(9, 192); }
+        else if (nt <= 256) { if (kl + 1 <= 28) SZ_QUAD(7, 256); else SZ_QUAD(9, 256); }
+        else if (nt <= 320) SZ_QUAD(9, 320);
+        else return SZ_ERR_LDS;
+#undef SZ_QUAD
+        return hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH;
+    }
+    if (sel && sel[0] == 'c' && kl + 1 <= COL4_TPC * 9) {
+        const int WC = kl + ku + 1;
+        const int nt = ((COL4_TPC * WC + 63) / 64) * 64;
+#define SZ_COL4(MR, NT_)                                                        \
+        hipLaunchKernelGGL((k_gbtrf_col4<R, MR, NT_>), dim3(nsys), dim3(NT_), 0,  \
+                           stream, n, kl, ku, reinterpret_cast<cpx<R> *>(ab),     \
+                           ldab, sab, ipiv, sipiv, info, skip, nsys)
+        if (nt <= 64) { if (kl + 1 <= 4 * 4) SZ_COL4(4, 64); else SZ_COL4(9, 64); }
+        else if (nt <= 128) { if (kl + 1 <= 4 * 7) SZ_COL4(7, 128); else SZ_COL4(9, 128); }
+        else if (nt <= 192) { if (kl + 1 <= 4 * 7) SZ_COL4(7, 192); else SZ_COL4(9, 192); }
+        else if (nt <= 256) { if (kl + 1 <= 4 * 7) SZ_COL4(7, 256); else SZ_COL4(9, 256); }
+        else if (nt <= 320) SZ_COL4(9, 320);
+        else return SZ_ERR_LDS;
+#undef SZ_COL4
+        return hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH;
+    }
     if (sel && sel[0] == 'w' && (kl == 34 || kl == 24) && kl + ku + 1 <= 64 + 64) {
         const int ntail = kl + ku >= 64 ? kl + ku - 63 : 0;
         const size_t lds = (size_t)ntail * (kl + 1) * sizeof(cpx<R>);
