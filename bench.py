@@ -128,6 +128,17 @@ def lu_roofline(ms, nsys, N, KL, KU, kernel="k_gbtrf<double>"):
             "fp64_tflops": round(flops_sys * nsys / (ms * 1e-3) / 1e12, 3)}
 
 
+def lu_kernel_name(KL, KU):
+    """The factorization kernel the library dispatches for this band shape
+    (the one-wave register kernel for the channel shapes unless
+    SZ_GBTRF_KERNEL selects the LDS window kernel)."""
+    sel = os.environ.get("SZ_GBTRF_KERNEL", "")
+    if sel.startswith("l") or (KL, KU) not in ((34, 34), (24, 24), (34, 20)):
+        return "k_gbtrf<double, 256>"
+    bs = {"v2": 2, "v4": 4}.get(sel, 8)
+    return f"k_gbtrf_wave<double, {KL + 1}, {KL + KU}, {bs}>"
+
+
 # HBM bytes per LU launch measured with FETCH_SIZE/WRITE_SIZE PMC passes and
 # scaled by a plain-copy calibration (profiles/gbtrf_4560_hbm_traffic_pmc.md),
 # keyed by (kernel, systems per launch).
@@ -268,9 +279,8 @@ def main_rk3(args):
         "roofline": lu_roofline(fms / max(fcount, 1), ch.nsys, N, KL, KU),
     }
     out["roofline"]["launches_timed"] = fcount
-    kern = os.environ.get("SZ_GBTRF_KERNEL", "")
-    kname = "k_gbtrf<double, 256>" if kern in ("", "lds", "lds256") else None
-    out["roofline"]["kernel"] = kname or f"k_gbtrf ({kern})"
+    kname = lu_kernel_name(KL, KU)
+    out["roofline"]["kernel"] = kname
     out["roofline"]["traffic"] = PMC_TRAFFIC.get((kname, ch.nsys))
     if not args.no_cpu:
         threads = max(1, min(16, os.cpu_count() or 1))
@@ -379,7 +389,7 @@ def main():
                                "KL=KU=34, 1152 (kx,kz) systems, 1 RHS",
                    "systems": nsys, "N": N, "KL": KL, "KU": KU,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "k_gbtrf<double>",
+        "roofline": {"bound": "hbm", "kernel": lu_kernel_name(KL, KU),
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "avg_launch_ms": round(trf_ms, 4),

@@ -13,12 +13,16 @@
 //   * info = first 1-based column with an exactly zero pivot, else 0; the
 //     factorization continues past a zero pivot without scaling or update.
 //
-// One workgroup factors one system.  The active (kl+1) x (kl+ku+1) window of
-// rows j..j+kl and columns j..j+kl+ku lives in LDS as a circular buffer in
-// both directions, so each elimination step moves exactly one row in from
-// HBM (row j+kl+1) and one row (U row j) plus one column (multipliers of
-// column j) out; nothing else touches HBM.  Wave 0 does the pivot search, the
-// row swap and the scaling; then all four waves apply the rank-1 update.
+// Two factorization kernels, both moving each element through HBM once: the
+// active (kl+1) x (kl+ku+1) window of rows j..j+kl and columns j..j+kl+ku
+// stays on chip, so an elimination step brings in one row (row j+kl+1) and
+// sends out one row (U row j) and one column (the multipliers of column j).
+//   * k_gbtrf_wave: one wavefront per system, one window column per lane,
+//     the rows in registers; no barriers, about 400 VALU instructions per
+//     step.  Used for the band shapes of the channel operators.
+//   * k_gbtrf: one workgroup per system, the window in LDS as a circular
+//     buffer in both directions; wave 0 does the pivot search, the swap and
+//     the scaling, then all waves apply the rank-1 update.  Any band shape.
 #include "sz_common.hpp"
 
 #include <cstdlib>
@@ -244,319 +248,6 @@ void k_gbtrf(int n, int kl, int ku, cpx<R> *__restrict__ ab, int ldab,
     if (tid == 0 && info) info[sys] = s_info;
 }
 
-// ---------------------------------------------------------------------------
-// Register-tiled factorization.  The same circular (kl+1) x (kl+ku+1) window
-// as k_gbtrf, but every element lives in a register of the thread that owns
-// its (row slot, column slot) tile for the whole factorization: thread
-// (tr, tc) owns row slots tr*RT .. tr*RT+RT-1 and column slots
-// tc*CT .. tc*CT+CT-1.  Only the row leaving the window (U row j), the
-// multipliers of column j, the pivot candidates and the incoming row pass
-// through LDS, double-buffered by step parity, so a step costs two barriers
-// and RT*CT complex FMAs per thread.  Rows and columns outside the active
-// ranges receive zero multipliers, which leaves them exactly unchanged.
-// Compile-time loops: the body sees std::integral_constant indices, so tile
-// arrays are only ever indexed by constants and stay in registers.
-template <typename F, int... I>
-__device__ __forceinline__ void sfor_impl(F &&f, std::integer_sequence<int, I...>)
-{
-    (f(std::integral_constant<int, I>{}), ...);
-}
-
-template <int N, typename F>
-__device__ __forceinline__ void sfor(F &&f)
-{
-    sfor_impl(f, std::make_integer_sequence<int, N>{});
-}
-
-template <int RT, int CT>
-struct tile_sel {
-    // element (a, b) with a runtime row a (get_row) or column b (get_col)
-    template <int B, typename R>
-    static __device__ __forceinline__ R get_row(const R (&v)[RT * CT], int a)
-    {
-        R x = v[B];
-        sfor<RT>([&](auto aa) {
-            if constexpr (aa.value > 0) x = (aa.value == a) ? v[aa.value * CT + B] : x;
-        });
-        return x;
-    }
-    template <int A, typename R>
-    static __device__ __forceinline__ R get_col(const R (&v)[RT * CT], int b)
-    {
-        R x = v[A * CT];
-        sfor<CT>([&](auto bb) {
-            if constexpr (bb.value > 0) x = (bb.value == b) ? v[A * CT + bb.value] : x;
-        });
-        return x;
-    }
-    template <int B, typename R>
-    static __device__ __forceinline__ void set_row(R (&v)[RT * CT], int a, R x)
-    {
-        sfor<RT>([&](auto aa) {
-            v[aa.value * CT + B] = (aa.value == a) ? x : v[aa.value * CT + B];
-        });
-    }
-    template <int A, typename R>
-    static __device__ __forceinline__ void set_col(R (&v)[RT * CT], int b, R x)
-    {
-        sfor<CT>([&](auto bb) {
-            v[A * CT + bb.value] = (bb.value == b) ? x : v[A * CT + bb.value];
-        });
-    }
-};
-
-constexpr int REG_PF = 4;   // incoming rows prefetched ahead of their step
-
-template <typename R, int RT, int CT, int NT>
-__global__ __launch_bounds__(NT)
-void k_gbtrf_reg(int n, int kl, int ku, cpx<R> *__restrict__ ab, int ldab,
-                 long long sab, int *__restrict__ ipiv, long long sipiv,
-                 int *__restrict__ info, const int *__restrict__ skip, int nsys)
-{
-    using T = cpx<R>;
-    using S = tile_sel<RT, CT>;
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int sys = blockIdx.x;
-    if (sys >= nsys) return;
-    if (skip && skip[sys] != 0) return;
-    ab += (size_t)sys * (size_t)sab;
-    ipiv += (size_t)sys * (size_t)sipiv;
-    const int tid = threadIdx.x;
-    const int kv = kl + ku;
-    const int WR = kl + 1, WC = kv + 1;
-    const int NRG = (WR + RT - 1) / RT, NCG = (WC + CT - 1) / CT;
-    const int tr = tid / NCG, tc = tid - tr * NCG;
-    const bool tile = tr < NRG;
-
-    // LDS: per parity {Uold[WC], Unew[WC], newrow[WC], Lcol[WR], cval[NRG]},
-    // then cabs[2][NRG] and crow[2][NRG]
-    const int PB = 3 * WC + WR + NRG;
-    T *tb = reinterpret_cast<T *>(smem);
-    R *cabs_ = reinterpret_cast<R *>(tb + 2 * PB);
-    int *crow_ = reinterpret_cast<int *>(cabs_ + 2 * NRG);
-
-    R vx[RT * CT], vy[RT * CT];
-    sfor<RT * CT>([&](auto e) {
-        constexpr int a = e.value / CT, b = e.value % CT;
-        const int r = tr * RT + a, c = tc * CT + b;
-        const bool ok = tile && r < WR && c < WC && r < n && c < n &&
-                        r - c <= kl && c - r <= ku;
-        const T v = ab[ok ? (size_t)c * ldab + kv + r - c : 0];
-        vx[e.value] = ok ? v.x : R(0);
-        vy[e.value] = ok ? v.y : R(0);
-    });
-    // Every thread issues the same global loads and stores each step (see
-    // k_gbtrf); thread column cl brings in element cl of the entering row.
-    const int cl = tid % WC;
-    const int brow = kv + kl - cl;
-    const int rl = 1 + tid % (kl > 0 ? kl : 1);
-    T q[REG_PF];
-#pragma unroll
-    for (int u = 0; u < REG_PF; ++u) {
-        const int i = u + WR, k = u + 1 + cl;
-        const bool ok = i < n && k < n;
-        q[u] = ab[ok ? (size_t)k * ldab + brow : 0];
-    }
-
-    int ju = 0, first_zero = 0;
-    int rs0 = 0, cs0 = 0;
-    for (int j0 = 0; j0 < n; j0 += REG_PF) {
-#pragma unroll
-        for (int u = 0; u < REG_PF; ++u) {
-            const int j = j0 + u;
-            if (j >= n) break;
-            const int par = j & 1;
-            T *Uold = tb + par * PB, *Unew = Uold + WC, *newrow = Unew + WC;
-            T *Lcol = newrow + WC, *cval = Lcol + WR;
-            R *cabs = cabs_ + par * NRG;
-            int *crow = crow_ + par * NRG;
-            const int km = min(kl, n - 1 - j);
-            const int tr0 = rs0 / RT, a0i = rs0 - tr0 * RT;
-            const int tc0 = cs0 / CT, b0 = cs0 - tc0 * CT;
-
-            // ---- phase A: local pivot candidates, row j, incoming row
-            if (tile && tc == tc0) {
-                R best = R(-2);
-                int br = 0x7fffffff;
-                R bx = R(0), by = R(0);
-                sfor<RT>([&](auto ac) {
-                    constexpr int a = ac.value;
-                    const int rs = tr * RT + a;
-                    int r = rs - rs0; if (r < 0) r += WR;
-                    const R x = S::template get_col<a>(vx, b0);
-                    const R y = S::template get_col<a>(vy, b0);
-                    R ab1 = fabs(x) + fabs(y);
-                    if (ab1 != ab1) ab1 = R(-1);
-                    const bool cand = rs < WR && r <= km;
-                    const bool take = cand && (ab1 > best || (ab1 == best && r < br));
-                    best = take ? ab1 : best;
-                    br = take ? r : br;
-                    bx = take ? x : bx;
-                    by = take ? y : by;
-                });
-                cabs[tr] = best;
-                crow[tr] = br;
-                cval[tr] = T{bx, by};
-            }
-            if (tile && tr == tr0) {
-                sfor<CT>([&](auto bc) {
-                    constexpr int b = bc.value;
-                    const int cs = tc * CT + b;
-                    int c = cs - cs0; if (c < 0) c += WC;
-                    if (cs < WC)
-                        Uold[c] = T{S::template get_row<b>(vx, a0i),
-                                    S::template get_row<b>(vy, a0i)};
-                });
-            }
-            if (tid < WC) {
-                const bool in = (j + WR < n) && (j + 1 + tid < n);
-                T qv;
-                qv.x = in ? q[u].x : R(0);
-                qv.y = in ? q[u].y : R(0);
-                newrow[tid] = qv;
-            }
-            LDS_BARRIER();
-
-            // ---- phase B: pivot, interchange, multipliers
-            R best = R(-3);
-            int p = 0;
-            T piv = {0, 0};
-            for (int g = 0; g < NRG; ++g) {
-                const R a = cabs[g];
-                const int r = crow[g];
-                if (a > best || (a == best && r < p)) {
-                    best = a; p = r; piv = cval[g];
-                }
-            }
-            const T a0 = Uold[0];
-            const R a0abs = sz::cabs1(a0);
-            if (a0abs != a0abs) { p = 0; piv = a0; }
-            const bool nz = sz::cnonzero(piv);
-            if (nz) ju = max(ju, min(j + ku + p, n - 1));
-            else if (first_zero == 0) first_zero = j + 1;
-            const bool swap = nz && p != 0;
-            int prs = rs0 + p; if (prs >= WR) prs -= WR;
-            const int w = ju - j;
-            T rcp;
-            rcp.x = R(0);
-            rcp.y = R(0);
-            if (nz) rcp = sz::crcp(piv);
-            const int trp = prs / RT, ap = prs - trp * RT;
-            if (swap && tile && tr == trp) {
-                sfor<CT>([&](auto bc) {
-                    constexpr int b = bc.value;
-                    const int cs = tc * CT + b;
-                    int c = cs - cs0; if (c < 0) c += WC;
-                    if (cs < WC && c <= w) {
-                        Unew[c] = T{S::template get_row<b>(vx, ap),
-                                    S::template get_row<b>(vy, ap)};
-                        const T o = Uold[c];
-                        S::template set_row<b>(vx, ap, o.x);
-                        S::template set_row<b>(vy, ap, o.y);
-                    }
-                });
-            }
-            if (tile && tc == tc0) {
-                sfor<RT>([&](auto ac) {
-                    constexpr int a = ac.value;
-                    const int rs = tr * RT + a;
-                    int r = rs - rs0; if (r < 0) r += WR;
-                    if (rs < WR && r >= 1 && r <= km) {
-                        T v;
-                        v.x = S::template get_col<a>(vx, b0);
-                        v.y = S::template get_col<a>(vy, b0);
-                        const T l = sz::cmul(v, rcp);
-                        T o;
-                        o.x = nz ? l.x : v.x;
-                        o.y = nz ? l.y : v.y;
-                        Lcol[r] = o;
-                    }
-                });
-            }
-            LDS_BARRIER();
-
-            // ---- phase C: rank-1 update, outputs, window shift
-            {
-                R lx[RT], ly[RT], ux[CT], uy[CT];
-                sfor<RT>([&](auto ac) {
-                    constexpr int a = ac.value;
-                    const int rs = tr * RT + a;
-                    int r = rs - rs0; if (r < 0) r += WR;
-                    const bool lr = nz && rs < WR && r >= 1 && r <= km;
-                    const T l = Lcol[lr ? r : 0];
-                    lx[a] = lr ? l.x : R(0);
-                    ly[a] = lr ? l.y : R(0);
-                });
-                sfor<CT>([&](auto bc) {
-                    constexpr int b = bc.value;
-                    const int cs = tc * CT + b;
-                    int c = cs - cs0; if (c < 0) c += WC;
-                    const bool lc = cs < WC && c >= 1 && c <= w;
-                    const T uo = Uold[lc ? c : 0], un = Unew[lc ? c : 0];
-                    ux[b] = lc ? (swap ? un.x : uo.x) : R(0);
-                    uy[b] = lc ? (swap ? un.y : uo.y) : R(0);
-                });
-                sfor<RT * CT>([&](auto e) {
-                    constexpr int a = e.value / CT, b = e.value % CT;
-                    const T nv = sz::cfnms(T{vx[e.value], vy[e.value]},
-                                           T{lx[a], ly[a]}, T{ux[b], uy[b]});
-                    vx[e.value] = nv.x;
-                    vy[e.value] = nv.y;
-                });
-            }
-            {
-                const int cmax = min(kv, n - 1 - j);
-                const int ce = cl <= cmax ? cl : 0;
-                const T uo = Uold[ce], un = Unew[ce];
-                const bool sw = swap && ce <= w;
-                T ur;
-                ur.x = sw ? un.x : uo.x;
-                ur.y = sw ? un.y : uo.y;
-                ab[(size_t)(j + ce) * ldab + kv - ce] = ur;
-                const int re = (kl > 0 && rl <= km) ? rl : 0;
-                const T lv = Lcol[re > 0 ? re : 0];
-                const T u0 = swap ? Unew[0] : Uold[0];
-                T lo;
-                lo.x = re ? lv.x : u0.x;
-                lo.y = re ? lv.y : u0.y;
-                ab[(size_t)j * ldab + kv + re] = lo;
-                ipiv[j] = j + p + 1;
-            }
-            if (tile && tc == tc0) {
-                sfor<RT>([&](auto ac) {
-                    constexpr int a = ac.value;
-                    const int rs = tr * RT + a;
-                    if (rs != rs0) {
-                        S::template set_col<a>(vx, b0, R(0));
-                        S::template set_col<a>(vy, b0, R(0));
-                    }
-                });
-            }
-            if (tile && tr == tr0) {
-                sfor<CT>([&](auto bc) {
-                    constexpr int b = bc.value;
-                    const int cs = tc * CT + b;
-                    int c1 = cs - cs0 - 1; if (c1 < 0) c1 += WC;
-                    const T nv = newrow[cs < WC ? c1 : 0];
-                    S::template set_row<b>(vx, a0i, cs < WC ? nv.x : R(0));
-                    S::template set_row<b>(vy, a0i, cs < WC ? nv.y : R(0));
-                });
-            }
-            {
-                const int i = j + REG_PF + WR, k = j + REG_PF + 1 + cl;
-                const bool ok = i < n && k < n;
-                q[u] = ab[ok ? (size_t)k * ldab + brow : 0];
-            }
-            if (++rs0 == WR) rs0 = 0;
-            if (++cs0 == WC) cs0 = 0;
-        }
-    }
-    if (tid == 0 && info) info[sys] = first_zero;
-}
-
-// Columns of the factors stream through LDS in chunks; the next chunk is
-// read into registers (TRS_PRE elements per lane, addresses clamped so every
-// lane issues the same loads) while the current one is consumed.
 constexpr int TRS_PRE = 18;
 
 template <typename R>
@@ -758,503 +449,13 @@ void k_gbtrs(int trans, int n, int kl, int ku, int nrhs,
     }
 }
 
-template <typename R, int RT, int CT, int NT>
-bool try_reg(int n, int kl, int ku, void *ab, int ldab, long long sab,
-             int *ipiv, long long sipiv, int *info, const int *skip, int nsys,
-             hipStream_t stream, int *rc)
-{
-    const int WR = kl + 1, WC = kl + ku + 1;
-    const int NRG = (WR + RT - 1) / RT, NCG = (WC + CT - 1) / CT;
-    if (NRG * NCG > NT || WC > NT) return false;
-    const size_t lds = (size_t)2 * (3 * WC + WR + NRG) * sizeof(cpx<R>) +
-                       (size_t)2 * NRG * (sizeof(R) + sizeof(int));
-    hipLaunchKernelGGL((k_gbtrf_reg<R, RT, CT, NT>), dim3(nsys), dim3(NT), lds,
-                       stream, n, kl, ku, reinterpret_cast<cpx<R> *>(ab), ldab,
-                       sab, ipiv, sipiv, info, skip, nsys);
-    *rc = hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH;
-    return true;
-}
-
-// ---------------------------------------------------------------- w64
-// One wavefront per system, no workgroup barriers.  Window column j+c
-// (0 <= c < 64) lives in the registers of lane (j+c) mod 64 as kl+1 row
-// slots (row i in slot i mod (kl+1)), kept in short ext_vector chunks so a
-// uniform slot index becomes one gpr-indexed move; columns j+64 .. j+kl+ku
-// (only when kl+ku >= 64) live in a small LDS tail indexed by column mod
-// the tail width.  Per step the pivot column is transposed through LDS so the
-// IZAMAX search and the scaling run across lanes, the multipliers are
-// broadcast from LDS, and each lane applies the rank-1 update to its own
-// column with the same fnms per element, in the same order, as the LDS
-// kernel above and the unblocked LAPACK loop -- results are bit-identical.
-
+// Orders LDS accesses of one wavefront (no s_barrier: a single wave).
 #define WAVE_SYNC()                                                           \
     do {                                                                      \
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");                \
         __builtin_amdgcn_wave_barrier();                                      \
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");                \
     } while (0)
-
-// The window column of one lane: 40 row slots in five 8-wide chunks per
-// component, held as separate named vectors (arrays of vectors are demoted to
-// scratch).  A uniform slot index selects a chunk with independent uniform
-// branches and the element with a gpr-indexed move.
-#define W_GET(S, V)                                                           \
-    do {                                                                      \
-        const int q_ = (S) >> 3, e_ = (S) & 7;                                \
-        if (q_ == 0) { V.x = r0[e_]; V.y = i0[e_]; }                          \
-        if (q_ == 1) { V.x = r1[e_]; V.y = i1[e_]; }                          \
-        if (q_ == 2) { V.x = r2[e_]; V.y = i2[e_]; }                          \
-        if (q_ == 3) { V.x = r3[e_]; V.y = i3[e_]; }                          \
-        if (q_ == 4) { V.x = r4[e_]; V.y = i4[e_]; }                          \
-    } while (0)
-#define W_SET(S, V)                                                           \
-    do {                                                                      \
-        const int q_ = (S) >> 3, e_ = (S) & 7;                                \
-        if (q_ == 0) { r0[e_] = V.x; i0[e_] = V.y; }                          \
-        if (q_ == 1) { r1[e_] = V.x; i1[e_] = V.y; }                          \
-        if (q_ == 2) { r2[e_] = V.x; i2[e_] = V.y; }                          \
-        if (q_ == 3) { r3[e_] = V.x; i3[e_] = V.y; }                          \
-        if (q_ == 4) { r4[e_] = V.x; i4[e_] = V.y; }                          \
-    } while (0)
-
-constexpr int W64_PF = 4;   // entering rows prefetched ahead of their step
-
-template <typename R, int WR>
-__global__ __launch_bounds__(64)
-void k_gbtrf_w64(int n, int ku, cpx<R> *__restrict__ ab, int ldab,
-                 long long sab, int *__restrict__ ipiv, long long sipiv,
-                 int *__restrict__ info, const int *__restrict__ skip,
-                 int nsys)
-{
-    using T = cpx<R>;
-    constexpr int kl = WR - 1;
-    static_assert(WR <= 40, "five 8-slot chunks per component");
-    __shared__ T colbuf[64];
-    __shared__ T lbuf[64];
-    extern __shared__ __align__(16) unsigned char smem[];
-    T *tail = reinterpret_cast<T *>(smem);
-
-    const int sys = blockIdx.x;
-    if (sys >= nsys) return;
-    if (skip && skip[sys] != 0) return;
-    ab += (size_t)sys * (size_t)sab;
-    ipiv += (size_t)sys * (size_t)sipiv;
-    const int lane = threadIdx.x;
-    const int kv = kl + ku;
-    const int ntail = kv >= 64 ? kv - 63 : 0;
-    auto tix = [&](int k) { return (k % (ntail > 0 ? ntail : 1)) * WR; };
-    auto band = [&](int i, int k) -> T {   // A(i, k) on entry, 0 outside
-        // unconditional load from a clamped (always valid) address, then a
-        // select, so outstanding-load counting stays exact
-        const bool ok = i < n && k < n && k - i <= ku && i - k <= kl;
-        const T *p = ab + (ok ? (size_t)k * ldab + kv + i - k : 0);
-        const R x = p->x, y = p->y;
-        return T{ok ? x : R(0), ok ? y : R(0)};
-    };
-
-    // initial window: rows 0..kl, columns 0..kv
-    typedef R v8 __attribute__((ext_vector_type(8)));
-    v8 r0 = R(0), r1 = R(0), r2 = R(0), r3 = R(0), r4 = R(0);
-    v8 i0 = R(0), i1 = R(0), i2 = R(0), i3 = R(0), i4 = R(0);
-    int colk = lane;
-#pragma unroll
-    for (int s = 0; s < WR; ++s) {
-        const T v = band(s, colk);
-        W_SET(s, v);
-    }
-    for (int e = lane; e < ntail * WR; e += 64) {
-        const int t = e / WR, s = e - t * WR;
-        const int k = 64 + t;
-        tail[tix(k) + s] = band(s, k);
-    }
-    WAVE_SYNC();
-
-    // prefetch queue of entering-row elements: step j brings row j+WR into
-    // slot j mod WR of the lane's column at that time (and tail columns)
-    T q[W64_PF], qt[W64_PF];
-    auto lane_col_at = [&](int j) {      // lane's column during step j+1
-        const int b = j + 1;
-        return b + (((lane - b) % 64) + 64) % 64;
-    };
-#pragma unroll
-    for (int u = 0; u < W64_PF; ++u) {
-        const int i = u + WR;
-        q[u] = band(i, lane_col_at(u));
-        qt[u] = lane < ntail ? band(i, u + 65 + lane) : T{R(0), R(0)};
-    }
-
-    int ju = 0, first_zero = 0, j0 = 0;
-    for (int jb = 0; jb < n; jb += W64_PF) {
-#pragma unroll
-        for (int u = 0; u < W64_PF; ++u) {
-            const int j = jb + u;
-            if (j < n) {
-            const int km = min(kl, n - 1 - j);
-            const int pl = j & 63;
-            // ---- pivot column to LDS, by logical row
-            if (lane == pl) {
-#pragma unroll
-                for (int s = 0; s < WR; ++s) {
-                    int r = s - j0;
-                    if (r < 0) r += WR;
-                    T v;
-                    W_GET(s, v);
-                    colbuf[r] = v;
-                }
-            }
-            WAVE_SYNC();
-            T v = lane <= km ? colbuf[lane] : T{R(0), R(0)};
-            R key = R(-2);
-            if (lane <= km) {
-                const R a = fabs(v.x) + fabs(v.y);
-                key = (a != a) ? (lane == 0 ? R(__builtin_inff()) : R(-1)) : a;
-            }
-            int bidx = lane;
-            argmax_cabs1(key, bidx);
-            const int jp = __builtin_amdgcn_readfirstlane(bidx);
-            const T piv = colbuf[jp];
-            if (lane == 0) ipiv[j] = j + jp + 1;
-            const bool nz = sz::cnonzero(piv);
-            if (nz) {
-                ju = max(ju, min(j + ku + jp, n - 1));
-                const bool inr = colk >= j && colk <= ju;
-                if (jp != 0) {
-                    const int sp = j0 + jp >= WR ? j0 + jp - WR : j0 + jp;
-                    T a0, a1;
-                    W_GET(j0, a0);
-                    W_GET(sp, a1);
-                    if (inr) { W_SET(j0, a1); W_SET(sp, a0); }
-                    if (ntail > 0 && ju >= j + 64) {
-                        if (lane < ntail) {
-                            const int k = j + 64 + lane;
-                            if (k <= ju) {
-                                T *tc = tail + tix(k);
-                                const T t0 = tc[j0], t1 = tc[sp];
-                                tc[j0] = t1;
-                                tc[sp] = t0;
-                            }
-                        }
-                        WAVE_SYNC();
-                    }
-                    const T c0 = colbuf[0];
-                    if (lane == 0) v = piv;
-                    else if (lane == jp) v = c0;
-                }
-                if (km > 0) {
-                    const T rcp = sz::crcp(piv);
-                    const T l = sz::cmul(v, rcp);
-                    if (lane >= 1 && lane <= km) {
-                        lbuf[lane] = l;
-                        ab[(size_t)j * ldab + kv + lane] = l;
-                    }
-                    WAVE_SYNC();
-                    const bool upd = colk > j && colk <= ju;
-                    T uc;
-                    W_GET(j0, uc);
-                    // lanes outside (j, ju] skip the whole loop (exec mask);
-                    // inside, the per-slot row test is uniform
-                    if (upd) {
-#pragma unroll
-                        for (int s = 0; s < WR; ++s) {
-                            int r = s - j0;
-                            if (r < 0) r += WR;
-                            if (r >= 1 && r <= km) {
-                                const T lr = lbuf[r];
-                                T a;
-                                W_GET(s, a);
-                                a = sz::cfnms(a, lr, uc);
-                                W_SET(s, a);
-                            }
-                        }
-                    }
-                    if (ntail > 0 && ju >= j + 64) {
-                        for (int e = lane; e < ntail * WR; e += 64) {
-                            const int t = e / WR, s = e - t * WR;
-                            const int k = j + 64 + t;
-                            int r = s - j0;
-                            if (r < 0) r += WR;
-                            if (k <= ju && r >= 1 && r <= km) {
-                                T *tc = tail + tix(k);
-                                tc[s] = sz::cfnms(tc[s], lbuf[r], tc[j0]);
-                            }
-                        }
-                        WAVE_SYNC();
-                    }
-                }
-            } else {
-                if (first_zero == 0) first_zero = j + 1;
-                if (lane >= 1 && lane <= km) ab[(size_t)j * ldab + kv + lane] = v;
-            }
-            // ---- U row j: slot j0 of every column j..min(j+kv, n-1)
-            {
-                T uj;
-                W_GET(j0, uj);
-                if (colk >= j && colk <= j + kv && colk < n)
-                    ab[(size_t)colk * ldab + kv + j - colk] = uj;
-                if (lane < ntail) {
-                    const int k = j + 64 + lane;
-                    if (k <= j + kv && k < n)
-                        ab[(size_t)k * ldab + kv + j - k] = tail[tix(k) + j0];
-                }
-            }
-            // ---- retire column j: lane pl takes column j+64
-            if (lane == pl) {
-                colk += 64;
-                if (ntail > 0) {
-                    const T *tc = tail + tix(j + 64);
-#pragma unroll
-                    for (int s = 0; s < WR; ++s) {
-                        const T t = tc[s];
-                        W_SET(s, t);
-                    }
-                } else {
-                    r0 = R(0); r1 = R(0); r2 = R(0); r3 = R(0); r4 = R(0);
-                    i0 = R(0); i1 = R(0); i2 = R(0); i3 = R(0); i4 = R(0);
-                }
-            }
-            if (ntail > 0) {
-                WAVE_SYNC();
-                // tail slot of column j+64 now belongs to column j+kv+1
-                if (lane < WR) tail[tix(j + 64) + lane] = T{R(0), R(0)};
-                WAVE_SYNC();
-            }
-            // ---- entering row j+WR into slot j0 (prefetched W64_PF ago)
-            {
-                const int i = j + WR;
-                const bool irow = i < n;
-                T qv = q[u];
-                if (!(irow && colk < n && colk - i <= ku)) qv = T{R(0), R(0)};
-                W_SET(j0, qv);
-                if (lane < ntail) {
-                    const int k = j + 65 + lane;
-                    T tv = qt[u];
-                    if (!(irow && k < n && k - i <= ku)) tv = T{R(0), R(0)};
-                    tail[tix(k) + j0] = tv;
-                }
-                // refill this queue slot for step j + W64_PF
-                const int jn = j + W64_PF;
-                q[u] = band(jn + WR, lane_col_at(jn));
-                qt[u] = lane < ntail ? band(jn + WR, jn + 65 + lane) : T{R(0), R(0)};
-            }
-            if (ntail > 0) WAVE_SYNC();
-            j0 = j0 + 1 == WR ? 0 : j0 + 1;
-            }
-        }
-    }
-    if (lane == 0 && info) info[sys] = first_zero;
-}
-
-// ---------------------------------------------------------------- col4
-// The window in registers, four threads per column slot: thread (cs, k)
-// owns row slots k, k+4, k+8, ... of column slot cs (row i sits in slot
-// i mod (kl+1), column c in slot c mod (kl+ku+1)), so every register index is
-// a compile-time constant.  Per step only the pivot column (to wave 0), the
-// multipliers, the new pivot row and the displaced row j cross LDS; wave 0
-// searches the pivot and forms all multipliers at once, one per lane.  Each
-// element receives exactly the fnms sequence of the unblocked loop.
-constexpr int COL4_TPC = 4;
-
-template <typename R, int MR, int NT>
-__global__ __launch_bounds__(NT)
-void k_gbtrf_col4(int n, int kl, int ku, cpx<R> *__restrict__ ab, int ldab,
-                  long long sab, int *__restrict__ ipiv, long long sipiv,
-                  int *__restrict__ info, const int *__restrict__ skip,
-                  int nsys)
-{
-    using T = cpx<R>;
-    constexpr int TPC = COL4_TPC;
-    __shared__ T colbuf[TPC * MR];
-    __shared__ T lbuf[TPC * MR];
-    __shared__ T ubuf[NT / TPC];
-    __shared__ T swp[NT / TPC];
-    __shared__ int s_jp, s_nz;
-
-    const int sys = blockIdx.x;
-    if (sys >= nsys) return;
-    if (skip && skip[sys] != 0) return;
-    ab += (size_t)sys * (size_t)sab;
-    ipiv += (size_t)sys * (size_t)sipiv;
-    const int tid = threadIdx.x;
-    const int cs = tid / TPC, k = tid - cs * TPC;
-    const int kv = kl + ku;
-    const int WR = kl + 1, WC = kv + 1;
-    const bool active = cs < WC;
-    auto band = [&](int i, int c) -> T {
-        const bool ok = i < n && c < n && c - i <= ku && i - c <= kl;
-        const T *p = ab + (ok ? (size_t)c * ldab + kv + i - c : 0);
-        const R x = p->x, y = p->y;
-        return T{ok ? x : R(0), ok ? y : R(0)};
-    };
-
-    T a[MR];
-#pragma unroll
-    for (int m = 0; m < MR; ++m) {
-        const int s = k + TPC * m;
-        a[m] = (active && s < WR) ? band(s, cs) : T{R(0), R(0)};
-    }
-
-    // entering rows: at the end of step j slot j mod WR receives row j+WR;
-    // this thread receives when (j mod WR) mod TPC == k
-    auto colend = [&](int j) {          // this slot's column after step j
-        const int d = (cs - j % WC + WC) % WC;
-        return d == 0 ? j + WC : j + d;
-    };
-    auto next_recv = [&](int j) {       // first step > j this thread receives
-        int t = 1;
-        while ((((j + t) % WR) % TPC) != k) ++t;
-        return j + t;
-    };
-    int nrecv = k < WR ? next_recv(k - 1) : n + WR;
-    T nval = (active && nrecv < n) ? band(nrecv + WR, colend(nrecv)) : T{R(0), R(0)};
-
-    int c0 = 0, j0 = 0, ju = 0, first_zero = 0;
-    for (int j = 0; j < n; ++j) {
-        const int km = min(kl, n - 1 - j);
-        const int d = active ? (cs - c0 + WC) % WC : WC;
-        const int m0 = j0 / TPC, k0 = j0 - m0 * TPC;
-        // (1) pivot column to LDS by logical row
-        if (d == 0) {
-#pragma unroll
-            for (int m = 0; m < MR; ++m) {
-                const int s = k + TPC * m;
-                if (s < WR) {
-                    int r = s - j0;
-                    if (r < 0) r += WR;
-                    colbuf[r] = a[m];
-                }
-            }
-        }
-        LDS_BARRIER();
-        // (2) wave 0: IZAMAX, multipliers, pivots
-        if (tid < 64) {
-            T v = tid <= km ? colbuf[tid] : T{R(0), R(0)};
-            R key = R(-2);
-            if (tid <= km) {
-                const R av = fabs(v.x) + fabs(v.y);
-                key = (av != av) ? (tid == 0 ? R(__builtin_inff()) : R(-1)) : av;
-            }
-            int bidx = tid;
-            argmax_cabs1(key, bidx);
-            const int jp = __builtin_amdgcn_readfirstlane(bidx);
-            const T piv = colbuf[jp];
-            const bool nz = sz::cnonzero(piv);
-            if (tid == 0) {
-                s_jp = jp;
-                s_nz = nz;
-                ipiv[j] = j + jp + 1;
-            }
-            if (nz) {
-                if (tid == jp) v = colbuf[0];
-                const T l = sz::cmul(v, sz::crcp(piv));
-                if (tid >= 1 && tid <= km) {
-                    lbuf[tid] = l;
-                    ab[(size_t)j * ldab + kv + tid] = l;
-                }
-            } else if (tid >= 1 && tid <= km) {
-                ab[(size_t)j * ldab + kv + tid] = v;
-            }
-        }
-        LDS_BARRIER();
-        const int jp = s_jp;
-        const bool nz = s_nz != 0;
-        if (nz) ju = max(ju, min(j + ku + jp, n - 1));
-        else if (first_zero == 0) first_zero = j + 1;
-        const int sp = j0 + jp >= WR ? j0 + jp - WR : j0 + jp;
-        const int mp = sp / TPC, kp = sp - mp * TPC;
-        const bool swap = nz && jp != 0 && d <= ju - j;
-        // (3) new pivot row (old row jp) and displaced row j through LDS
-        if (active && k == kp) {
-#pragma unroll
-            for (int m = 0; m < MR; ++m)
-                if (m == mp) ubuf[cs] = a[m];
-        }
-        if (swap && k == k0) {
-#pragma unroll
-            for (int m = 0; m < MR; ++m)
-                if (m == m0) swp[cs] = a[m];
-        }
-        LDS_BARRIER();
-        if (swap) {
-            if (k == k0) {
-                const T t = ubuf[cs];
-#pragma unroll
-                for (int m = 0; m < MR; ++m)
-                    if (m == m0) a[m] = t;
-            }
-            if (k == kp) {
-                const T t = swp[cs];
-#pragma unroll
-                for (int m = 0; m < MR; ++m)
-                    if (m == mp) a[m] = t;
-            }
-        }
-        // (4) U row j of columns j .. j+kv, then the rank-1 update
-        if (active && k == k0) {
-            const int cg = j + d;
-            if (cg <= j + kv && cg < n) {
-#pragma unroll
-                for (int m = 0; m < MR; ++m)
-                    if (m == m0) ab[(size_t)cg * ldab + kv + j - cg] = a[m];
-            }
-        }
-        if (nz && km > 0 && d >= 1 && d <= ju - j) {
-            const T u = ubuf[cs];
-#pragma unroll
-            for (int m = 0; m < MR; ++m) {
-                const int s = k + TPC * m;
-                if (s < WR) {
-                    int r = s - j0;
-                    if (r < 0) r += WR;
-                    if (r >= 1 && r <= km) a[m] = sz::cfnms(a[m], lbuf[r], u);
-                }
-            }
-        }
-        // (5) column j leaves (its slot becomes column j+kv+1, zero fill);
-        // row j leaves (its slot receives row j+kl+1)
-        if (d == 0) {
-#pragma unroll
-            for (int m = 0; m < MR; ++m) a[m] = T{R(0), R(0)};
-        }
-        if (j == nrecv) {
-            if (active) {
-#pragma unroll
-                for (int m = 0; m < MR; ++m)
-                    if (m == m0) a[m] = nval;
-            }
-            nrecv = next_recv(j);
-            nval = (active && nrecv < n) ? band(nrecv + WR, colend(nrecv))
-                                         : T{R(0), R(0)};
-        }
-        c0 = c0 + 1 == WC ? 0 : c0 + 1;
-        j0 = j0 + 1 == WR ? 0 : j0 + 1;
-    }
-    if (tid == 0 && info) info[sys] = first_zero;
-}
-
-// ---------------------------------------------------------------- quad
-// One barrier per elimination step.  The window sits in registers as in
-// k_gbtrf_col4 (quad cs = threads 4cs..4cs+3 owns column slot cs; lane k of
-// the quad owns row slots k, k+4, ...), slots 0..7 of each component in a
-// named 8-vector and slot 8 in a scalar so that a uniform slot index selects
-// with a gpr-indexed move.  Step j:
-//   A  the quad holding column j (one quad, no other thread involved) finds
-//      the pivot with a local search and two quad DPP rounds, forms every
-//      multiplier, publishes them with the pivot index and value in LDS
-//      (double-buffered by step parity) and clears its column slot, which
-//      becomes the fill-in column j+kl+ku+1;
-//   -  barrier;
-//   B  every quad takes the pivot-row and row-j values of its column by DPP
-//      broadcast inside the quad, swaps, stores its U(j, .) element, one
-//      multiplier and the pivot index (unconditional stores; threads with
-//      nothing to write store U(j,j) again), applies the rank-1 update and
-//      receives the entering row, prefetched QD_PF steps ahead by every
-//      thread of the quad.
-// Every element sees the fnms sequence of the unblocked loop.  Rows past n
-// get zero multipliers and padding slots are never read as window rows, so
-// the update itself needs no per-slot predicate; slot j of lane k0 is
-// updated with a meaningless multiplier and then overwritten by the
-// entering row.
-constexpr int QD_PF = 4;
 
 template <int CTRL, typename R>
 __device__ __forceinline__ R dpp_r(R v)
@@ -1273,12 +474,6 @@ __device__ __forceinline__ R dpp_r(R v)
     }
 }
 
-template <int CTRL>
-__device__ __forceinline__ int dpp_i(int v)
-{
-    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
-}
-
 template <typename R>
 __device__ __forceinline__ R readlane_r(R v, int lane)
 {
@@ -1294,257 +489,299 @@ __device__ __forceinline__ R readlane_r(R v, int lane)
     }
 }
 
-// value held by lane kq (uniform) of the caller's quad
-template <typename R>
-__device__ __forceinline__ cpx<R> quad_bcast(cpx<R> v, int kq)
-{
-    cpx<R> o;
-    switch (kq) {
-    case 0: o.x = dpp_r<0x00>(v.x); o.y = dpp_r<0x00>(v.y); break;
-    case 1: o.x = dpp_r<0x55>(v.x); o.y = dpp_r<0x55>(v.y); break;
-    case 2: o.x = dpp_r<0xAA>(v.x); o.y = dpp_r<0xAA>(v.y); break;
-    default: o.x = dpp_r<0xFF>(v.x); o.y = dpp_r<0xFF>(v.y); break;
-    }
-    return o;
-}
+// ---------------------------------------------------------------- wave
+// One wavefront per system, no barriers.  Lane l holds the window column
+// j + ((l - j) mod 64) as WR = kl+1 row slots in registers.  Steps run in
+// blocks of WV_BS, unrolled: at the start of a block slot s holds row
+// jb+s, row jb+u leaves slot u at step u and row jb+u+WR enters it, and at
+// the end of the block the slots are renamed by WV_BS so the next block
+// starts aligned again.  Every register slot index in a step is therefore a
+// compile-time constant (the rank-1 update, the pivot-column transpose, U
+// row j, the entering row, the column retire); only the pivot row of an
+// interchange is data dependent, and that path selects over the slots.
+// Columns j+64 .. j+kl+ku (kl+ku >= 64) live in an LDS tail, column j+64+t
+// in tail slot (th+t) mod NTAIL, row i at i mod WR (stored twice, so a
+// column reads back in register order from one base address).  Step j:
+//   * the lane holding column j writes it to LDS, lane r <= km reads row
+//     j+r and the pivot is the first lane whose |Re|+|Im| equals the DPP and
+//     readlane maximum (a NaN in row j wins, any other NaN never does);
+//   * lane r writes multiplier r to LDS at row j+r's slot (zero for row j and
+//     rows past n), so the update is one static loop without predicates;
+//   * every lane stores U(j, its column), U(j, a tail column), L(lane, j)
+//     and the pivot index, and issues two prefetch loads, all unconditional
+//     (lanes with nothing to write store U(j,j) again), so outstanding
+//     memory operations are counted exactly.
+// Every updated element sees the fnms sequence of the unblocked loop.
+constexpr int WV_PF = 2;
 
-#define Q_GET(M, V)                                                           \
-    do {                                                                      \
-        const int m_ = (M);                                                   \
-        if (MR <= 8 || m_ < 8) { V.x = xr[m_ & 7]; V.y = xi[m_ & 7]; }        \
-        else { V.x = xr8; V.y = xi8; }                                        \
-    } while (0)
-#define Q_SET(M, V)                                                           \
-    do {                                                                      \
-        const int m_ = (M);                                                   \
-        if (MR <= 8 || m_ < 8) { xr[m_ & 7] = V.x; xi[m_ & 7] = V.y; }        \
-        else { xr8 = V.x; xi8 = V.y; }                                        \
-    } while (0)
-
-template <typename R, int MR, int NT>
-__global__ __launch_bounds__(NT)
-void k_gbtrf_quad(int n, int kl, int ku, cpx<R> *__restrict__ ab, int ldab,
-                  long long sab, int *__restrict__ ipiv, long long sipiv,
+template <typename R, int WR, int KV, int WV_BS>
+__global__ __launch_bounds__(64)
+void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
+                  int *__restrict__ ipiv, long long sipiv,
                   int *__restrict__ info, const int *__restrict__ skip,
                   int nsys)
 {
     using T = cpx<R>;
-    static_assert(MR >= 1 && MR <= 9, "at most 36 row slots");
-    __shared__ T lbuf[2][64];
-    __shared__ T s_piv[2];
-    __shared__ int s_jp[2], s_nz[2];
+    constexpr int kl = WR - 1, ku = KV - kl;
+    constexpr int NTAIL = KV >= 64 ? KV - 63 : 0;
+    constexpr int TW = 2 * WR;                     // tail column stride
+    static_assert(WR > WV_BS && WR <= 40 && WV_BS % WV_PF == 0, "row slots");
+    static_assert(KV <= 127 && ku >= 0, "window");
+    __shared__ T colbuf[WR];
+    __shared__ T lbuf[WR];
+    __shared__ T tail[NTAIL > 0 ? NTAIL * TW : 1];
 
     const int sys = blockIdx.x;
     if (sys >= nsys) return;
     if (skip && skip[sys] != 0) return;
     ab += (size_t)sys * (size_t)sab;
     ipiv += (size_t)sys * (size_t)sipiv;
-    const int tid = threadIdx.x;
-    const int cs = tid >> 2, k = tid & 3;
-    const int kv = kl + ku;
-    const int WR = kl + 1, WC = kv + 1;
-    const bool active = cs < WC;
+    const int lane = threadIdx.x;
+    const T zero{R(0), R(0)};
 
-    typedef R v8 __attribute__((ext_vector_type(8)));
-    v8 xr = R(0), xi = R(0);
-    R xr8 = R(0), xi8 = R(0);
-#pragma unroll
-    for (int m = 0; m < MR; ++m) {
-        const int s = k + 4 * m;
-        const bool ok = active && s < WR && s < n && cs < n && cs - s <= ku;
-        const T *p = ab + (ok ? (size_t)cs * ldab + kv + s - cs : 0);
-        const R x = p->x, y = p->y;
-        T v;
-        v.x = ok ? x : R(0);
-        v.y = ok ? y : R(0);
-        Q_SET(m, v);
-    }
-    // raw entering-row elements (clamped address; zeroed when consumed):
-    // at the end of step jj the slot of row jj receives row jj+WR, whose
-    // element in this column slot is column jj + (d ? d : WC), d the slot's
-    // offset from column jj
-    auto enter_raw = [&](int jj, int dd) -> T {
-        const int c = jj + (dd == 0 ? WC : dd), i = jj + WR;
-        const bool ok = active && i < n && c < n;
-        return ab[ok ? (size_t)c * ldab + kv + i - c : 0];
+    // component-wise select (a select of whole structs goes through memory)
+    auto sel = [](bool c, T x, T y) __attribute__((always_inline)) -> T {
+        T o;
+        o.x = c ? x.x : y.x;
+        o.y = c ? x.y : y.y;
+        return o;
     };
-    T qv[QD_PF];
-#pragma unroll
-    for (int u = 0; u < QD_PF; ++u) {
-        int dd = cs - u % WC;
-        if (dd < 0) dd += WC;
-        qv[u] = enter_raw(u, dd);
-    }
-    const int rl = kl > 0 ? 1 + tid % kl : 0;   // multiplier row this thread stores
+    auto band = [&](int i, int c) __attribute__((always_inline)) -> T {
+        const bool ok = i < n && c < n && c - i <= ku && i - c <= kl;
+        const T *p = ab + (ok ? (size_t)c * ldab + KV + i - c : 0);
+        const R x = p->x, y = p->y;
+        return T{ok ? x : R(0), ok ? y : R(0)};
+    };
+    // raw element (clamped address; validity is applied when consumed)
+    auto raw = [&](int i, int c) __attribute__((always_inline)) -> T {
+        const bool ok = i < n && c < n;
+        return ab[ok ? (size_t)c * ldab + KV + i - c : 0];
+    };
 
-    int c0 = 0, j0 = 0, ju = 0, first_zero = 0;
-    for (int jb = 0; jb < n; jb += QD_PF) {
+    T a[WR];
 #pragma unroll
-        for (int u = 0; u < QD_PF; ++u) {
-            const int j = jb + u;
-            if (j >= n) break;
-            const int p = u & 1;                    // QD_PF is even
-            const int km = min(kl, n - 1 - j);
-            const int m0 = j0 >> 2, k0 = j0 & 3;
-            int d = cs - c0;
-            if (d < 0) d += WC;
-            if (!active) d = WC;
-
-            // ---- A: pivot column
-            if (d == 0) {
-                R key[MR];
-                R best = R(-1);
-#pragma unroll
-                for (int m = 0; m < MR; ++m) {
-                    const int s = k + 4 * m;
-                    int r = s - j0;
-                    if (r < 0) r += WR;
-                    T v;
-                    Q_GET(m, v);
-                    const R a = fabs(v.x) + fabs(v.y);
-                    key[m] = (s < WR && r <= km) ? a : R(-1);
-                    best = fmax(best, key[m]);        // NaN keys never win here
-                }
-                best = fmax(best, dpp_r<0xB1>(best));
-                best = fmax(best, dpp_r<0x4E>(best));
-                int rb = 64;
-#pragma unroll
-                for (int m = 0; m < MR; ++m) {
-                    const int s = k + 4 * m;
-                    int r = s - j0;
-                    if (r < 0) r += WR;
-                    rb = key[m] == best ? min(rb, r) : rb;
-                }
-                rb = min(rb, dpp_i<0xB1>(rb));
-                rb = min(rb, dpp_i<0x4E>(rb));
-                const int L0 = (4 * c0) & 63;
-                T rj, t;
-                Q_GET(m0, t);
-                rj.x = readlane_r(t.x, L0 + k0);
-                rj.y = readlane_r(t.y, L0 + k0);
-                int jp = __builtin_amdgcn_readfirstlane(rb);
-                const R a0 = fabs(rj.x) + fabs(rj.y);
-                if (a0 != a0) jp = 0;                 // a NaN in row j stays
-                const int sp = j0 + jp >= WR ? j0 + jp - WR : j0 + jp;
-                T piv;
-                Q_GET(sp >> 2, t);
-                piv.x = readlane_r(t.x, L0 + (sp & 3));
-                piv.y = readlane_r(t.y, L0 + (sp & 3));
-                const bool nz = sz::cnonzero(piv);
-                if (nz) {
-                    const T rcp = sz::crcp(piv);
-#pragma unroll
-                    for (int m = 0; m < MR; ++m) {
-                        const int s = k + 4 * m;
-                        int r = s - j0;
-                        if (r < 0) r += WR;
-                        T v;
-                        Q_GET(m, v);
-                        const T l = sz::cmul(v, rcp);
-                        if (s < WR) lbuf[p][r] = r <= km ? l : T{R(0), R(0)};
-                    }
-                    // row jp now holds old row j: same wave, written last
-                    if (jp != 0 && k == 0) lbuf[p][jp] = sz::cmul(rj, rcp);
-                } else {
-#pragma unroll
-                    for (int m = 0; m < MR; ++m) {
-                        const int s = k + 4 * m;
-                        int r = s - j0;
-                        if (r < 0) r += WR;
-                        T v;
-                        Q_GET(m, v);
-                        if (s < WR) lbuf[p][r] = r <= km ? v : T{R(0), R(0)};
-                    }
-                }
-                if (k == 0) {
-                    s_jp[p] = jp;
-                    s_nz[p] = nz ? 1 : 0;
-                    s_piv[p] = piv;
-                }
-                xr = R(0);
-                xi = R(0);
-                xr8 = R(0);
-                xi8 = R(0);
-            }
-            LDS_BARRIER();
-
-            // ---- B: swap, stores, update, entering row
-            const int jp = __builtin_amdgcn_readfirstlane(s_jp[p]);
-            const bool nz = __builtin_amdgcn_readfirstlane(s_nz[p]) != 0;
-            const T piv = s_piv[p];
-            if (nz) ju = max(ju, min(j + ku + jp, n - 1));
-            else if (first_zero == 0) first_zero = j + 1;
-            const int sp = j0 + jp >= WR ? j0 + jp - WR : j0 + jp;
-            const int mp = sp >> 2, kp = sp & 3;
-            T t;
-            Q_GET(mp, t);
-            const T pv = quad_bcast(t, kp);           // pivot row, this column
-            T uval = pv;
-            const bool inr = d <= ju - j;
-            if (jp != 0) {
-                Q_GET(m0, t);
-                const T rj = quad_bcast(t, k0);       // row j, this column
-                if (inr) {
-                    if (k == k0) Q_SET(m0, pv);
-                    if (k == kp) Q_SET(mp, rj);
-                }
-                uval.x = inr ? pv.x : rj.x;
-                uval.y = inr ? pv.y : rj.y;
-            }
-            {
-                const int cg = j + d;
-                const bool ok = d > 0 && active && cg < n;
-                const size_t ujj = (size_t)j * ldab + kv;
-                T v;
-                v.x = ok ? uval.x : piv.x;
-                v.y = ok ? uval.y : piv.y;
-                ab[ok ? (size_t)cg * ldab + kv - d : ujj] = v;
-                const bool okl = rl >= 1 && rl <= km;
-                const T lv = lbuf[p][okl ? rl : 0];
-                T w;
-                w.x = okl ? lv.x : piv.x;
-                w.y = okl ? lv.y : piv.y;
-                ab[okl ? ujj + rl : ujj] = w;
-                ipiv[j] = j + jp + 1;
-            }
-            if (nz && km > 0 && d >= 1 && d <= ju - j) {
-#pragma unroll
-                for (int m = 0; m < MR; ++m) {
-                    const int s = k + 4 * m;
-                    int r = s - j0;
-                    if (r < 0) r += WR;
-                    const T l = lbuf[p][r];
-                    T a;
-                    Q_GET(m, a);
-                    a = sz::cfnms(a, l, pv);
-                    Q_SET(m, a);
-                }
-            }
-            {
-                const int ce = j + (d == 0 ? WC : d);
-                const bool ok = active && j + WR < n && ce < n;
-                T e = qv[u];
-                e.x = ok ? e.x : R(0);
-                e.y = ok ? e.y : R(0);
-                if (k == k0) Q_SET(m0, e);
-                int dd = d - QD_PF;
-                while (dd < 0) dd += WC;
-                qv[u] = enter_raw(j + QD_PF, dd);
-            }
-            c0 = c0 + 1 == WC ? 0 : c0 + 1;
-            j0 = j0 + 1 == WR ? 0 : j0 + 1;
+    for (int s = 0; s < WR; ++s) a[s] = band(s, lane);
+    if constexpr (NTAIL > 0) {
+        for (int e = lane; e < NTAIL * WR; e += 64) {
+            const int t = e / WR, p = e - t * WR;
+            const T v = band(p, 64 + t);
+            tail[t * TW + p] = v;
+            tail[t * TW + p + WR] = v;
         }
     }
-    if (tid == 0 && info) info[sys] = first_zero;
-}
-#undef Q_GET
-#undef Q_SET
+    // entering row u+WR: this lane's column after step u, and tail column
+    // u+65+lane
+    T qv[WV_PF], qt[WV_PF];
+#pragma unroll
+    for (int u = 0; u < WV_PF; ++u) {
+        qv[u] = raw(u + WR, u + 1 + ((lane - (u + 1)) & 63));
+        qt[u] = raw(u + WR, lane < NTAIL ? u + 65 + lane : n);
+    }
+    WAVE_SYNC();
 
-// SZ_GBTRF_KERNEL selects the factorization variant.  Default: the LDS
-// window on 256 threads ("lds64"/"lds128" use fewer threads); "reg" and
-// "reg128" select the register-tiled kernel (5x2 tiles on 256 threads or 7x3
-// on 128), which measures slower on gfx950 because its per-step selects and
-// index arithmetic outweigh the saved LDS traffic.
+    int c0 = 0, th = 0, ju = 0, first_zero = 0;
+    int ob = 0;                                      // jb mod WR
+    for (int jb = 0; jb < n; jb += WV_BS) {
+#pragma unroll
+        for (int u = 0; u < WV_BS; ++u) {
+            const int j = jb + u;
+            if (j >= n) break;
+            const int j0 = ob + u >= WR ? ob + u - WR : ob + u;   // j mod WR
+            const int km = min(kl, n - 1 - j);
+            const int d = (lane - c0) & 63;          // this lane: column j + d
+            const size_t ujj = (size_t)j * ldab + KV;
+
+            // ---- pivot search over column j
+            if (d == 0) {
+#pragma unroll
+                for (int s = 0; s < WR; ++s) colbuf[s] = a[s];
+            }
+            WAVE_SYNC();
+            int ps = u + lane;                        // slot of row j+lane
+            if (ps >= WR) ps -= WR;
+            const T v = sel(lane < WR, colbuf[lane < WR ? ps : 0], zero);
+            R key = R(-1);
+            {
+                const R m1 = fabs(v.x) + fabs(v.y);
+                if (lane <= km) key = (m1 != m1) ? (lane == 0 ? R(__builtin_inff()) : R(-1)) : m1;
+            }
+            R m = key;
+            m = fmax(m, dpp_r<0xB1>(m));             // quad_perm [1,0,3,2]
+            m = fmax(m, dpp_r<0x4E>(m));             // quad_perm [2,3,0,1]
+            m = fmax(m, dpp_r<0x124>(m));            // row_ror:4
+            m = fmax(m, dpp_r<0x128>(m));            // row_ror:8
+            const R mx = fmax(fmax(readlane_r(m, 0), readlane_r(m, 16)),
+                              fmax(readlane_r(m, 32), readlane_r(m, 48)));
+            const unsigned long long ball = __ballot(key == mx);
+            const int jp = (int)__builtin_ctzll(ball);
+            T piv, v0;
+            piv.x = readlane_r(v.x, jp);
+            piv.y = readlane_r(v.y, jp);
+            v0.x = readlane_r(v.x, 0);
+            v0.y = readlane_r(v.y, 0);
+            const bool nz = sz::cnonzero(piv);
+            if (nz) ju = max(ju, min(j + ku + jp, n - 1));
+            else if (first_zero == 0) first_zero = j + 1;
+            ipiv[j] = j + jp + 1;
+
+            // ---- multipliers (row jp now holds old row j)
+            T l = v;
+            if (nz) {
+                const T rcp = sz::crcp(piv);
+                T w = v;
+                if (lane == jp) w = v0;
+                l = sz::cmul(w, rcp);
+            }
+            const bool isl = lane >= 1 && lane <= km;
+            if (lane < WR) lbuf[ps] = sel(isl, l, zero);
+            {
+                T st;
+                st.x = isl ? l.x : piv.x;
+                st.y = isl ? l.y : piv.y;
+                ab[isl ? ujj + lane : ujj] = st;
+            }
+
+            // ---- interchange rows j and j+jp in columns j .. ju
+            T uc = a[u];
+            if (nz && jp != 0) {
+                const bool inr = d <= ju - j;
+                const int sp = u + jp >= WR ? u + jp - WR : u + jp;
+                T rp = uc;
+#pragma unroll
+                for (int s = 0; s < WR; ++s) {
+                    if (s != u) {
+                        rp.x = s == sp ? a[s].x : rp.x;
+                        rp.y = s == sp ? a[s].y : rp.y;
+                        a[s].x = (s == sp && inr) ? uc.x : a[s].x;
+                        a[s].y = (s == sp && inr) ? uc.y : a[s].y;
+                    }
+                }
+                uc.x = inr ? rp.x : uc.x;
+                uc.y = inr ? rp.y : uc.y;
+                a[u] = uc;
+                if constexpr (NTAIL > 0) {
+                    if (ju >= j + 64 && lane < NTAIL && j + 64 + lane <= ju) {
+                        const int pp = j0 + jp >= WR ? j0 + jp - WR : j0 + jp;
+                        int ts = th + lane;
+                        if (ts >= NTAIL) ts -= NTAIL;
+                        T *tc = tail + ts * TW;
+                        const T t0 = tc[j0], t1 = tc[pp];
+                        tc[j0] = t1;
+                        tc[j0 + WR] = t1;
+                        tc[pp] = t0;
+                        tc[pp + WR] = t0;
+                    }
+                }
+            }
+            WAVE_SYNC();
+
+            // ---- U row j
+            {
+                const int cg = j + d;
+                const bool ok = d > 0 && d <= KV && cg < n;
+                T st;
+                st.x = ok ? uc.x : piv.x;
+                st.y = ok ? uc.y : piv.y;
+                ab[ok ? (size_t)cg * ldab + KV - d : ujj] = st;
+            }
+            if constexpr (NTAIL > 0) {
+                int ts = th + lane;
+                if (ts >= NTAIL) ts -= NTAIL;
+                const bool ok = lane < NTAIL && j + 64 + lane < n;
+                const T tv = tail[(lane < NTAIL ? ts : 0) * TW + j0];
+                T st;
+                st.x = ok ? tv.x : piv.x;
+                st.y = ok ? tv.y : piv.y;
+                ab[ok ? (size_t)(j + 64 + lane) * ldab + KV - 64 - lane : ujj] = st;
+            }
+
+            // ---- rank-1 update of rows j+1 .. j+km, columns j+1 .. ju
+            if (nz && km > 0) {
+                if (d >= 1 && d <= ju - j) {
+#pragma unroll
+                    for (int s = 0; s < WR; ++s) a[s] = sz::cfnms(a[s], lbuf[s], uc);
+                }
+                if constexpr (NTAIL > 0) {
+                    if (ju >= j + 64 && lane < NTAIL && j + 64 + lane <= ju) {
+                        int ts = th + lane;
+                        if (ts >= NTAIL) ts -= NTAIL;
+                        T *tc = tail + ts * TW;
+                        const T ut = tc[j0];
+                        for (int p = 0; p < WR; ++p) {
+                            int s = p - ob;                 // register slot of row p
+                            if (s < 0) s += WR;
+                            const T x = sz::cfnms(tc[p], lbuf[s], ut);
+                            tc[p] = x;
+                            tc[p + WR] = x;
+                        }
+                    }
+                }
+            }
+            WAVE_SYNC();
+
+            // ---- column j leaves: its lane takes column j+64 (the tail's
+            // first column, or zeros); row j leaves: slot u takes row j+WR
+            if (d == 0) {
+                if constexpr (NTAIL > 0) {
+                    const T *tc = tail + th * TW + ob;   // register order
+#pragma unroll
+                    for (int s = 0; s < WR; ++s) a[s] = tc[s];
+                } else {
+#pragma unroll
+                    for (int s = 0; s < WR; ++s) a[s] = zero;
+                }
+            }
+            {
+                const int dn = (lane - c0 - 1) & 63;   // offset from column j+1
+                const bool ok = j + WR < n && j + 1 + dn < n && dn <= KV;
+                const T e = qv[u % WV_PF];
+                a[u].x = ok ? e.x : R(0);
+                a[u].y = ok ? e.y : R(0);
+                qv[u % WV_PF] = raw(j + WV_PF + WR,
+                                    j + WV_PF + 1 + ((lane - c0 - WV_PF - 1) & 63));
+            }
+            if constexpr (NTAIL > 0) {
+                WAVE_SYNC();
+                if (lane < WR) {
+                    tail[th * TW + lane] = zero;
+                    tail[th * TW + lane + WR] = zero;
+                }
+                WAVE_SYNC();
+                const int th1 = th + 1 == NTAIL ? 0 : th + 1;
+                if (lane < NTAIL) {
+                    int ts = th1 + lane;
+                    if (ts >= NTAIL) ts -= NTAIL;
+                    const bool ok = j + WR < n && j + 65 + lane < n;
+                    T e = qt[u % WV_PF];
+                    e.x = ok ? e.x : R(0);
+                    e.y = ok ? e.y : R(0);
+                    tail[ts * TW + j0] = e;
+                    tail[ts * TW + j0 + WR] = e;
+                }
+                qt[u % WV_PF] = raw(j + WV_PF + WR,
+                                    lane < NTAIL ? j + WV_PF + 65 + lane : n);
+                th = th1;
+            }
+            WAVE_SYNC();
+            c0 = (c0 + 1) & 63;
+        }
+        // rename the slots: row jb+WV_BS+s moves to slot s
+        T t[WR];
+#pragma unroll
+        for (int s = 0; s < WR; ++s) t[s] = a[(s + WV_BS) % WR];
+#pragma unroll
+        for (int s = 0; s < WR; ++s) a[s] = t[s];
+        ob = ob + WV_BS >= WR ? ob + WV_BS - WR : ob + WV_BS;
+    }
+    if (lane == 0 && info) info[sys] = first_zero;
+}
+
+// Factorization kernels: k_gbtrf_wave (one wavefront per system, window in
+// registers) for the band shapes it is instantiated for -- kl = ku = 34
+// (five interleaved fields, k = 8 operators), kl = ku = 24 (k = 6) and
+// kl = 34, ku = 20 -- and the LDS window kernel k_gbtrf for every other
+// shape.  SZ_GBTRF_KERNEL: "lds" (or "lds64"/"lds128", fewer threads) forces
+// the LDS kernel; "v4"/"v2" run the wave kernel in shorter unrolled blocks.
 template <typename R>
 int launch_gbtrf(int n, int kl, int ku, void *ab, int ldab, long long sab,
                  int *ipiv, long long sipiv, int *info, const int *skip,
@@ -1559,64 +796,27 @@ int launch_gbtrf(int n, int kl, int ku, void *ab, int ldab, long long sab,
     if (nsys < 0) return -10;
     if (nsys == 0 || n == 0) return 0;
     static const char *sel = getenv("SZ_GBTRF_KERNEL");
-    if (sel && sel[0] == 'q' && kl + 1 <= 36) {
-        const int WC = kl + ku + 1;
-        const int nt = ((4 * WC + 63) / 64) * 64;
-#define SZ_QUAD(MR, NT_)                                                        \
-        hipLaunchKernelGGL((k_gbtrf_quad<R, MR, NT_>), dim3(nsys), dim3(NT_), 0,  \
-                           stream, n, kl, ku, reinterpret_cast<cpx<R> *>(ab),     \
-                           ldab, sab, ipiv, sipiv, info, skip, nsys)
-        if (nt <= 64) { if (kl + 1 <= 16) SZ_QUAD(4, 64); else SZ_QUAD(9, 64); }
-        else if (nt <= 128) { if (kl + 1 <= 28) SZ_QUAD(7, 128); else SZ_QUAD(9, 128); }
-        else if (nt <= 192) { if (kl + 1 <= 28) SZ_QUAD(7, 192); else SZ_QUAD(9, 192); }
-        else if (nt <= 256) { if (kl + 1 <= 28) SZ_QUAD(7, 256); else SZ_QUAD(9, 256); }
-        else if (nt <= 320) SZ_QUAD(9, 320);
-        else return SZ_ERR_LDS;
-#undef SZ_QUAD
-        return hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH;
-    }
-    if (sel && sel[0] == 'c' && kl + 1 <= COL4_TPC * 9) {
-        const int WC = kl + ku + 1;
-        const int nt = ((COL4_TPC * WC + 63) / 64) * 64;
-#define SZ_COL4(MR, NT_)                                                        \
-        hipLaunchKernelGGL((k_gbtrf_col4<R, MR, NT_>), dim3(nsys), dim3(NT_), 0,  \
-                           stream, n, kl, ku, reinterpret_cast<cpx<R> *>(ab),     \
-                           ldab, sab, ipiv, sipiv, info, skip, nsys)
-        if (nt <= 64) { if (kl + 1 <= 4 * 4) SZ_COL4(4, 64); else SZ_COL4(9, 64); }
-        else if (nt <= 128) { if (kl + 1 <= 4 * 7) SZ_COL4(7, 128); else SZ_COL4(9, 128); }
-        else if (nt <= 192) { if (kl + 1 <= 4 * 7) SZ_COL4(7, 192); else SZ_COL4(9, 192); }
-        else if (nt <= 256) { if (kl + 1 <= 4 * 7) SZ_COL4(7, 256); else SZ_COL4(9, 256); }
-        else if (nt <= 320) SZ_COL4(9, 320);
-        else return SZ_ERR_LDS;
-#undef SZ_COL4
-        return hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH;
-    }
-    if (sel && sel[0] == 'w' && (kl == 34 || kl == 24) && kl + ku + 1 <= 64 + 64) {
-        const int ntail = kl + ku >= 64 ? kl + ku - 63 : 0;
-        const size_t lds = (size_t)ntail * (kl + 1) * sizeof(cpx<R>);
-        if (kl == 34)
-            hipLaunchKernelGGL((k_gbtrf_w64<R, 35>), dim3(nsys), dim3(64), lds,
-                               stream, n, ku, reinterpret_cast<cpx<R> *>(ab), ldab,
-                               sab, ipiv, sipiv, info, skip, nsys);
-        else
-            hipLaunchKernelGGL((k_gbtrf_w64<R, 25>), dim3(nsys), dim3(64), lds,
-                               stream, n, ku, reinterpret_cast<cpx<R> *>(ab), ldab,
-                               sab, ipiv, sipiv, info, skip, nsys);
-        return hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH;
-    }
-    const bool want_lds = !(sel && sel[0] == 'r');
-    const bool want_128 = sel && sel[0] == 'r' && sel[3] == '1';
-    int rc = 0;
-    if (!want_lds) {
-        if (want_128 && try_reg<R, 7, 3, 128>(n, kl, ku, ab, ldab, sab, ipiv,
-                                              sipiv, info, skip, nsys, stream, &rc))
-            return rc;
-        if (try_reg<R, 5, 2, 256>(n, kl, ku, ab, ldab, sab, ipiv, sipiv, info,
-                                  skip, nsys, stream, &rc))
-            return rc;
-        if (try_reg<R, 8, 4, 256>(n, kl, ku, ab, ldab, sab, ipiv, sipiv, info,
-                                  skip, nsys, stream, &rc))
-            return rc;
+    if (!(sel && sel[0] == 'l')) {
+        const int bs = (sel && sel[0] == 'v' && sel[1] == '2') ? 2
+                     : (sel && sel[0] == 'v' && sel[1] == '4') ? 4 : 8;
+        bool done = true;
+#define SZ_WAVE(WR_, KV_, BS_)                                                  \
+        hipLaunchKernelGGL((k_gbtrf_wave<R, WR_, KV_, BS_>), dim3(nsys), dim3(64), 0, \
+                           stream, n, reinterpret_cast<cpx<R> *>(ab), ldab, sab, ipiv, \
+                           sipiv, info, skip, nsys)
+        if (kl == 34 && ku == 34) {
+            if (bs == 2) SZ_WAVE(35, 68, 2);
+            else if (bs == 4) SZ_WAVE(35, 68, 4);
+            else SZ_WAVE(35, 68, 8);
+        } else if (kl == 24 && ku == 24) {
+            SZ_WAVE(25, 48, 8);
+        } else if (kl == 34 && ku == 20) {
+            SZ_WAVE(35, 54, 8);
+        } else {
+            done = false;
+        }
+#undef SZ_WAVE
+        if (done) return hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH;
     }
     const size_t lds = ((size_t)(kl + 1) * (kl + ku + 1) + (kl + ku + 1) + (kl + 1))
                        * sizeof(cpx<R>);
