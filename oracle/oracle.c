@@ -123,7 +123,7 @@ int sz_or_zpack(int S, int n, int ihat, int jhat, int kl, int ku,
  *   1/z, a/b = Smith's algorithm
  *   dot products of the transposed solves: 64 strided partial sums
  *   (term t to partial (t-1) mod 64, increasing t) combined by a butterfly
- *   (w[l] += w[l ^ off] for off = 32, 16, ..., 1), result w[0].
+ *   (w[l] += w[l ^ off] for off = 1, 2, ..., 32), result w[0].
  */
 #define DEFINE_ZOPS(P, T, R, FMA, MK, ABS, RE, IM)                                    \
 static inline T P##fnms(T a, T b, T c)                                        \
@@ -171,7 +171,7 @@ static inline T P##cj(T a, int c) { return c ? MK(RE(a), -IM(a)) : a; } \
 static inline T P##tree(T *w)                                                 \
 {                                                                             \
     T t[64];                                                                  \
-    for (int off = 32; off >= 1; off >>= 1) {                                 \
+    for (int off = 1; off <= 32; off <<= 1) {                                 \
         for (int l = 0; l < 64; ++l)                                          \
             t[l] = MK(RE(w[l]) + RE(w[l ^ off]),                        \
                       IM(w[l]) + IM(w[l ^ off]));                       \

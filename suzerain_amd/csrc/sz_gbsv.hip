@@ -54,6 +54,60 @@ __device__ inline void argmax_cabs1(R &best, int &bidx)
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");       \
     } while (0)
 
+template <int CTRL, typename R>
+__device__ __forceinline__ R dpp_r(R v)
+{
+    if constexpr (sizeof(R) == 8) {
+        const long long b = __builtin_bit_cast(long long, v);
+        const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF,
+                                                   0xF, false);
+        return __builtin_bit_cast(R, (long long)(((unsigned long long)(unsigned)hi << 32)
+                                                 | (unsigned)lo));
+    } else {
+        return __builtin_bit_cast(
+            R, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF,
+                                           0xF, false));
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ R readlane_r(R v, int lane)
+{
+    if constexpr (sizeof(R) == 8) {
+        const long long b = __builtin_bit_cast(long long, v);
+        const int lo = __builtin_amdgcn_readlane((int)b, lane);
+        const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+        return __builtin_bit_cast(R, (long long)(((unsigned long long)(unsigned)hi << 32)
+                                                 | (unsigned)lo));
+    } else {
+        return __builtin_bit_cast(R, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v),
+                                                               lane));
+    }
+}
+
+// Sum over the 64 lanes as the butterfly w[l] += w[l ^ off], off = 1, 2, 4,
+// ..., 32, evaluated at lane 0 and returned to every lane.  After the
+// offsets 1 and 2 a quad holds one value, so the half-row mirror (lane
+// l ^ 7) delivers the l ^ 4 partner; after 4 the same holds for the row
+// mirror (l ^ 15) and 8; rows then combine as (r0 + r1) + (r2 + r3).
+template <typename R>
+__device__ inline R wave_sum_asc(R v)
+{
+    v += dpp_r<0xB1>(v);                    // quad_perm [1,0,3,2]
+    v += dpp_r<0x4E>(v);                    // quad_perm [2,3,0,1]
+    v += dpp_r<0x141>(v);                   // row_half_mirror
+    v += dpp_r<0x140>(v);                   // row_mirror
+    return (readlane_r(v, 0) + readlane_r(v, 16)) +
+           (readlane_r(v, 32) + readlane_r(v, 48));
+}
+
+template <typename R>
+__device__ inline cpx<R> wave_csum(cpx<R> v)
+{
+    return cpx<R>{wave_sum_asc(v.x), wave_sum_asc(v.y)};
+}
+
 constexpr int TRF_PF = 8;   // incoming rows prefetched ahead of their step
 
 template <typename R>
@@ -281,15 +335,6 @@ __device__ __forceinline__ void chunk_store(const R (&rx)[TRS_PRE],
     }
 }
 
-template <typename R>
-__device__ inline cpx<R> wave_csum(cpx<R> v)
-{
-    for (int off = 32; off >= 1; off >>= 1) {
-        v.x += __shfl_xor(v.x, off);
-        v.y += __shfl_xor(v.y, off);
-    }
-    return v;
-}
 
 template <typename R>
 __device__ inline cpx<R> cj_if(cpx<R> v, bool c)
@@ -456,38 +501,6 @@ void k_gbtrs(int trans, int n, int kl, int ku, int nrhs,
         __builtin_amdgcn_wave_barrier();                                      \
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");                \
     } while (0)
-
-template <int CTRL, typename R>
-__device__ __forceinline__ R dpp_r(R v)
-{
-    if constexpr (sizeof(R) == 8) {
-        const long long b = __builtin_bit_cast(long long, v);
-        const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
-        const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF,
-                                                   0xF, false);
-        return __builtin_bit_cast(R, (long long)(((unsigned long long)(unsigned)hi << 32)
-                                                 | (unsigned)lo));
-    } else {
-        return __builtin_bit_cast(
-            R, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF,
-                                           0xF, false));
-    }
-}
-
-template <typename R>
-__device__ __forceinline__ R readlane_r(R v, int lane)
-{
-    if constexpr (sizeof(R) == 8) {
-        const long long b = __builtin_bit_cast(long long, v);
-        const int lo = __builtin_amdgcn_readlane((int)b, lane);
-        const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-        return __builtin_bit_cast(R, (long long)(((unsigned long long)(unsigned)hi << 32)
-                                                 | (unsigned)lo));
-    } else {
-        return __builtin_bit_cast(R, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v),
-                                                               lane));
-    }
-}
 
 // ---------------------------------------------------------------- wave
 // One wavefront per system, no barriers.  Lane l holds the window column
