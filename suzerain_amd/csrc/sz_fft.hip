@@ -29,6 +29,7 @@
 
 #include <cmath>
 #include <new>
+#include <utility>
 #include <vector>
 
 namespace {
@@ -122,6 +123,81 @@ __device__ zd *lds_fft(zd *a, zd *b, const radix_plan &P, int ntr,
     return a;
 }
 
+// Compile-time radix plan (same factor order as make_plan: 4s, then 2s,
+// then 3s) so that every stage's radix, stride and twiddle step fold into
+// constants.
+constexpr int cp_nf(int n)
+{
+    int c = 0;
+    while (n % 4 == 0) { n /= 4; ++c; }
+    while (n % 2 == 0) { n /= 2; ++c; }
+    while (n % 3 == 0) { n /= 3; ++c; }
+    return c;
+}
+
+constexpr int cp_factor(int n, int s)
+{
+    int c = 0;
+    while (n % 4 == 0) { if (c++ == s) return 4; n /= 4; }
+    while (n % 2 == 0) { if (c++ == s) return 2; n /= 2; }
+    while (n % 3 == 0) { if (c++ == s) return 3; n /= 3; }
+    return 1;
+}
+
+constexpr int cp_prefix(int n, int s)
+{
+    int p = 1;
+    for (int q = 0; q < s; ++q) p *= cp_factor(n, q);
+    return p;
+}
+
+template <typename F, int... I>
+__device__ __forceinline__ void sfor_impl(F &&f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+
+template <int K, typename F>
+__device__ __forceinline__ void sfor(F &&f)
+{
+    sfor_impl(f, std::make_integer_sequence<int, K>{});
+}
+
+// lds_fft for a length N known at compile time.  The twiddle index
+// q*k*(N/(p r)) stays below N (k < p, q < r), so no reduction is needed.
+template <int N>
+__device__ zd *lds_fft_c(zd *a, zd *b, int ntr, const zd *tw, int sign)
+{
+    static_assert(cp_prefix(N, cp_nf(N)) == N, "length must be 2^a 3^b");
+    sfor<cp_nf(N)>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        constexpr int r = cp_factor(N, s), nr = N / r, p = cp_prefix(N, s);
+        constexpr int step = N / (p * r);
+        for (int it = threadIdx.x; it < ntr * nr; it += NT) {
+            const int t = it / nr, i = it - t * nr;
+            const int k = i % p;
+            const zd *in = a + t * N;
+            zd x[r];
+#pragma unroll
+            for (int q = 0; q < r; ++q) {
+                zd v = in[i + q * nr];
+                if (q > 0 && k > 0) v = sz::cmul(v, twiddle(tw, q * k * step, sign));
+                x[q] = v;
+            }
+            dft_small(r, sign, x);
+            zd *out = b + t * N;
+            const int j = (i - k) * r + k;
+#pragma unroll
+            for (int q = 0; q < r; ++q) out[j + q * p] = x[q];
+        }
+        __syncthreads();
+        zd *tmp = a;
+        a = b;
+        b = tmp;
+    });
+    return a;
+}
+
 struct fft_geom {
     int Ny, Nx, dNx, Nz, dNz, Xw, xkeep, zabs, yt;
 };
@@ -135,13 +211,14 @@ __device__ inline bool z_retained(const fft_geom &g, int kz)
 // ---- c2c along z for the retained kx: wave <-> mixed.
 // dir = +1: wave (retained kz) -> mixed (all z); dir = -1: mixed -> wave,
 // zeros outside the retained modes (blocks beyond xkeep zero kx >= xkeep).
+template <int NN, int YT>
 __global__ __launch_bounds__(NT)
 void k_fft_z(fft_geom g, radix_plan P, const zd *__restrict__ twz, int dir,
              const zd *__restrict__ in, long long sin_, zd *__restrict__ out,
              long long sout)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int n = g.dNz, yt = g.yt;
+    const int n = NN ? NN : g.dNz, yt = YT ? YT : g.yt;
     zd *tw = reinterpret_cast<zd *>(smem);
     zd *a = tw + n;
     zd *b = a + yt * n;
@@ -176,7 +253,9 @@ void k_fft_z(fft_geom g, radix_plan P, const zd *__restrict__ twz, int dir,
         a[yy * n + z] = v;
     }
     __syncthreads();
-    zd *res = lds_fft(a, b, P, yt, tw, dir);
+    zd *res;
+    if constexpr (NN > 0) res = lds_fft_c<NN>(a, b, yt, tw, dir);
+    else res = lds_fft(a, b, P, yt, tw, dir);
     for (int e = threadIdx.x; e < yt * n; e += NT) {
         const int z = e / yt, yy = e - z * yt;
         if (yy >= ny) continue;
@@ -192,13 +271,14 @@ void k_fft_z(fft_geom g, radix_plan P, const zd *__restrict__ twz, int dir,
 
 // ---- c2r along x: mixed (retained kx) -> physical, two y lines per
 // complex transform: Z_k = X1_k + i X2_k with Hermitian extension.
+template <int NN, int YT>
 __global__ __launch_bounds__(NT)
 void k_fft_x_c2r(fft_geom g, radix_plan P, const zd *__restrict__ twx,
                  const zd *__restrict__ mixed, long long smix,
                  double *__restrict__ phys, long long sphys)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int n = g.dNx, yt = g.yt, np = yt / 2;
+    const int n = NN ? NN : g.dNx, yt = YT ? YT : g.yt, np = yt / 2;
     zd *tw = reinterpret_cast<zd *>(smem);
     zd *a = tw + n;
     zd *b = a + np * n;
@@ -233,7 +313,9 @@ void k_fft_x_c2r(fft_geom g, radix_plan P, const zd *__restrict__ twx,
         a[t * n + k] = zd{x1.x - x2.y, x1.y + x2.x};   // x1 + i x2
     }
     __syncthreads();
-    zd *res = lds_fft(a, b, P, np, tw, +1);
+    zd *res;
+    if constexpr (NN > 0) res = lds_fft_c<NN>(a, b, np, tw, +1);
+    else res = lds_fft(a, b, P, np, tw, +1);
     for (int e = threadIdx.x; e < yt * n; e += NT) {
         const int yy = e / n, x = e - yy * n;
         if (yy >= ny) continue;
@@ -246,13 +328,14 @@ void k_fft_x_c2r(fft_geom g, radix_plan P, const zd *__restrict__ twx,
 // ---- r2c along x: physical -> mixed (retained kx), two y lines per
 // complex transform: X1_k = (Z_k + conj Z_{n-k})/2, X2_k = (Z_k - conj
 // Z_{n-k})/(2i).
+template <int NN, int YT>
 __global__ __launch_bounds__(NT)
 void k_fft_x_r2c(fft_geom g, radix_plan P, const zd *__restrict__ twx,
                  const double *__restrict__ phys, long long sphys,
                  zd *__restrict__ mixed, long long smix)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int n = g.dNx, yt = g.yt, np = yt / 2;
+    const int n = NN ? NN : g.dNx, yt = YT ? YT : g.yt, np = yt / 2;
     zd *tw = reinterpret_cast<zd *>(smem);
     zd *a = tw + n;
     zd *b = a + np * n;
@@ -272,7 +355,9 @@ void k_fft_x_r2c(fft_geom g, radix_plan P, const zd *__restrict__ twx,
         a[t * n + x] = zd{p1, p2};
     }
     __syncthreads();
-    zd *res = lds_fft(a, b, P, np, tw, -1);
+    zd *res;
+    if constexpr (NN > 0) res = lds_fft_c<NN>(a, b, np, tw, -1);
+    else res = lds_fft(a, b, P, np, tw, -1);
     for (int e = threadIdx.x; e < g.xkeep * yt; e += NT) {
         const int kx = e / yt, yy = e - kx * yt;
         if (yy >= ny) continue;
@@ -319,6 +404,24 @@ size_t lds_x(const sz_fft_plan *p)
 }
 
 }  // namespace
+
+// Launch a kernel template for the (length, tile) pairs compiled with
+// constant sizes, else its runtime-size instance.
+#define SZ_FFT_LAUNCH(KERN, N_, YT_, GRID, LDS, ST, ...)                        \
+    do {                                                                      \
+        if ((N_) == 144 && (YT_) == 8)                                        \
+            hipLaunchKernelGGL((KERN<144, 8>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
+        else if ((N_) == 96 && (YT_) == 16)                                   \
+            hipLaunchKernelGGL((KERN<96, 16>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
+        else if ((N_) == 192 && (YT_) == 8)                                   \
+            hipLaunchKernelGGL((KERN<192, 8>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
+        else if ((N_) == 72 && (YT_) == 16)                                   \
+            hipLaunchKernelGGL((KERN<72, 16>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
+        else if ((N_) == 48 && (YT_) == 16)                                   \
+            hipLaunchKernelGGL((KERN<48, 16>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
+        else                                                                  \
+            hipLaunchKernelGGL((KERN<0, 0>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
+    } while (0)
 
 extern "C" {
 
@@ -393,13 +496,13 @@ int sz_fft_wave_to_physical(const sz_fft_plan *p, int nfields,
     const fft_geom &g = p->g;
     const int ntile = (g.Ny + g.yt - 1) / g.yt;
     hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_fft_z, dim3(g.xkeep * ntile, nfields), dim3(NT), lds_z(p),
-                       st, g, p->pz, p->twz, +1,
-                       reinterpret_cast<const zd *>(wave), swave,
-                       reinterpret_cast<zd *>(mixed), smixed);
-    hipLaunchKernelGGL(k_fft_x_c2r, dim3(g.dNz * ntile, nfields), dim3(NT),
-                       lds_x(p), st, g, p->px, p->twx,
-                       reinterpret_cast<const zd *>(mixed), smixed, phys, sphys);
+    SZ_FFT_LAUNCH(k_fft_z, g.dNz, g.yt, dim3(g.xkeep * ntile, nfields), lds_z(p),
+                  st, g, p->pz, p->twz, +1,
+                  reinterpret_cast<const zd *>(wave), swave,
+                  reinterpret_cast<zd *>(mixed), smixed);
+    SZ_FFT_LAUNCH(k_fft_x_c2r, g.dNx, g.yt, dim3(g.dNz * ntile, nfields), lds_x(p),
+                  st, g, p->px, p->twx,
+                  reinterpret_cast<const zd *>(mixed), smixed, phys, sphys);
     return hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH;
 }
 
@@ -414,13 +517,13 @@ int sz_fft_physical_to_wave(const sz_fft_plan *p, int nfields,
     const fft_geom &g = p->g;
     const int ntile = (g.Ny + g.yt - 1) / g.yt;
     hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_fft_x_r2c, dim3(g.dNz * ntile, nfields), dim3(NT),
-                       lds_x(p), st, g, p->px, p->twx, phys, sphys,
-                       reinterpret_cast<zd *>(mixed), smixed);
-    hipLaunchKernelGGL(k_fft_z, dim3(g.Xw * ntile, nfields), dim3(NT), lds_z(p),
-                       st, g, p->pz, p->twz, -1,
-                       reinterpret_cast<const zd *>(mixed), smixed,
-                       reinterpret_cast<zd *>(wave), swave);
+    SZ_FFT_LAUNCH(k_fft_x_r2c, g.dNx, g.yt, dim3(g.dNz * ntile, nfields), lds_x(p),
+                  st, g, p->px, p->twx, phys, sphys,
+                  reinterpret_cast<zd *>(mixed), smixed);
+    SZ_FFT_LAUNCH(k_fft_z, g.dNz, g.yt, dim3(g.Xw * ntile, nfields), lds_z(p),
+                  st, g, p->pz, p->twz, -1,
+                  reinterpret_cast<const zd *>(mixed), smixed,
+                  reinterpret_cast<zd *>(wave), swave);
     return hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH;
 }
 
