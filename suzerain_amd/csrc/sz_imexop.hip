@@ -221,24 +221,31 @@ struct wallbc {
 constexpr int PK_NT = 256;
 constexpr int PK_RB = 8;  // wall-normal rows (5 columns each) per workgroup
 
+// Block and group tables of the operator staged in LDS by the pack kernel,
+// with the band limits of M, D1, D2.
+struct pack_tab {
+    const int *bg0, *bng, *gop;
+    const int *kl, *ku;                 // band limits of M, D1, D2
+};
+
 // Element (r, c) of block (fi, fj) of M + phi L; W holds this chunk's group
 // weights for row r, Dl the chunk's D_T rows.
-__device__ inline zd element(const table *__restrict__ tab, const bops &w,
-                             const zd *Wr, const double *Dl, int rl,
-                             int fi, int fj, int r, int c)
+__device__ inline zd element(const pack_tab &pt, const bops &w, const zd *Wr,
+                             const double *Dl, int rl, int fi, int fj, int r,
+                             int c)
 {
     double ex = 0, ey = 0;
     if (c >= r - w.max_ku && c <= r + w.max_kl) {
-        const int b = fi * 5 + fj, g0 = tab->bg0[b], ng = tab->bng[b];
+        const int b = fi * 5 + fj, g0 = pt.bg0[b], ng = pt.bng[b];
         for (int g = g0; g < g0 + ng; ++g) {
-            const int op = tab->gop[g];
-            if (c < r - w.ku[op] || c > r + w.kl[op]) continue;
+            const int op = pt.gop[g];
+            if (c < r - pt.ku[op] || c > r + pt.kl[op]) continue;
             const double a = Dl[(op * PK_RB + rl) * w.ld + w.max_ku + c - r];
             const zd W = Wr[g];
             ex = fma(W.x, a, ex);
             ey = fma(W.y, a, ey);
         }
-        if (fi == fj && c >= r - w.ku[M] && c <= r + w.kl[M])
+        if (fi == fj && c >= r - pt.ku[M] && c <= r + pt.kl[M])
             ex = ex + Dl[(M * PK_RB + rl) * w.ld + w.max_ku + c - r];
     }
     return zd{ex, ey};
@@ -247,10 +254,11 @@ __device__ inline zd element(const table *__restrict__ tab, const bops &w,
 // One workgroup assembles the 5*PK_RB columns of P (M + phi L)^T P^T whose
 // equation rows are r0 .. r0+PK_RB-1 (column C = r*S + p holds equation
 // field_of[p] at wall-normal row r) for one system.
+template <int SS>
 __global__ __launch_bounds__(PK_NT)
 void k_imexop_pack(const table *__restrict__ tab, bops w, refs rf, zd phi,
                    const double *__restrict__ km, const double *__restrict__ kn,
-                   int S, int4 fof_lo, int fof_hi, int rho_pos, int KL, int KU,
+                   int S_, int4 fof_lo, int fof_hi, int rho_pos, int KL, int KU,
                    zd *__restrict__ papt, int ldpapt, long long stride,
                    int nchunks, wallbc bc)
 {
@@ -258,18 +266,34 @@ void k_imexop_pack(const table *__restrict__ tab, bops w, refs rf, zd phi,
     zd *alpha = reinterpret_cast<zd *>(smem);                   // MAXTERMS
     zd *Wl = alpha + MAXTERMS;                                  // PK_RB x ngroups
     double *Dl = reinterpret_cast<double *>(Wl + PK_RB * MAXGROUPS);  // 3 x PK_RB x ld
+    __shared__ int sbg0[25], sbng[25], sgop[MAXGROUPS], skl[3], sku[3];
 
+    const int S = SS ? SS : S_;                 // fields per row (5)
     const int s = blockIdx.x / nchunks;
     const int chunk = blockIdx.x - s * nchunks;
     const int n = w.n, N = S * n;
     const int r0 = chunk * PK_RB;
     const int nr = min(PK_RB, n - r0);
     const int tid = threadIdx.x;
-    const int field_of[5] = {fof_lo.x, fof_lo.y, fof_lo.z, fof_lo.w, fof_hi};
+    // field of interleave position p (selects: a small local array indexed
+    // per lane would live in scratch)
+    auto field_of = [&](int p) __attribute__((always_inline)) {
+        return p == 0 ? fof_lo.x : p == 1 ? fof_lo.y : p == 2 ? fof_lo.z
+             : p == 3 ? fof_lo.w : fof_hi;
+    };
 
     const double kms = km[s], kns = kn[s];
     for (int t = tid; t < tab->nterms; t += PK_NT)
         alpha[t] = sz::cmul(phi, kappa(tab->tk[t], kms, kns));
+    if (tid < 25) {
+        sbg0[tid] = tab->bg0[tid];
+        sbng[tid] = tab->bng[tid];
+    }
+    for (int g = tid; g < tab->ngroups; g += PK_NT) sgop[g] = tab->gop[g];
+    if (tid < 3) {
+        skl[tid] = w.kl[tid];
+        sku[tid] = w.ku[tid];
+    }
     for (int e = tid; e < 3 * PK_RB * w.ld; e += PK_NT) {
         const int op = e / (PK_RB * w.ld), rem = e - op * PK_RB * w.ld;
         const int rl = rem / w.ld, k = rem - rl * w.ld;
@@ -282,30 +306,43 @@ void k_imexop_pack(const table *__restrict__ tab, bops w, refs rf, zd phi,
         Wl[rl * MAXGROUPS + g] = weight(tab, alpha, rf, g, r0 + rl);
     }
     __syncthreads();
+    const pack_tab pt{sbg0, sbng, sgop, skl, sku};
 
     zd *P = papt + (size_t)s * stride;
     const int LD = KL + 1 + KU;
-    const int ncols = S * nr;
-    for (int e = tid; e < ncols * LD; e += PK_NT) {
-        const int cl = e / LD, rr = e - cl * LD;
+    const int total = S * nr * LD;
+    // element e = cl*LD + rr, stepped by PK_NT without dividing in the loop
+    int cl = tid / LD, rr = tid - cl * LD;
+    const int dcl = PK_NT / LD, drr = PK_NT - dcl * LD;
+    for (int e = tid; e < total; e += PK_NT) {
         const int C = S * r0 + cl;
         const int R = C - KU + rr;
-        if (R < 0 || R >= N) continue;
-        const int pi = C % S, r = C / S, rl = r - r0;
-        const int pj = R % S, c = R / S;
-        const int fi = field_of[pi], fj = field_of[pj];
-        const zd *Wr = Wl + rl * MAXGROUPS;
-        zd v = element(tab, w, Wr, Dl, rl, fi, fj, r, c);
-        // isothermal walls replace the energy and momentum equations
-        const int wall = r == 0 ? 0 : (r == n - 1 ? 1 : -1);
-        if (wall >= 0 && bc.enforce[wall] && fi < 4) {
-            zd sc = element(tab, w, Wr, Dl, rl, fi, fi, r, r);
-            if (sc.x == 0 && sc.y == 0) sc = zd{1, 0};
-            const int rhoR = r * S + rho_pos;
-            const double f = bc.fac[wall][fi];
-            v = R == C ? sc : (R == rhoR ? zd{-sc.x * f, -sc.y * f} : zd{0, 0});
+        if (R >= 0 && R < N) {
+            const int r = C / S, pi = C - r * S, rl = r - r0;
+            const int c = R / S, pj = R - c * S;
+            const int fi = field_of(pi), fj = field_of(pj);
+            const zd *Wr = Wl + rl * MAXGROUPS;
+            zd v = element(pt, w, Wr, Dl, rl, fi, fj, r, c);
+            // isothermal walls replace the energy and momentum equations
+            const bool lo = r == 0, hi = r == n - 1;
+            if (((lo && bc.enforce[0]) || (hi && bc.enforce[1])) && fi < 4) {
+                zd sc = element(pt, w, Wr, Dl, rl, fi, fi, r, r);
+                if (sc.x == 0 && sc.y == 0) sc = zd{1, 0};
+                const int rhoR = r * S + rho_pos;
+                double f = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (q == fi) f = lo ? bc.fac[0][q] : bc.fac[1][q];
+                v = R == C ? sc : (R == rhoR ? zd{-sc.x * f, -sc.y * f} : zd{0, 0});
+            }
+            P[(size_t)C * ldpapt + KU + R - C] = v;
         }
-        P[(size_t)C * ldpapt + KU + R - C] = v;
+        rr += drr;
+        cl += dcl;
+        if (rr >= LD) {
+            rr -= LD;
+            ++cl;
+        }
     }
 }
 
@@ -533,12 +570,20 @@ int sz_rholut_imexop_packc_batched(
     const size_t lds = sizeof(zd) * (MAXTERMS + PK_RB * MAXGROUPS)
                      + sizeof(double) * 3 * PK_RB * w.ld;
     if (lds > 64 * 1024) return SZ_ERR_LDS;
-    hipLaunchKernelGGL(k_imexop_pack, dim3(nsys * nchunks), dim3(PK_NT), lds,
-                       (hipStream_t)stream, op->dev, w, make_refs(r, ld),
-                       zd{phi[0], phi[1]}, km, kn, S,
-                       make_int4(field_of[0], field_of[1], field_of[2], field_of[3]),
-                       field_of[4], rho < 0 ? 0 : rho, g.KL, g.KU, (zd *)patpt,
-                       ldpatpt, stride, nchunks, wb);
+    if (S == 5)
+        hipLaunchKernelGGL(k_imexop_pack<5>, dim3(nsys * nchunks), dim3(PK_NT), lds,
+                           (hipStream_t)stream, op->dev, w, make_refs(r, ld),
+                           zd{phi[0], phi[1]}, km, kn, S,
+                           make_int4(field_of[0], field_of[1], field_of[2], field_of[3]),
+                           field_of[4], rho < 0 ? 0 : rho, g.KL, g.KU, (zd *)patpt,
+                           ldpatpt, stride, nchunks, wb);
+    else
+        hipLaunchKernelGGL(k_imexop_pack<0>, dim3(nsys * nchunks), dim3(PK_NT), lds,
+                           (hipStream_t)stream, op->dev, w, make_refs(r, ld),
+                           zd{phi[0], phi[1]}, km, kn, S,
+                           make_int4(field_of[0], field_of[1], field_of[2], field_of[3]),
+                           field_of[4], rho < 0 ? 0 : rho, g.KL, g.KU, (zd *)patpt,
+                           ldpatpt, stride, nchunks, wb);
     return launch_ok() ? 0 : SZ_ERR_LAUNCH;
 }
 
