@@ -113,11 +113,14 @@ def cpu_baseline(inp, op, nsample):
     return nsample / dt, threads, dt
 
 
-def lu_roofline(ms, nsys, N, KL, KU, kernel="k_gbtrf<double>"):
+def lu_roofline(ms, nsys, N, KL, KU, kernel="k_gbtrf<double>", rhs=False):
     """Algorithmic HBM bytes of one factorization launch (read the band of A,
-    write the LU storage and the pivots) over its average duration."""
+    write the LU storage and the pivots; rhs: also read b and write y) over
+    its average duration."""
     ldab = 2 * KL + KU + 1
     bytes_sys = N * (KL + KU + 1) * 16 + N * ldab * 16 + N * 4
+    if rhs:
+        bytes_sys += 2 * N * 16
     achieved = bytes_sys * nsys / (ms * 1e-3) / 1e9
     flops_sys = 8.0 * N * KL * (KL + KU)
     return {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1),
@@ -128,13 +131,17 @@ def lu_roofline(ms, nsys, N, KL, KU, kernel="k_gbtrf<double>"):
             "fp64_tflops": round(flops_sys * nsys / (ms * 1e-3) / 1e12, 3)}
 
 
-def lu_kernel_name(KL, KU):
+def lu_kernel_name(KL, KU, fused_ut=False):
     """The factorization kernel the library dispatches for this band shape
     (the one-wave register kernel for the channel shapes unless
-    SZ_GBTRF_KERNEL selects the LDS window kernel)."""
+    SZ_GBTRF_KERNEL selects the LDS window kernel); fused_ut: the variant
+    that also solves U^T y = b (a fresh factorization followed by one
+    transposed solve, as in every implicit substep)."""
     sel = os.environ.get("SZ_GBTRF_KERNEL", "")
     if sel.startswith("l") or (KL, KU) not in ((34, 34), (24, 24), (34, 20)):
         return "k_gbtrf<double, 256>"
+    if fused_ut:
+        return f"k_gbtrf_wave<double, {KL + 1}, {KL + KU}, 8, true>"
     bs = {"v2": 2, "v4": 4}.get(sel, 8)
     return f"k_gbtrf_wave<double, {KL + 1}, {KL + KU}, {bs}>"
 
@@ -276,10 +283,11 @@ def main_rk3(args):
                    "parallelism": "single GPU"},
         "implicit_solves_per_s": round(ch.nsys * substeps / elapsed, 1),
         "stages_ms_per_step": stages,
-        "roofline": lu_roofline(fms / max(fcount, 1), ch.nsys, N, KL, KU),
+        "roofline": lu_roofline(fms / max(fcount, 1), ch.nsys, N, KL, KU,
+                                rhs=True),
     }
     out["roofline"]["launches_timed"] = fcount
-    kname = lu_kernel_name(KL, KU)
+    kname = lu_kernel_name(KL, KU, fused_ut=True)
     out["roofline"]["kernel"] = kname
     out["roofline"]["traffic"] = PMC_TRAFFIC.get((kname, ch.nsys))
     if not args.no_cpu:

@@ -121,9 +121,11 @@ int sz_or_zpack(int S, int n, int ihat, int jhat, int kl, int ku,
  *   a + b*c  = (fma(-b.y, c.y, fma(b.x, c.x, a.x)), fma(b.y, c.x, fma(b.x, c.y, a.y)))
  *   a * b    = (fma(a.x, b.x, -(a.y*b.y)), fma(a.x, b.y, a.y*b.x))
  *   1/z, a/b = Smith's algorithm
- *   dot products of the transposed solves: 64 strided partial sums
- *   (term t to partial (t-1) mod 64, increasing t) combined by a butterfly
- *   (w[l] += w[l ^ off] for off = 1, 2, ..., 32), result w[0].
+ *   U^T solve: sequential, x_j - U(j-kd,j) x_{j-kd} - ... - U(j-1,j) x_{j-1}
+ *   (ZTBSV's order), then the division;
+ *   L^T dot products: 64 strided partial sums (term t to partial (t-1) mod
+ *   64, increasing t) combined by a butterfly (w[l] += w[l ^ off] for
+ *   off = 1, 2, ..., 32), result w[0].
  */
 #define DEFINE_ZOPS(P, T, R, FMA, MK, ABS, RE, IM)                                    \
 static inline T P##fnms(T a, T b, T c)                                        \
@@ -276,13 +278,10 @@ int NAME(char trans, int n, int kl, int ku, int nrhs, const T *ab, int ldab,  \
             }                                                                 \
         } else {                                                              \
             for (int j = 0; j < n; ++j) {           /* U^T x = y */           \
-                const T *u = ab + j * ldab + kd;                              \
-                for (int l = 0; l < 64; ++l) w[l] = 0;                        \
-                for (int t = 1; t <= imin(kd, j); ++t)                        \
-                    w[(t - 1) & 63] = P##fma_(w[(t - 1) & 63],                \
-                                              P##cj(u[-t], cj), x[j - t]);    \
-                const T acc = P##tree(w);                                     \
-                x[j] = P##div(x[j] - acc, P##cj(u[0], cj));                   \
+                const T *u = ab + j * ldab + kd;    /* u[-t] = U(j-t, j) */   \
+                for (int t = imin(kd, j); t >= 1; --t)                        \
+                    x[j] = P##fnms(x[j], P##cj(u[-t], cj), x[j - t]);         \
+                x[j] = P##div(x[j], P##cj(u[0], cj));                         \
             }                                                                 \
             for (int j = n - 2; j >= 0 && kl > 0; --j) {  /* L^T */           \
                 const int lm = imin(kl, n - 1 - j);                           \

@@ -347,13 +347,13 @@ __device__ inline cpx<R> cj_if(cpx<R> v, bool c)
 
 // trans: 0 = 'N', 1 = 'T', 2 = 'C'.  One 64-lane workgroup per system; the
 // right-hand side lives in LDS, one barrier per elimination step.
-template <typename R>
+template <typename R, bool TR>
 __global__ __launch_bounds__(TRS_NT)
 void k_gbtrs(int trans, int n, int kl, int ku, int nrhs,
              const cpx<R> *__restrict__ ab, int ldab, long long sab,
              const int *__restrict__ ipiv, long long sipiv,
              cpx<R> *__restrict__ b, int ldb, long long sb,
-             const int *__restrict__ skip, int nsys)
+             const int *__restrict__ skip, int nsys, int skip_ut)
 {
     using T = cpx<R>;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -377,7 +377,7 @@ void k_gbtrs(int trans, int n, int kl, int ku, int nrhs,
         T *bs = b + (size_t)s * ldb;
         for (int i = tid; i < n; i += TRS_NT) x[i] = bs[i];
         LDS_BARRIER();
-        if (trans == 0) {
+        if constexpr (!TR) {
             // L y = P b: multipliers are rows kv+1..kv+kl
             if (kl > 0 && n > 1) {
                 chunk_load(rx, ry, ab, ldab, 0, min(chL, n - 1), kv + 1, kl);
@@ -433,57 +433,103 @@ void k_gbtrs(int trans, int n, int kl, int ku, int nrhs,
                 }
             }
         } else {
-            // U^T y = b, ascending
-            chunk_load(rx, ry, ab, ldab, 0, min(chU, n), 0, kv + 1);
-            for (int j0 = 0; j0 < n; j0 += chU) {
-                const int nc = min(chU, n - j0);
-                chunk_store(rx, ry, st, nc, kv + 1);
-                if (j0 + chU < n)
-                    chunk_load(rx, ry, ab, ldab, j0 + chU, min(chU, n - j0 - chU), 0, kv + 1);
-                LDS_BARRIER();
-                for (int jj = 0; jj < nc; ++jj) {
-                    const int j = j0 + jj;
-                    const T *u = st + jj * (kv + 1) + kv;
-                    T acc = {0, 0};
-                    const int tmax = min(kv, j);
-                    for (int t = 1 + tid; t <= tmax; t += TRS_NT)
-                        acc = sz::cfma(acc, cj_if(u[-t], cj), x[j - t]);
-                    acc = wave_csum(acc);
-                    if (tid == 0) x[j] = sz::cdiv(sz::csub(x[j], acc), cj_if(u[0], cj));
-                    LDS_BARRIER();
+            // U^T y = b, ascending and right-looking: once y_j is known,
+            // lane t-1 subtracts U(j, j+t) y_j from the pending x[j+t], so
+            // x[k] receives U(k-t, k) y_{k-t} for t = kv .. 1, the order of
+            // the sequential LAPACK loop, and no step needs a reduction.
+            // Lane 0 keeps x[j+1] (its own update) for the next division.
+            // skip_ut: b already holds y (solved during the factorization).
+            if (!skip_ut) {
+                constexpr int QP = 4;                  // steps prefetched ahead
+                const int t1 = tid + 1, t2 = tid + 1 + TRS_NT;
+                auto uaddr = [&](int jj, int t) -> size_t {   // U(jj, jj+t)
+                    const bool ok = t <= kv && jj + t < n;
+                    return ok ? (size_t)(jj + t) * ldab + kv - t : 0;
+                };
+                T q1[QP], q2[QP], qd[QP];
+#pragma unroll
+                for (int u = 0; u < QP; ++u) {
+                    q1[u] = ab[uaddr(u, t1)];
+                    q2[u] = ab[uaddr(u, t2)];
+                    qd[u] = ab[(size_t)min(u, n - 1) * ldab + kv];
+                }
+                T xn = x[0];
+                for (int jb = 0; jb < n; jb += QP) {
+#pragma unroll
+                    for (int u = 0; u < QP; ++u) {
+                        const int j = jb + u;
+                        if (j >= n) break;
+                        T xj;
+                        xj.x = readlane_r(xn.x, 0);
+                        xj.y = readlane_r(xn.y, 0);
+                        const T y = sz::cdiv(xj, cj_if(qd[u], cj));
+                        if (tid == 0) x[j] = y;
+                        if (t1 <= kv && j + t1 < n) {
+                            const T v = sz::cfnms(x[j + t1], cj_if(q1[u], cj), y);
+                            x[j + t1] = v;
+                            if (tid == 0) xn = v;
+                        }
+                        if (t2 <= kv && j + t2 < n)
+                            x[j + t2] = sz::cfnms(x[j + t2], cj_if(q2[u], cj), y);
+                        for (int t = t2 + TRS_NT; t <= kv && j + t < n; t += TRS_NT)
+                            x[j + t] = sz::cfnms(x[j + t], cj_if(ab[uaddr(j, t)], cj), y);
+                        q1[u] = ab[uaddr(j + QP, t1)];
+                        q2[u] = ab[uaddr(j + QP, t2)];
+                        qd[u] = ab[(size_t)min(j + QP, n - 1) * ldab + kv];
+                        LDS_BARRIER();
+                        if (kv == 0 && tid == 0 && j + 1 < n) xn = x[j + 1];
+                    }
                 }
             }
-            // L^T x = y then undo the interchanges, descending
+            // L^T x = y then undo the interchanges, descending.  Lane r
+            // forms the terms r, r+64, ... of the dot product with the
+            // multipliers of column j (one contiguous run, prefetched a few
+            // steps ahead) and the butterfly combines them.
             if (kl > 0 && n > 1) {
-                {
-                    const int nn = min(chL, n - 1);
-                    chunk_load(rx, ry, ab, ldab, n - 1 - nn, nn, kv + 1, kl);
+                constexpr int QP = 4;
+                auto laddr = [&](int jj, int r) -> size_t {   // L(jj+1+r, jj)
+                    const bool ok = jj >= 0 && r < min(kl, n - 1 - jj);
+                    return ok ? (size_t)jj * ldab + kv + 1 + r : 0;
+                };
+                T l1[QP], l2[QP];
+                int qp[QP];
+#pragma unroll
+                for (int u = 0; u < QP; ++u) {
+                    l1[u] = ab[laddr(n - 2 - u, tid)];
+                    l2[u] = ab[laddr(n - 2 - u, tid + TRS_NT)];
+                    qp[u] = ipiv[max(n - 2 - u, 0)];
                 }
-                for (int j1 = n - 2; j1 >= 0; j1 -= chL) {
-                    const int nc = min(chL, j1 + 1);
-                    const int c0 = j1 - nc + 1;
-                    chunk_store(rx, ry, st, nc, kl);
-                    for (int e = tid; e < nc; e += TRS_NT) pv[e] = ipiv[c0 + e];
-                    if (c0 > 0) {
-                        const int nn = min(chL, c0);
-                        chunk_load(rx, ry, ab, ldab, c0 - nn, nn, kv + 1, kl);
-                    }
-                    LDS_BARRIER();
-                    for (int jj = nc - 1; jj >= 0; --jj) {
-                        const int j = c0 + jj;
+                for (int jb = n - 2; jb >= 0; jb -= QP) {
+#pragma unroll
+                    for (int u = 0; u < QP; ++u) {
+                        const int j = jb - u;
+                        if (j < 0) break;
                         const int lm = min(kl, n - 1 - j);
                         T acc = {0, 0};
-                        for (int r = tid; r < lm; r += TRS_NT)
-                            acc = sz::cfma(acc, cj_if(st[jj * kl + r], cj), x[j + 1 + r]);
+                        if (tid < lm) acc = sz::cfma(acc, cj_if(l1[u], cj), x[j + 1 + tid]);
+                        if (tid + TRS_NT < lm)
+                            acc = sz::cfma(acc, cj_if(l2[u], cj), x[j + 1 + tid + TRS_NT]);
+                        for (int r = tid + 2 * TRS_NT; r < lm; r += TRS_NT)
+                            acc = sz::cfma(acc, cj_if(ab[laddr(j, r)], cj), x[j + 1 + r]);
                         acc = wave_csum(acc);
                         if (tid == 0) {
-                            const T xj = sz::csub(x[j], acc);
-                            int l = pv[jj] - 1;
+                            // component-wise (a struct copy here goes through
+                            // scratch)
+                            const T xo = x[j];
+                            const R xjx = xo.x - acc.x, xjy = xo.y - acc.y;
+                            int l = qp[u] - 1;
                             if (l < j || l >= n) l = j;
-                            const T xl = x[l];
-                            x[l] = xj;
-                            if (l != j) x[j] = xl;
+                            const R xlx = x[l].x, xly = x[l].y;
+                            x[l].x = xjx;
+                            x[l].y = xjy;
+                            if (l != j) {
+                                x[j].x = xlx;
+                                x[j].y = xly;
+                            }
                         }
+                        l1[u] = ab[laddr(j - QP, tid)];
+                        l2[u] = ab[laddr(j - QP, tid + TRS_NT)];
+                        qp[u] = ipiv[max(j - QP, 0)];
                         LDS_BARRIER();
                     }
                 }
@@ -525,14 +571,20 @@ void k_gbtrs(int trans, int n, int kl, int ku, int nrhs,
 //     (lanes with nothing to write store U(j,j) again), so outstanding
 //     memory operations are counted exactly.
 // Every updated element sees the fnms sequence of the unblocked loop.
+//
+// UT = true also solves U^T y = b on the way (the first half of ZGBTRS 'T'):
+// lane l keeps the pending b of its column, y_j = b_j / U(j,j) is formed
+// when U row j is final and every lane with a column j+d, 1 <= d <= kl+ku,
+// subtracts U(j, j+d) y_j -- x_k receives its terms for ascending rows, the
+// sequential order of the LAPACK loop.  y overwrites b.
 constexpr int WV_PF = 2;
 
-template <typename R, int WR, int KV, int WV_BS>
+template <typename R, int WR, int KV, int WV_BS, bool UT = false>
 __global__ __launch_bounds__(64)
 void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
                   int *__restrict__ ipiv, long long sipiv,
                   int *__restrict__ info, const int *__restrict__ skip,
-                  int nsys)
+                  int nsys, cpx<R> *__restrict__ bv = nullptr, long long sb = 0)
 {
     using T = cpx<R>;
     constexpr int kl = WR - 1, ku = KV - kl;
@@ -543,20 +595,28 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
     __shared__ T colbuf[WR];
     __shared__ T lbuf[WR];
     __shared__ T tail[NTAIL > 0 ? NTAIL * TW : 1];
+    __shared__ T tailb[NTAIL > 0 ? NTAIL : 1];     // pending b of tail columns
 
     const int sys = blockIdx.x;
     if (sys >= nsys) return;
     if (skip && skip[sys] != 0) return;
     ab += (size_t)sys * (size_t)sab;
     ipiv += (size_t)sys * (size_t)sipiv;
+    if constexpr (UT) bv += (size_t)sys * (size_t)sb;
     const int lane = threadIdx.x;
-    const T zero{R(0), R(0)};
 
     // component-wise select (a select of whole structs goes through memory)
     auto sel = [](bool c, T x, T y) __attribute__((always_inline)) -> T {
         T o;
         o.x = c ? x.x : y.x;
         o.y = c ? x.y : y.y;
+        return o;
+    };
+    // c ? x : 0 (literal zeros: a zero struct kept live gets spilled)
+    auto sel0 = [](bool c, T x) __attribute__((always_inline)) -> T {
+        T o;
+        o.x = c ? x.x : R(0);
+        o.y = c ? x.y : R(0);
         return o;
     };
     auto band = [&](int i, int c) __attribute__((always_inline)) -> T {
@@ -590,6 +650,18 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
         qv[u] = raw(u + WR, u + 1 + ((lane - (u + 1)) & 63));
         qt[u] = raw(u + WR, lane < NTAIL ? u + 65 + lane : n);
     }
+    // U^T solve: the pending b of this lane's column and, prefetched, the b
+    // of the column that enters the window at each step (uniform address)
+    constexpr int BNEW = NTAIL > 0 ? KV + 1 : 64;   // column j+BNEW enters at step j
+    T breg{R(0), R(0)}, qb[WV_PF];
+    if constexpr (UT) {
+        breg = sel0(lane < n, bv[lane < n ? lane : 0]);
+        if (NTAIL > 0 && lane < NTAIL) {
+            tailb[lane] = sel0(64 + lane < n, bv[64 + lane < n ? 64 + lane : 0]);
+        }
+#pragma unroll
+        for (int u = 0; u < WV_PF; ++u) qb[u] = bv[min(u + BNEW, n - 1)];
+    }
     WAVE_SYNC();
 
     int c0 = 0, th = 0, ju = 0, first_zero = 0;
@@ -612,7 +684,7 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
             WAVE_SYNC();
             int ps = u + lane;                        // slot of row j+lane
             if (ps >= WR) ps -= WR;
-            const T v = sel(lane < WR, colbuf[lane < WR ? ps : 0], zero);
+            const T v = sel0(lane < WR, colbuf[lane < WR ? ps : 0]);
             R key = R(-1);
             {
                 const R m1 = fabs(v.x) + fabs(v.y);
@@ -646,7 +718,7 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
                 l = sz::cmul(w, rcp);
             }
             const bool isl = lane >= 1 && lane <= km;
-            if (lane < WR) lbuf[ps] = sel(isl, l, zero);
+            if (lane < WR) lbuf[ps] = sel0(isl, l);
             {
                 T st;
                 st.x = isl ? l.x : piv.x;
@@ -677,12 +749,19 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
                         const int pp = j0 + jp >= WR ? j0 + jp - WR : j0 + jp;
                         int ts = th + lane;
                         if (ts >= NTAIL) ts -= NTAIL;
-                        T *tc = tail + ts * TW;
-                        const T t0 = tc[j0], t1 = tc[pp];
-                        tc[j0] = t1;
-                        tc[j0 + WR] = t1;
-                        tc[pp] = t0;
-                        tc[pp + WR] = t0;
+                        // component-wise: struct copies here go through
+                        // scratch memory
+                        R *tc = reinterpret_cast<R *>(tail + ts * TW);
+                        const R t0x = tc[2 * j0], t0y = tc[2 * j0 + 1];
+                        const R t1x = tc[2 * pp], t1y = tc[2 * pp + 1];
+                        tc[2 * j0] = t1x;
+                        tc[2 * j0 + 1] = t1y;
+                        tc[2 * (j0 + WR)] = t1x;
+                        tc[2 * (j0 + WR) + 1] = t1y;
+                        tc[2 * pp] = t0x;
+                        tc[2 * pp + 1] = t0y;
+                        tc[2 * (pp + WR)] = t0x;
+                        tc[2 * (pp + WR) + 1] = t0y;
                     }
                 }
             }
@@ -708,6 +787,24 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
                 ab[ok ? (size_t)(j + 64 + lane) * ldab + KV - 64 - lane : ujj] = st;
             }
 
+            // ---- U^T solve: y_j, then U(j, j+d) y_j out of every pending b
+            if constexpr (UT) {
+                T bj;
+                bj.x = readlane_r(breg.x, c0);
+                bj.y = readlane_r(breg.y, c0);
+                const T y = sz::cdiv(bj, piv);       // U(j, j) = piv
+                bv[j] = y;
+                if (d >= 1 && d <= KV) breg = sz::cfnms(breg, uc, y);
+                if constexpr (NTAIL > 0) {
+                    if (lane < NTAIL && j + 64 + lane < n) {
+                        int ts = th + lane;
+                        if (ts >= NTAIL) ts -= NTAIL;
+                        const T tv = tail[ts * TW + j0];
+                        tailb[ts] = sz::cfnms(tailb[ts], tv, y);
+                    }
+                }
+            }
+
             // ---- rank-1 update of rows j+1 .. j+km, columns j+1 .. ju
             if (nz && km > 0) {
                 if (d >= 1 && d <= ju - j) {
@@ -718,14 +815,16 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
                     if (ju >= j + 64 && lane < NTAIL && j + 64 + lane <= ju) {
                         int ts = th + lane;
                         if (ts >= NTAIL) ts -= NTAIL;
-                        T *tc = tail + ts * TW;
-                        const T ut = tc[j0];
+                        R *tc = reinterpret_cast<R *>(tail + ts * TW);
+                        const T ut{tc[2 * j0], tc[2 * j0 + 1]};
                         for (int p = 0; p < WR; ++p) {
                             int s = p - ob;                 // register slot of row p
                             if (s < 0) s += WR;
-                            const T x = sz::cfnms(tc[p], lbuf[s], ut);
-                            tc[p] = x;
-                            tc[p + WR] = x;
+                            const T x = sz::cfnms(T{tc[2 * p], tc[2 * p + 1]}, lbuf[s], ut);
+                            tc[2 * p] = x.x;
+                            tc[2 * p + 1] = x.y;
+                            tc[2 * (p + WR)] = x.x;
+                            tc[2 * (p + WR) + 1] = x.y;
                         }
                     }
                 }
@@ -739,9 +838,14 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
                     const T *tc = tail + th * TW + ob;   // register order
 #pragma unroll
                     for (int s = 0; s < WR; ++s) a[s] = tc[s];
+                    if constexpr (UT) breg = tailb[th];
                 } else {
 #pragma unroll
-                    for (int s = 0; s < WR; ++s) a[s] = zero;
+                    for (int s = 0; s < WR; ++s) {
+                        a[s].x = R(0);
+                        a[s].y = R(0);
+                    }
+                    if constexpr (UT) breg = sel0(j + 64 < n, qb[u % WV_PF]);
                 }
             }
             {
@@ -756,8 +860,8 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
             if constexpr (NTAIL > 0) {
                 WAVE_SYNC();
                 if (lane < WR) {
-                    tail[th * TW + lane] = zero;
-                    tail[th * TW + lane + WR] = zero;
+                    tail[th * TW + lane] = T{R(0), R(0)};
+                    tail[th * TW + lane + WR] = T{R(0), R(0)};
                 }
                 WAVE_SYNC();
                 const int th1 = th + 1 == NTAIL ? 0 : th + 1;
@@ -773,8 +877,14 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
                 }
                 qt[u % WV_PF] = raw(j + WV_PF + WR,
                                     lane < NTAIL ? j + WV_PF + 65 + lane : n);
+                // the column entering the tail (j+kl+ku+1) brings its b
+                if constexpr (UT) {
+                    if (lane == 0) tailb[th] = sel0(j + KV + 1 < n, qb[u % WV_PF]);
+                }
                 th = th1;
             }
+            if constexpr (UT)
+                qb[u % WV_PF] = bv[min(j + WV_PF + BNEW, n - 1)];
             WAVE_SYNC();
             c0 = (c0 + 1) & 63;
         }
@@ -854,6 +964,37 @@ int launch_gbtrf(int n, int kl, int ku, void *ab, int ldab, long long sab,
     return hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH;
 }
 
+// Factorization fused with the U^T half of a transposed solve of one right
+// hand side per system (b, stride sb, overwritten by y); returns 1 when the
+// band shape has no fused kernel (the caller then factors and solves apart).
+template <typename R>
+int launch_gbtrf_ut(int n, int kl, int ku, void *ab, int ldab, long long sab,
+                    int *ipiv, long long sipiv, int *info, void *b, long long sb,
+                    int nsys, hipStream_t stream)
+{
+    if (n < 0) return -1;
+    if (kl < 0) return -2;
+    if (ku < 0) return -3;
+    if (ldab < 2 * kl + ku + 1) return -5;
+    if (sab < (long long)ldab * n) return -6;
+    if (sipiv < n) return -8;
+    if (sb < n) return -12;
+    if (nsys < 0) return -10;
+    if (nsys == 0 || n == 0) return 0;
+    static const char *sel = getenv("SZ_GBTRF_KERNEL");
+    if (sel && sel[0] == 'l') return 1;
+#define SZ_WAVE_UT(WR_, KV_)                                                    \
+    hipLaunchKernelGGL((k_gbtrf_wave<R, WR_, KV_, 8, true>), dim3(nsys), dim3(64), 0, \
+                       stream, n, reinterpret_cast<cpx<R> *>(ab), ldab, sab, ipiv, \
+                       sipiv, info, nullptr, nsys, reinterpret_cast<cpx<R> *>(b), sb)
+    if (kl == 34 && ku == 34) SZ_WAVE_UT(35, 68);
+    else if (kl == 24 && ku == 24) SZ_WAVE_UT(25, 48);
+    else if (kl == 34 && ku == 20) SZ_WAVE_UT(35, 54);
+    else return 1;
+#undef SZ_WAVE_UT
+    return hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH;
+}
+
 int trans_code(char t)
 {
     switch (t) {
@@ -868,7 +1009,7 @@ template <typename R>
 int launch_gbtrs(char trans, int n, int kl, int ku, int nrhs, const void *ab,
                  int ldab, long long sab, const int *ipiv, long long sipiv,
                  void *b, int ldb, long long sb, const int *skip, int nsys,
-                 hipStream_t stream)
+                 hipStream_t stream, int skip_ut = 0)
 {
     const int tc = trans_code(trans);
     if (tc < 0) return -1;
@@ -884,13 +1025,22 @@ int launch_gbtrs(char trans, int n, int kl, int ku, int nrhs, const void *ab,
     if (nsys < 0) return -14;
     if (nsys == 0 || n == 0 || nrhs == 0) return 0;
     if (kl + ku + 1 > TRS_PRE * TRS_NT) return SZ_ERR_LDS;
-    const size_t lds = ((size_t)n + (size_t)TRS_PRE * TRS_NT) * sizeof(cpx<R>)
-                       + 64 * sizeof(int);
+    // the transposed solves keep only x in LDS; the forward one also stages
+    // factor columns and pivots
+    const size_t lds = tc == 0 ? ((size_t)n + (size_t)TRS_PRE * TRS_NT) * sizeof(cpx<R>)
+                                     + 64 * sizeof(int)
+                               : (size_t)n * sizeof(cpx<R>);
     if (lds > LDS_MAX) return SZ_ERR_LDS;
-    hipLaunchKernelGGL(k_gbtrs<R>, dim3(nsys), dim3(TRS_NT), lds, stream, tc, n,
+    if (tc == 0)
+        hipLaunchKernelGGL((k_gbtrs<R, false>), dim3(nsys), dim3(TRS_NT), lds, stream, tc, n,
                        kl, ku, nrhs, reinterpret_cast<const cpx<R> *>(ab), ldab,
                        sab, ipiv, sipiv, reinterpret_cast<cpx<R> *>(b), ldb, sb,
-                       skip, nsys);
+                       skip, nsys, skip_ut);
+    else
+        hipLaunchKernelGGL((k_gbtrs<R, true>), dim3(nsys), dim3(TRS_NT), lds, stream, tc, n,
+                       kl, ku, nrhs, reinterpret_cast<const cpx<R> *>(ab), ldab,
+                       sab, ipiv, sipiv, reinterpret_cast<cpx<R> *>(b), ldb, sb,
+                       skip, nsys, skip_ut);
     return hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH;
 }
 
@@ -932,6 +1082,29 @@ int sz_cgbtrs_batched(char trans, int n, int kl, int ku, int nrhs,
     return launch_gbtrs<float>(trans, n, kl, ku, nrhs, ab, ldab, strideab, ipiv,
                                strideipiv, b, ldb, strideb, nullptr, nbatch,
                                (hipStream_t)stream);
+}
+
+// Factor and solve U^T y = b in one pass (see launch_gbtrf_ut); 1 when the
+// band shape has no fused kernel.
+int sz_internal_zgbtrf_ut(int n, int kl, int ku, void *ab, int ldab,
+                          long long strideab, int *ipiv, long long strideipiv,
+                          int *info, void *b, long long strideb, int nbatch,
+                          void *stream)
+{
+    return launch_gbtrf_ut<double>(n, kl, ku, ab, ldab, strideab, ipiv, strideipiv,
+                                   info, b, strideb, nbatch, (hipStream_t)stream);
+}
+
+// The rest of ZGBTRS 'T' after sz_internal_zgbtrf_ut: L^T x = y and the
+// interchanges.
+int sz_internal_zgbtrs_lt(int n, int kl, int ku, const void *ab, int ldab,
+                          long long strideab, const int *ipiv,
+                          long long strideipiv, void *b, long long strideb,
+                          int nbatch, void *stream)
+{
+    return launch_gbtrs<double>('T', n, kl, ku, 1, ab, ldab, strideab, ipiv,
+                                strideipiv, b, n, strideb, nullptr, nbatch,
+                                (hipStream_t)stream, 1);
 }
 
 int sz_internal_gbtrf_masked(int single, int n, int kl, int ku, void *ab,

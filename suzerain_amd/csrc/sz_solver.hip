@@ -407,9 +407,35 @@ double bsmbsm_solver::factor_time_ms(int *count)
     return total;
 }
 
+extern "C" int sz_internal_zgbtrf_ut(int n, int kl, int ku, void *ab, int ldab,
+                                     long long strideab, int *ipiv,
+                                     long long strideipiv, int *info, void *b,
+                                     long long strideb, int nbatch, void *stream);
+extern "C" int sz_internal_zgbtrs_lt(int n, int kl, int ku, const void *ab,
+                                     int ldab, long long strideab,
+                                     const int *ipiv, long long strideipiv,
+                                     void *b, long long strideb, int nbatch,
+                                     void *stream);
+
 int bsmbsm_solver_zgbsv::solve_hook(char trans, int nrhs_)
 {
     int rc = 0;
+    // a fresh factorization followed by one transposed solve: the U^T half
+    // of the solve runs inside the factorization kernel
+    if (std::toupper((unsigned char)fact_) == 'N' && nrhs_ == 1 &&
+        std::toupper((unsigned char)trans) == 'T') {
+        if (time_factor) factor_mark();
+        rc = sz_internal_zgbtrf_ut(N, KL, KU, LU, ld_LU, stride_LU, ipiv, N, info,
+                                   PB, stride_PB, nbatch, stream);
+        if (time_factor) factor_mark();
+        if (rc == 0) {
+            fact_ = 'F';
+            return sz_internal_zgbtrs_lt(N, KL, KU, LU, ld_LU, stride_LU, ipiv, N,
+                                         PB, stride_PB, nbatch, stream);
+        }
+        if (rc != 1) return rc;
+        rc = 0;
+    }
     if (std::toupper((unsigned char)fact_) == 'N') {
         if (time_factor) factor_mark();
         rc = sz_zgbtrf_batched(N, KL, KU, LU, ld_LU, stride_LU, ipiv, N, info,
