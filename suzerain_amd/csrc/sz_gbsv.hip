@@ -347,7 +347,9 @@ __device__ inline cpx<R> cj_if(cpx<R> v, bool c)
 
 // trans: 0 = 'N', 1 = 'T', 2 = 'C'.  One 64-lane workgroup per system; the
 // right-hand side lives in LDS, one barrier per elimination step.
-template <typename R, bool TR>
+// TR: transposed ('T'/'C'); LT_ONLY: only the L^T half (U^T already done
+// by the fused factorization), compiled without the U^T prefetch queues.
+template <typename R, bool TR, bool LT_ONLY = false>
 __global__ __launch_bounds__(TRS_NT)
 void k_gbtrs(int trans, int n, int kl, int ku, int nrhs,
              const cpx<R> *__restrict__ ab, int ldab, long long sab,
@@ -439,7 +441,7 @@ void k_gbtrs(int trans, int n, int kl, int ku, int nrhs,
             // the sequential LAPACK loop, and no step needs a reduction.
             // Lane 0 keeps x[j+1] (its own update) for the next division.
             // skip_ut: b already holds y (solved during the factorization).
-            if (!skip_ut) {
+            if (!LT_ONLY && !skip_ut) {
                 constexpr int QP = 4;                  // steps prefetched ahead
                 const int t1 = tid + 1, t2 = tid + 1 + TRS_NT;
                 auto uaddr = [&](int jj, int t) -> size_t {   // U(jj, jj+t)
@@ -1033,6 +1035,11 @@ int launch_gbtrs(char trans, int n, int kl, int ku, int nrhs, const void *ab,
     if (lds > LDS_MAX) return SZ_ERR_LDS;
     if (tc == 0)
         hipLaunchKernelGGL((k_gbtrs<R, false>), dim3(nsys), dim3(TRS_NT), lds, stream, tc, n,
+                       kl, ku, nrhs, reinterpret_cast<const cpx<R> *>(ab), ldab,
+                       sab, ipiv, sipiv, reinterpret_cast<cpx<R> *>(b), ldb, sb,
+                       skip, nsys, skip_ut);
+    else if (skip_ut)
+        hipLaunchKernelGGL((k_gbtrs<R, true, true>), dim3(nsys), dim3(TRS_NT), lds, stream, tc, n,
                        kl, ku, nrhs, reinterpret_cast<const cpx<R> *>(ab), ldab,
                        sab, ipiv, sipiv, reinterpret_cast<cpx<R> *>(b), ldb, sb,
                        skip, nsys, skip_ut);
