@@ -20,8 +20,10 @@
 #include "../../include/suzerain_amd.h"
 #include "../../include/suzerain_amd_solver.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -100,6 +102,140 @@ void k_scatter(zd *__restrict__ st, long long F,
     const int f = a / n, y = a - f * n;
     const int i = y * S + f;
     st[f * F + sysoff[s] + y] = PX[(size_t)s * strideX + i];
+}
+
+// ---- wave-space derivatives of the five state fields, one launch
+// (navier_stokes.hpp:225-311).  A block stages pb coefficient pencils of one
+// field in LDS; each thread makes one y point of one pencil: the collocation
+// values M c and the wall-normal derivatives D1 c, D2 c (banded products with
+// ascending columns and one fma per term, then the 1 * acc complex product,
+// as sz_bsplineop_accumulate_complex_batched computes them), and the x/z
+// derivatives of the values and of D1 c in the arithmetic of
+// sz_diffwave_accumulate (square-and-multiply wavenumber powers, separately
+// rounded complex products).  Only retained pencils are visited, so neither
+// the dealiased state modes are read (they count as zero, the effect of
+// zeroing them first) nor the dealiased output modes written: the backward
+// transforms read retained wavenumbers only.  On a retained pencil
+// diffwave.c's wavenumber_diff is the signed index itself (kx = x,
+// kz = wz), and its keep test reduces to "nonzero when differentiated".
+struct deriv_args {
+    const zd *s;
+    zd *vals, *auxw;
+    long long F;
+    const double *dt;            // D_T of derivatives 0, 1, 2, each n * ld
+    int n, ld, kuaddr;
+    int kl0, ku0, kl1, ku1, kl2, ku2;
+    int xkeep, zabs, Xw, dNz, pb;
+    double tpx, tpz;
+};
+
+__device__ inline double pow_int(double x, int n)
+{
+    double v = 1;
+    do {
+        if (n & 1) v *= x;
+        n >>= 1;
+        x *= x;
+    } while (n);
+    return v;
+}
+
+__device__ inline zd band_apply(const double *col, const zd *xp, int i, int n,
+                                int kl, int ku)
+{
+    const int j0 = max(0, i - ku), j1 = min(n - 1, i + kl);
+    zd acc = {0, 0};
+    for (int j = j0; j <= j1; ++j) {
+        const double d = col[j];
+        const zd xv = xp[j];
+        acc.x = fma(d, xv.x, acc.x);
+        acc.y = fma(d, xv.y, acc.y);
+    }
+    return acc;
+}
+
+// (i kx)^DX (i kz)^DZ x, and the same plus 1 * y
+template <int DX, int DZ>
+__device__ inline zd dwave(zd x, int mf, int nf, double tpx, double tpz)
+{
+    if ((DX > 0 && mf == 0) || (DZ > 0 && nf == 0)) return zd{0, 0};
+    const double nscale = pow_int(tpz * nf, DZ);
+    const double mscale = nscale * pow_int(tpx * mf, DX);
+    constexpr int r = (DX + DZ) & 3;
+    const double ax = r == 0 ? 1.0 : r == 1 ? -0.0 : r == 2 ? -1.0 : 0.0;
+    const double ay = r == 0 ? 0.0 : r == 1 ? 1.0 : r == 2 ? -0.0 : -1.0;
+    return sz::cmul_rn(sz::cscale(mscale, zd{ax, ay}), x);
+}
+
+template <int DX, int DZ>
+__device__ inline zd dwave_acc(zd x, zd y, int mf, int nf, double tpx, double tpz)
+{
+    const zd by = sz::cmul_rn(zd{1.0, 0.0}, y);
+    if ((DX > 0 && mf == 0) || (DZ > 0 && nf == 0)) return by;
+    return sz::cadd_rn(dwave<DX, DZ>(x, mf, nf, tpx, tpz), by);
+}
+
+__global__ __launch_bounds__(NT)
+void k_derivs(deriv_args a)
+{
+    extern __shared__ __align__(16) unsigned char dsm[];
+    zd *pen = reinterpret_cast<zd *>(dsm);
+    const int f = blockIdx.y, n = a.n;
+    const int nzr = 2 * a.zabs + 1;
+    const int q0 = blockIdx.x * a.pb;
+    const int np = min(a.pb, a.xkeep * nzr - q0);
+    const long long F = a.F;
+    // retained pencil q -> offset of its first y point; z runs over
+    // 0..zabs then dNz-zabs..dNz-1, x over 0..xkeep-1 (the system order)
+    auto offset = [&](int q, int &x, int &wz) {
+        const int zi = q / a.xkeep;
+        x = q - zi * a.xkeep;
+        wz = zi <= a.zabs ? zi : zi - nzr;
+        return ((long long)(wz < 0 ? wz + a.dNz : wz) * a.Xw + x) * n;
+    };
+    const zd *sf = a.s + (size_t)f * F;
+    for (int e = threadIdx.x; e < np * n; e += blockDim.x) {
+        const int lp = e / n, i = e - lp * n;
+        int x, wz;
+        pen[e] = sf[offset(q0 + lp, x, wz) + i];
+    }
+    __syncthreads();
+    const zd one = {1.0, 0.0};
+    for (int e = threadIdx.x; e < np * n; e += blockDim.x) {
+        const int lp = e / n, i = e - lp * n;
+        int x, wz;
+        const long long o = offset(q0 + lp, x, wz) + i;
+        const zd *xp = pen + lp * n;
+        const double *col = a.dt + (size_t)i * a.ld + a.kuaddr - i;
+        const size_t dstep = (size_t)n * a.ld;
+        const zd v = sz::cmul(one, band_apply(col, xp, i, n, a.kl0, a.ku0));
+        const zd y = sz::cmul(one, band_apply(col + dstep, xp, i, n, a.kl1, a.ku1));
+        const zd yy = sz::cmul(one, band_apply(col + 2 * dstep, xp, i, n, a.kl2, a.ku2));
+        const double tx = a.tpx, tz = a.tpz;
+        a.vals[f * F + o] = v;
+        if (f == 0) {
+            // e_y, e_yy, e_yy + e_xx + e_zz (accumulated in that order), e_x, e_z
+            zd *A = a.auxw + o;
+            A[0] = y;
+            A[F] = yy;
+            zd lap = dwave_acc<2, 0>(v, yy, x, wz, tx, tz);
+            A[2 * F] = dwave_acc<0, 2>(v, lap, x, wz, tx, tz);
+            A[3 * F] = dwave<1, 0>(v, x, wz, tx, tz);
+            A[4 * F] = dwave<0, 1>(v, x, wz, tx, tz);
+        } else {
+            // _y _yy _x _xx _xz _z _zz _xy _yz
+            zd *A = a.auxw + (size_t)(5 + 9 * (f - 1)) * F + o;
+            A[0] = y;
+            A[F] = yy;
+            A[2 * F] = dwave<1, 0>(v, x, wz, tx, tz);
+            A[3 * F] = dwave<2, 0>(v, x, wz, tx, tz);
+            A[4 * F] = dwave<1, 1>(v, x, wz, tx, tz);
+            A[5 * F] = dwave<0, 1>(v, x, wz, tx, tz);
+            A[6 * F] = dwave<0, 2>(v, x, wz, tx, tz);
+            A[7 * F] = dwave<1, 0>(y, x, wz, tx, tz);
+            A[8 * F] = dwave<0, 1>(y, x, wz, tx, tz);
+        }
+    }
 }
 
 bool ok() { return hipGetLastError() == hipSuccess; }
@@ -397,9 +533,48 @@ int sz_channel_nonlinear(sz_channel *h, int which, int zeroth,
     zd *s = h->state(which);
     const long long F = h->F;
     hipStream_t st = h->stream;
-    int rc = h->zero_dealias(s, 5);
+    int rc = 0;
+    // wave-space derivatives (navier_stokes.hpp:225-311): one fused launch;
+    // SZ_DERIV=split selects the per-operator launches, which give the same
+    // bits on every retained pencil
+    const char *dsel = std::getenv("SZ_DERIV");
+    const int n = sp.Ny;
+    const int pb = n >= NT ? 1 : NT / n;
+    const bool fused = !(dsel && dsel[0] == 's') && (size_t)pb * n * sizeof(zd) <= 65536;
+    if (fused) {
+        deriv_args a;
+        a.s = s;
+        a.vals = h->vals;
+        a.auxw = h->auxw;
+        a.F = F;
+        a.dt = sz_bsplineop_D_T_device(h->bop);
+        if (!a.dt) return SZ_ERR_ALLOC;
+        a.n = sz_bsplineop_n(h->bop);
+        if (a.n != n) return -3;
+        a.ld = sz_bsplineop_ld(h->bop);
+        a.kuaddr = sz_bsplineop_max_ku(h->bop);
+        a.kl0 = sz_bsplineop_kl(h->bop, 0);
+        a.ku0 = sz_bsplineop_ku(h->bop, 0);
+        a.kl1 = sz_bsplineop_kl(h->bop, 1);
+        a.ku1 = sz_bsplineop_ku(h->bop, 1);
+        a.kl2 = sz_bsplineop_kl(h->bop, 2);
+        a.ku2 = sz_bsplineop_ku(h->bop, 2);
+        a.xkeep = h->xkeep;
+        a.zabs = h->zabs;
+        a.Xw = h->Xw;
+        a.dNz = sp.dNz;
+        a.pb = pb;
+        // 2 pi / L as sz_diffwave_accumulate forms it (double, no contraction)
+        volatile double twopi = 2 * 3.1415926535897932384626433832795028841971693993751058209;
+        a.tpx = twopi / sp.Lx;
+        a.tpz = twopi / sp.Lz;
+        const int nthr = (int)std::min<long long>(NT, ((long long)pb * n + 63) / 64 * 64);
+        hipLaunchKernelGGL(k_derivs, dim3((unsigned)((h->nsys + pb - 1) / pb), 5),
+                           dim3(nthr), (size_t)pb * n * sizeof(zd), st, a);
+        if (!ok()) return SZ_ERR_LAUNCH;
+    }
+    if (!fused) rc = h->zero_dealias(s, 5);
     if (rc) return rc;
-    // wave-space derivatives (navier_stokes.hpp:225-311)
     auto A = [&](int idx) { return h->auxw + (size_t)idx * F; };
     const double one[2] = {1, 0}, zero[2] = {0, 0};
     const int npen = sp.dNz * h->Xw;
@@ -414,7 +589,7 @@ int sz_channel_nonlinear(sz_channel *h, int which, int zeroth,
                                       sp.dNz, st);
     };
     // aux indices: e 0..4, then mx 5.., my 14.., mz 23.., rho 32..
-    for (int f = 0; f < 5 && !rc; ++f) {
+    for (int f = 0; f < 5 && !rc && !fused; ++f) {
         const zd *c = s + (size_t)f * F;
         zd *v = h->vals + (size_t)f * F;
         if (f == 0) {

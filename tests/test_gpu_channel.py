@@ -160,6 +160,63 @@ def test_nonlinear_operator_matches_oracle(gpu, small):
     np.testing.assert_allclose(cand, wcand, rtol=1e-11)
 
 
+def _dealiased_modes(ch):
+    kx = np.arange(ch.Xw)
+    kz = np.arange(ch.dNz)
+    wz = np.where(kz < (ch.dNz + 1) // 2, kz, kz - ch.dNz)
+    keep = (kx[None, :] <= (ch.Nx - 1) // 2) & (np.abs(wz)[:, None] <= (ch.Nz - 1) // 2)
+    return ~keep
+
+
+def _nonlinear_fused_and_split(ch, W, gpu, monkeypatch):
+    out = {}
+    for sel in ("fused", "split"):
+        if sel == "split":
+            monkeypatch.setenv("SZ_DERIV", "split")
+        else:
+            monkeypatch.delenv("SZ_DERIV", raising=False)
+        ch.set_state(_t(W, gpu), which=1)
+        cand = ch.nonlinear(which=1, zeroth=True)
+        out[sel] = (ch.get_state(1).cpu().numpy(), cand, ch.refs())
+    monkeypatch.delenv("SZ_DERIV", raising=False)
+    return out
+
+
+def _fused_vs_split(ch, W, gpu, monkeypatch, seed):
+    """The one-launch derivative stage reproduces the per-operator launches
+    bit for bit, and both treat dealiased state modes as zero (the
+    reference zeroes them before differentiating)."""
+    base = _nonlinear_fused_and_split(ch, W, gpu, monkeypatch)
+    for k in range(3):
+        np.testing.assert_array_equal(base["fused"][k], base["split"][k])
+    rng = np.random.default_rng(seed)
+    G = W.copy()
+    junk = rng.standard_normal(W.shape) + 1j * rng.standard_normal(W.shape)
+    G[:, _dealiased_modes(ch), :] = junk[:, _dealiased_modes(ch), :]
+    dirty = _nonlinear_fused_and_split(ch, G, gpu, monkeypatch)
+    for sel in ("fused", "split"):
+        for k in range(3):
+            np.testing.assert_array_equal(dirty[sel][k], base["fused"][k])
+
+
+def test_fused_derivatives_match_split_small(gpu, small, monkeypatch):
+    ch, g = small
+    _fused_vs_split(ch, _random_state(g, seed=11), gpu, monkeypatch, 12)
+
+
+def test_fused_derivatives_match_split_tall(gpu, monkeypatch):
+    """Ny above one block's width: one pencil per block, strided rows."""
+    from suzerain_amd.channel import Channel, laminar_state
+    ch = Channel(Nx=4, Ny=300, Nz=6, dNx=6, dNz=9, k=6, htdelta=2.0,
+                 scenario=SCEN)
+    _fused_vs_split(ch, laminar_state(ch, amp=1e-2, kmax=3), gpu, monkeypatch, 13)
+
+
+def test_fused_derivatives_match_split_full_size(gpu, big, monkeypatch):
+    from suzerain_amd.channel import laminar_state
+    _fused_vs_split(big, laminar_state(big, amp=1e-2), gpu, monkeypatch, 14)
+
+
 def test_operator_apply_and_invert_round_trip(gpu, small):
     """invert(phi) after apply(phi) recovers the state on every retained
     wavenumber whose walls already satisfy the isothermal constraints."""
