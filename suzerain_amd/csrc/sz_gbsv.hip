@@ -563,9 +563,17 @@ void k_gbtrs(int trans, int n, int kl, int ku, int nrhs,
 // Columns j+64 .. j+kl+ku (kl+ku >= 64) live in an LDS tail, column j+64+t
 // in tail slot (th+t) mod NTAIL, row i at i mod WR (stored twice, so a
 // column reads back in register order from one base address).  Step j:
-//   * the lane holding column j writes it to LDS, lane r <= km reads row
-//     j+r and the pivot is the first lane whose |Re|+|Im| equals the DPP and
-//     readlane maximum (a NaN in row j wins, any other NaN never does);
+//   * lane r <= km needs row j+r of column j.  At a block start the lanes
+//     holding the block's WV_BS columns write them to LDS together, one
+//     store per slot for all of them (tcol, row per slot).  Each step then
+//     applies its update to the later block columns there as well: lane r
+//     updates the row its multiplier belongs to with U(j, k) read from the
+//     lane holding column k, the same fnms the registers see, and the lanes
+//     holding those columns store the entering row.  After an interchange
+//     or a skipped update the rest of the block falls back to the lane
+//     holding column j writing it out.  The pivot is the first lane whose
+//     |Re|+|Im| equals the DPP and readlane maximum (a NaN in row j wins,
+//     any other NaN never does);
 //   * lane r writes multiplier r to LDS at row j+r's slot (zero for row j and
 //     rows past n), so the update is one static loop without predicates;
 //   * every lane stores U(j, its column), U(j, a tail column), L(lane, j)
@@ -596,6 +604,10 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
     static_assert(KV <= 127 && ku >= 0, "window");
     __shared__ T colbuf[WR];
     __shared__ T lbuf[WR];
+    // the row-per-slot block columns pay off when LDS is the shared limit,
+    // i.e. with several waves per SIMD: the fused solve of the full batch
+    constexpr bool TCOL = UT;
+    __shared__ T tcol[TCOL ? WV_BS : 1][WR];       // block columns, row per slot
     __shared__ T tail[NTAIL > 0 ? NTAIL * TW : 1];
     __shared__ T tailb[NTAIL > 0 ? NTAIL : 1];     // pending b of tail columns
 
@@ -669,6 +681,7 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
     int c0 = 0, th = 0, ju = 0, first_zero = 0;
     int ob = 0;                                      // jb mod WR
     for (int jb = 0; jb < n; jb += WV_BS) {
+        bool replay = TCOL;                          // tcol is current
 #pragma unroll
         for (int u = 0; u < WV_BS; ++u) {
             const int j = jb + u;
@@ -678,15 +691,21 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
             const int d = (lane - c0) & 63;          // this lane: column j + d
             const size_t ujj = (size_t)j * ldab + KV;
 
-            // ---- pivot search over column j
-            if (d == 0) {
+            // ---- column j, row j+lane in lane (slot s holds row jb+s at u = 0)
+            if (TCOL && u == 0) {
+                if (d < WV_BS) {
+#pragma unroll
+                    for (int s = 0; s < WR; ++s) tcol[d][s] = a[s];
+                }
+            } else if (!replay && d == 0) {
 #pragma unroll
                 for (int s = 0; s < WR; ++s) colbuf[s] = a[s];
             }
             WAVE_SYNC();
             int ps = u + lane;                        // slot of row j+lane
             if (ps >= WR) ps -= WR;
-            const T v = sel0(lane < WR, colbuf[lane < WR ? ps : 0]);
+            const T v = sel0(lane < WR, replay ? tcol[u][lane < WR ? ps : 0]
+                                               : colbuf[lane < WR ? ps : 0]);
             R key = R(-1);
             {
                 const R m1 = fabs(v.x) + fabs(v.y);
@@ -720,7 +739,8 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
                 l = sz::cmul(w, rcp);
             }
             const bool isl = lane >= 1 && lane <= km;
-            if (lane < WR) lbuf[ps] = sel0(isl, l);
+            const T lz = sel0(isl, l);
+            if (lane < WR) lbuf[ps] = lz;
             {
                 T st;
                 st.x = isl ? l.x : piv.x;
@@ -764,6 +784,22 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
                         tc[2 * pp + 1] = t0y;
                         tc[2 * (pp + WR)] = t0x;
                         tc[2 * (pp + WR) + 1] = t0y;
+                    }
+                }
+            }
+            // the block's later columns, row per slot, receive this step's
+            // update as their registers do (same multiplier, same U(j, k),
+            // same fnms) unless an interchange or a skipped update ends that
+            if (!nz || jp != 0 || km == 0) replay = false;
+            if (replay) {
+#pragma unroll
+                for (int k = u + 1; k < WV_BS; ++k) {
+                    T uk;
+                    uk.x = readlane_r(uc.x, (c0 + k - u) & 63);
+                    uk.y = readlane_r(uc.y, (c0 + k - u) & 63);
+                    if (lane < WR) {
+                        const T t = tcol[k][ps];
+                        tcol[k][ps] = sz::cfnms(t, lz, uk);
                     }
                 }
             }
@@ -856,6 +892,7 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
                 const T e = qv[u % WV_PF];
                 a[u].x = ok ? e.x : R(0);
                 a[u].y = ok ? e.y : R(0);
+                if (replay && d >= 1 && d < WV_BS - u) tcol[u + d][u] = a[u];   // row j+WR
                 qv[u % WV_PF] = raw(j + WV_PF + WR,
                                     j + WV_PF + 1 + ((lane - c0 - WV_PF - 1) & 63));
             }
