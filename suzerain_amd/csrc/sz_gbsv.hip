@@ -574,8 +574,11 @@ void k_gbtrs(int trans, int n, int kl, int ku, int nrhs,
 //     holding column j writing it out.  The pivot is the first lane whose
 //     |Re|+|Im| equals the DPP and readlane maximum (a NaN in row j wins,
 //     any other NaN never does);
-//   * lane r writes multiplier r to LDS at row j+r's slot (zero for row j and
-//     rows past n), so the update is one static loop without predicates;
+//   * lane r forms multiplier r (zero for row j and rows past n); the update
+//     is one static loop without predicates that takes the multiplier of
+//     slot s from its row's lane by readlane (LDS issue, not VALU, is what
+//     several waves per SIMD contend for), and lane r also writes it to LDS
+//     at row j+r's slot for the far-pivot tail path;
 //   * every lane stores U(j, its column), U(j, a tail column), L(lane, j)
 //     and the pivot index, and issues two prefetch loads, all unconditional
 //     (lanes with nothing to write store U(j,j) again), so outstanding
@@ -739,7 +742,11 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
                 l = sz::cmul(w, rcp);
             }
             const bool isl = lane >= 1 && lane <= km;
-            const T lz = sel0(isl, l);
+            T lz = sel0(isl, l);
+            // the update below runs under a divergent mask and reads every
+            // lane's multiplier by readlane: pin it here, where all lanes are
+            // active, so it is not computed inside the mask instead
+            asm volatile("" : "+v"(lz.x), "+v"(lz.y));
             if (lane < WR) lbuf[ps] = lz;
             {
                 T st;
@@ -847,7 +854,15 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
             if (nz && km > 0) {
                 if (d >= 1 && d <= ju - j) {
 #pragma unroll
-                    for (int s = 0; s < WR; ++s) a[s] = sz::cfnms(a[s], lbuf[s], uc);
+                    for (int s = 0; s < WR; ++s) {
+                        // multiplier of slot s from the lane of its row (a
+                        // scalar operand: no LDS traffic in the update)
+                        const int r = s - u < 0 ? s - u + WR : s - u;
+                        T m;
+                        m.x = readlane_r(lz.x, r);
+                        m.y = readlane_r(lz.y, r);
+                        a[s] = sz::cfnms(a[s], m, uc);
+                    }
                 }
                 if constexpr (NTAIL > 0) {
                     if (ju >= j + 64 && lane < NTAIL && j + 64 + lane <= ju) {

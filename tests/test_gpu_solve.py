@@ -209,3 +209,54 @@ def test_model_operator_assembly_and_solve(gpu):
         want, rinfo = om.solve_permuted(A[s], KL, KU, S, n, inp["rhs"][s])
         assert rinfo == 0
         np.testing.assert_array_equal(X[s], want, err_msg=str(s))
+
+
+@pytest.mark.parametrize("N,kl,ku,dom", [(480, 34, 34, 0.0), (480, 34, 34, 6.0),
+                                         (300, 24, 24, 0.0), (300, 24, 24, 5.0),
+                                         (97, 34, 20, 0.0), (40, 34, 34, 0.0),
+                                         (61, 24, 24, 0.0)])
+def test_fused_factor_and_transposed_solve_match_oracle(gpu, N, kl, ku, dom):
+    """The implicit substep's solve: factorization with the U^T half of
+    ZGBTRS 'T' fused in (sz_internal_zgbtrf_ut), then the L^T sweep
+    (sz_internal_zgbtrs_lt), bit-identical to ZGBTRF then ZGBTRS 'T'.  Batches
+    with and without a dominant diagonal exercise both the interchange-free
+    blocks and the fallback after an interchange."""
+    import ctypes as C
+    import torch
+    from suzerain_amd._lib import load
+    lib = load()
+    i32, i64, vp = C.c_int, C.c_longlong, C.c_void_p
+    f_ut = lib.sz_internal_zgbtrf_ut
+    f_ut.restype = i32
+    f_ut.argtypes = [i32, i32, i32, vp, i32, i64, vp, i64, vp, vp, i64, i32, vp]
+    f_lt = lib.sz_internal_zgbtrs_lt
+    f_lt.restype = i32
+    f_lt.argtypes = [i32, i32, i32, vp, i32, i64, vp, i64, vp, i64, i32, vp]
+    rng = np.random.default_rng(N + 3 * kl + ku + int(dom))
+    nb = 6
+    ab = _random_bands(rng, nb, N, kl, ku, dom)
+    ldab = ab.shape[2]
+    b = rng.standard_normal((nb, N)) + 1j * rng.standard_normal((nb, N))
+    abd, bd = _t(ab, gpu), _t(b, gpu)
+    ipiv = torch.zeros((nb, N), dtype=torch.int32, device=gpu)
+    info = torch.zeros((nb,), dtype=torch.int32, device=gpu)
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert f_ut(N, kl, ku, abd.data_ptr(), ldab, ldab * N, ipiv.data_ptr(), N,
+                info.data_ptr(), bd.data_ptr(), N, nb, stream) == 0
+    assert f_lt(N, kl, ku, abd.data_ptr(), ldab, ldab * N, ipiv.data_ptr(), N,
+                bd.data_ptr(), N, nb, stream) == 0
+    torch.cuda.synchronize()
+    lu, piv, x = abd.cpu().numpy(), ipiv.cpu().numpy(), bd.cpu().numpy()
+    assert (info.cpu().numpy() == 0).all()
+    pivoted = 0
+    for s in range(nb):
+        ref = ab[s].reshape(-1).copy()
+        rinfo, rpiv = oracle.zgbtrf(N, kl, ku, ref, ldab)
+        assert rinfo == 0
+        pivoted += int((np.asarray(rpiv) != np.arange(1, N + 1)).sum())
+        np.testing.assert_array_equal(piv[s], rpiv)
+        np.testing.assert_array_equal(lu[s], ref.reshape(N, ldab))
+        want = oracle.zgbtrs("T", N, kl, ku, ref, ldab, rpiv, b[s].copy())
+        np.testing.assert_array_equal(x[s], want, err_msg=str(s))
+    if dom == 0.0:
+        assert pivoted > 0
