@@ -569,9 +569,10 @@ void k_gbtrs(int trans, int n, int kl, int ku, int nrhs,
 //     applies its update to the later block columns there as well: lane r
 //     updates the row its multiplier belongs to with U(j, k) read from the
 //     lane holding column k, the same fnms the registers see, and the lanes
-//     holding those columns store the entering row.  After an interchange
-//     or a skipped update the rest of the block falls back to the lane
-//     holding column j writing it out.  The pivot is the first lane whose
+//     holding those columns store the entering row; an interchange swaps
+//     the two rows there as it does in registers.  After a skipped update
+//     (zero pivot) the rest of the block falls back to the lane holding
+//     column j writing it out.  The pivot is the first lane whose
 //     |Re|+|Im| equals the DPP and readlane maximum (a NaN in row j wins,
 //     any other NaN never does);
 //   * lane r forms multiplier r (zero for row j and rows past n); the update
@@ -796,16 +797,24 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
             }
             // the block's later columns, row per slot, receive this step's
             // update as their registers do (same multiplier, same U(j, k),
-            // same fnms) unless an interchange or a skipped update ends that
-            if (!nz || jp != 0 || km == 0) replay = false;
+            // same fnms) unless a skipped update ends that
+            if (!nz || km == 0) replay = false;
             if (replay) {
+                // an interchange of rows j and j+jp reaches the block columns
+                // too (they lie within ju): rows j and j+jp read each other's
+                // slot before the update
+                int src = ps;
+                if (nz && jp != 0) {
+                    const int sp = u + jp >= WR ? u + jp - WR : u + jp;
+                    src = lane == 0 ? sp : (lane == jp ? u : ps);
+                }
 #pragma unroll
                 for (int k = u + 1; k < WV_BS; ++k) {
                     T uk;
                     uk.x = readlane_r(uc.x, (c0 + k - u) & 63);
                     uk.y = readlane_r(uc.y, (c0 + k - u) & 63);
                     if (lane < WR) {
-                        const T t = tcol[k][ps];
+                        const T t = tcol[k][src];
                         tcol[k][ps] = sz::cfnms(t, lz, uk);
                     }
                 }
