@@ -163,43 +163,59 @@ __device__ __forceinline__ void sfor(F &&f)
     sfor_impl(f, std::make_integer_sequence<int, K>{});
 }
 
-// lds_fft for a length N known at compile time.  The twiddle index
-// q*k*(N/(p r)) stays below N (k < p, q < r), so no reduction is needed.
-template <int N>
-__device__ zd *lds_fft_c(zd *a, zd *b, int ntr, const zd *tw, int sign)
+// In-place Stockham passes for NTR transforms of a length N known at compile
+// time: each pass reads its butterflies into registers, waits, and writes
+// them back into the same buffer, so a tile needs one buffer instead of two
+// (twice the lines per block for the same LDS).  The twiddle index
+// q*k*(N/(p r)) stays below N (k < p, q < r), so no reduction is needed; the
+// arithmetic is that of lds_fft.
+template <int N, int NTR>
+__device__ zd *lds_fft_ip(zd *a, const zd *tw, int sign)
 {
     static_assert(cp_prefix(N, cp_nf(N)) == N, "length must be 2^a 3^b");
     sfor<cp_nf(N)>([&](auto sc) {
         constexpr int s = decltype(sc)::value;
         constexpr int r = cp_factor(N, s), nr = N / r, p = cp_prefix(N, s);
         constexpr int step = N / (p * r);
-        for (int it = threadIdx.x; it < ntr * nr; it += NT) {
-            const int t = it / nr, i = it - t * nr;
-            const int k = i % p;
-            const zd *in = a + t * N;
-            zd x[r];
+        constexpr int TOT = NTR * nr, IT = (TOT + NT - 1) / NT;
+        zd x[IT][r];
 #pragma unroll
-            for (int q = 0; q < r; ++q) {
-                zd v = in[i + q * nr];
-                if (q > 0 && k > 0) v = sz::cmul(v, twiddle(tw, q * k * step, sign));
-                x[q] = v;
+        for (int m = 0; m < IT; ++m) {
+            const int it = threadIdx.x + m * NT;
+            if (TOT % NT == 0 || it < TOT) {
+                const int t = it / nr, i = it - t * nr;
+                const int k = i % p;
+                const zd *in = a + t * N;
+#pragma unroll
+                for (int q = 0; q < r; ++q) {
+                    zd v = in[i + q * nr];
+                    if (q > 0 && k > 0) v = sz::cmul(v, twiddle(tw, q * k * step, sign));
+                    x[m][q] = v;
+                }
+                dft_small(r, sign, x[m]);
             }
-            dft_small(r, sign, x);
-            zd *out = b + t * N;
-            const int j = (i - k) * r + k;
-#pragma unroll
-            for (int q = 0; q < r; ++q) out[j + q * p] = x[q];
         }
         __syncthreads();
-        zd *tmp = a;
-        a = b;
-        b = tmp;
+#pragma unroll
+        for (int m = 0; m < IT; ++m) {
+            const int it = threadIdx.x + m * NT;
+            if (TOT % NT == 0 || it < TOT) {
+                const int t = it / nr, i = it - t * nr;
+                const int k = i % p;
+                zd *out = a + t * N;
+                const int j = (i - k) * r + k;
+#pragma unroll
+                for (int q = 0; q < r; ++q) out[j + q * p] = x[m][q];
+            }
+        }
+        __syncthreads();
     });
     return a;
 }
 
+
 struct fft_geom {
-    int Ny, Nx, dNx, Nz, dNz, Xw, xkeep, zabs, yt;
+    int Ny, Nx, dNx, Nz, dNz, Xw, xkeep, zabs, yt, ytx;   // y lines per block: z, x
 };
 
 __device__ inline bool z_retained(const fft_geom &g, int kz)
@@ -221,7 +237,7 @@ void k_fft_z(fft_geom g, radix_plan P, const zd *__restrict__ twz, int dir,
     const int n = NN ? NN : g.dNz, yt = YT ? YT : g.yt;
     zd *tw = reinterpret_cast<zd *>(smem);
     zd *a = tw + n;
-    zd *b = a + yt * n;
+    zd *b = NN > 0 ? a : a + yt * n;        // in place for constant sizes
     const int ntile = (g.Ny + yt - 1) / yt;
     const int kx = blockIdx.x / ntile;
     const int y0 = (blockIdx.x - kx * ntile) * yt;
@@ -239,33 +255,32 @@ void k_fft_z(fft_geom g, radix_plan P, const zd *__restrict__ twz, int dir,
     }
     for (int e = threadIdx.x; e < n; e += NT) tw[e] = twz[e];
     // load: a[t*n + z] for t = y line
+    // component-wise: a struct assigned on several paths goes through
+    // scratch memory, and each scratch access waits for every load in flight
+    const size_t zs = (size_t)g.Ny * (dir > 0 ? g.Xw : g.xkeep);
     for (int e = threadIdx.x; e < yt * n; e += NT) {
         const int z = e / yt, yy = e - z * yt;
-        zd v = {0, 0};
-        if (yy < ny) {
-            if (dir > 0) {
-                if (z_retained(g, z))
-                    v = in[(size_t)(y0 + yy) + (size_t)g.Ny * (kx + (size_t)g.Xw * z)];
-            } else {
-                v = in[(size_t)(y0 + yy) + (size_t)g.Ny * (kx + (size_t)g.xkeep * z)];
-            }
+        const bool ok = yy < ny && (dir <= 0 || z_retained(g, z));
+        double vx = 0, vy = 0;
+        if (ok) {
+            const zd *src = in + (size_t)(y0 + yy) + (size_t)g.Ny * kx + zs * z;
+            vx = src->x;
+            vy = src->y;
         }
-        a[yy * n + z] = v;
+        a[yy * n + z] = zd{vx, vy};
     }
     __syncthreads();
     zd *res;
-    if constexpr (NN > 0) res = lds_fft_c<NN>(a, b, yt, tw, dir);
+    if constexpr (NN > 0) res = lds_fft_ip<NN, YT>(a, tw, dir);
     else res = lds_fft(a, b, P, yt, tw, dir);
     for (int e = threadIdx.x; e < yt * n; e += NT) {
         const int z = e / yt, yy = e - z * yt;
         if (yy >= ny) continue;
         const zd v = res[yy * n + z];
-        if (dir > 0) {
-            out[(size_t)(y0 + yy) + (size_t)g.Ny * (kx + (size_t)g.xkeep * z)] = v;
-        } else {
-            out[(size_t)(y0 + yy) + (size_t)g.Ny * (kx + (size_t)g.Xw * z)] =
-                z_retained(g, z) ? v : zd{0, 0};
-        }
+        const bool keep = dir > 0 || z_retained(g, z);
+        const size_t os = (size_t)g.Ny * (dir > 0 ? g.xkeep : g.Xw);
+        out[(size_t)(y0 + yy) + (size_t)g.Ny * kx + os * z] =
+            zd{keep ? v.x : 0.0, keep ? v.y : 0.0};
     }
 }
 
@@ -278,10 +293,10 @@ void k_fft_x_c2r(fft_geom g, radix_plan P, const zd *__restrict__ twx,
                  double *__restrict__ phys, long long sphys)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int n = NN ? NN : g.dNx, yt = YT ? YT : g.yt, np = yt / 2;
+    const int n = NN ? NN : g.dNx, yt = YT ? YT : g.ytx, np = yt / 2;
     zd *tw = reinterpret_cast<zd *>(smem);
     zd *a = tw + n;
-    zd *b = a + np * n;
+    zd *b = NN > 0 ? a : a + np * n;        // in place for constant sizes
     zd *spec = b + np * n;                  // [kx][yy], xkeep x yt
     const int ntile = (g.Ny + yt - 1) / yt;
     const int z = blockIdx.x / ntile;
@@ -314,7 +329,7 @@ void k_fft_x_c2r(fft_geom g, radix_plan P, const zd *__restrict__ twx,
     }
     __syncthreads();
     zd *res;
-    if constexpr (NN > 0) res = lds_fft_c<NN>(a, b, np, tw, +1);
+    if constexpr (NN > 0) res = lds_fft_ip<NN, YT / 2>(a, tw, +1);
     else res = lds_fft(a, b, P, np, tw, +1);
     for (int e = threadIdx.x; e < yt * n; e += NT) {
         const int yy = e / n, x = e - yy * n;
@@ -335,10 +350,10 @@ void k_fft_x_r2c(fft_geom g, radix_plan P, const zd *__restrict__ twx,
                  zd *__restrict__ mixed, long long smix)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int n = NN ? NN : g.dNx, yt = YT ? YT : g.yt, np = yt / 2;
+    const int n = NN ? NN : g.dNx, yt = YT ? YT : g.ytx, np = yt / 2;
     zd *tw = reinterpret_cast<zd *>(smem);
     zd *a = tw + n;
-    zd *b = a + np * n;
+    zd *b = NN > 0 ? a : a + np * n;        // in place for constant sizes
     const int ntile = (g.Ny + yt - 1) / yt;
     const int z = blockIdx.x / ntile;
     const int y0 = (blockIdx.x - z * ntile) * yt;
@@ -356,7 +371,7 @@ void k_fft_x_r2c(fft_geom g, radix_plan P, const zd *__restrict__ twx,
     }
     __syncthreads();
     zd *res;
-    if constexpr (NN > 0) res = lds_fft_c<NN>(a, b, np, tw, -1);
+    if constexpr (NN > 0) res = lds_fft_ip<NN, YT / 2>(a, tw, -1);
     else res = lds_fft(a, b, P, np, tw, -1);
     for (int e = threadIdx.x; e < g.xkeep * yt; e += NT) {
         const int kx = e / yt, yy = e - kx * yt;
@@ -392,16 +407,27 @@ std::vector<zd> twiddles(int n)
     return t;
 }
 
-size_t lds_z(const sz_fft_plan *p)
+// (length, tile) pairs compiled with constant sizes; they transform in place
+bool const_instance(int n, int yt)
 {
-    return ((size_t)p->g.dNz + 2 * (size_t)p->g.yt * p->g.dNz) * sizeof(zd);
+    return (yt == 16 || yt == 8) && (n == 144 || n == 96 || n == 192 || n == 72 || n == 48);
 }
 
-size_t lds_x(const sz_fft_plan *p)
+size_t lds_z_for(int dNz, int yt)
 {
-    return ((size_t)p->g.dNx + (size_t)p->g.yt * p->g.dNx +
-            (size_t)p->g.xkeep * p->g.yt) * sizeof(zd);
+    const size_t bufs = const_instance(dNz, yt) ? 1 : 2;
+    return ((size_t)dNz + bufs * (size_t)yt * dNz) * sizeof(zd);
 }
+
+size_t lds_x_for(int dNx, int xkeep, int yt)
+{
+    const size_t bufs = const_instance(dNx, yt) ? 1 : 2;
+    return ((size_t)dNx + bufs * (size_t)(yt / 2) * dNx + (size_t)xkeep * yt) * sizeof(zd);
+}
+
+size_t lds_z(const sz_fft_plan *p) { return lds_z_for(p->g.dNz, p->g.yt); }
+
+size_t lds_x(const sz_fft_plan *p) { return lds_x_for(p->g.dNx, p->g.xkeep, p->g.ytx); }
 
 }  // namespace
 
@@ -409,16 +435,26 @@ size_t lds_x(const sz_fft_plan *p)
 // constant sizes, else its runtime-size instance.
 #define SZ_FFT_LAUNCH(KERN, N_, YT_, GRID, LDS, ST, ...)                        \
     do {                                                                      \
-        if ((N_) == 144 && (YT_) == 8)                                        \
+        if ((N_) == 144 && (YT_) == 16)                                       \
+            hipLaunchKernelGGL((KERN<144, 16>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
+        else if ((N_) == 144 && (YT_) == 8)                                   \
             hipLaunchKernelGGL((KERN<144, 8>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
         else if ((N_) == 96 && (YT_) == 16)                                   \
             hipLaunchKernelGGL((KERN<96, 16>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
+        else if ((N_) == 96 && (YT_) == 8)                                    \
+            hipLaunchKernelGGL((KERN<96, 8>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
+        else if ((N_) == 192 && (YT_) == 16)                                  \
+            hipLaunchKernelGGL((KERN<192, 16>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
         else if ((N_) == 192 && (YT_) == 8)                                   \
             hipLaunchKernelGGL((KERN<192, 8>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
         else if ((N_) == 72 && (YT_) == 16)                                   \
             hipLaunchKernelGGL((KERN<72, 16>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
+        else if ((N_) == 72 && (YT_) == 8)                                    \
+            hipLaunchKernelGGL((KERN<72, 8>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
         else if ((N_) == 48 && (YT_) == 16)                                   \
             hipLaunchKernelGGL((KERN<48, 16>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
+        else if ((N_) == 48 && (YT_) == 8)                                    \
+            hipLaunchKernelGGL((KERN<48, 8>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
         else                                                                  \
             hipLaunchKernelGGL((KERN<0, 0>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
     } while (0)
@@ -446,12 +482,14 @@ sz_fft_plan *sz_fft_plan_create(int Ny, int Nx, int dNx, int Nz, int dNz,
     g.Xw = dNx / 2 + 1;
     g.xkeep = (Nx - 1) / 2 + 1;
     g.zabs = (Nz - 1) / 2;
-    const int nmax = dNx > dNz ? dNx : dNz;
+    // z: 16 lines per block (256-byte runs of y); x: 8 (measured faster
+    // for the c2r/r2c pair at 144)
     int yt = 16;
-    while (yt > 2 && ((size_t)nmax + 2 * (size_t)yt * nmax + (size_t)g.xkeep * yt) *
-                     sizeof(zd) > FFT_LDS)
-        yt /= 2;
+    while (yt > 2 && lds_z_for(dNz, yt) > FFT_LDS) yt /= 2;
+    int ytx = 8;
+    while (ytx > 2 && lds_x_for(dNx, g.xkeep, ytx) > FFT_LDS) ytx /= 2;
     g.yt = yt;
+    g.ytx = ytx;
     if (lds_z(p) > 160 * 1024 || lds_x(p) > 160 * 1024) {
         delete p;
         *info = SZ_ERR_LDS;
@@ -494,13 +532,13 @@ int sz_fft_wave_to_physical(const sz_fft_plan *p, int nfields,
     if (nfields < 0) return -2;
     if (nfields == 0) return 0;
     const fft_geom &g = p->g;
-    const int ntile = (g.Ny + g.yt - 1) / g.yt;
+    const int ntile = (g.Ny + g.yt - 1) / g.yt, ntx = (g.Ny + g.ytx - 1) / g.ytx;
     hipStream_t st = (hipStream_t)stream;
     SZ_FFT_LAUNCH(k_fft_z, g.dNz, g.yt, dim3(g.xkeep * ntile, nfields), lds_z(p),
                   st, g, p->pz, p->twz, +1,
                   reinterpret_cast<const zd *>(wave), swave,
                   reinterpret_cast<zd *>(mixed), smixed);
-    SZ_FFT_LAUNCH(k_fft_x_c2r, g.dNx, g.yt, dim3(g.dNz * ntile, nfields), lds_x(p),
+    SZ_FFT_LAUNCH(k_fft_x_c2r, g.dNx, g.ytx, dim3(g.dNz * ntx, nfields), lds_x(p),
                   st, g, p->px, p->twx,
                   reinterpret_cast<const zd *>(mixed), smixed, phys, sphys);
     return hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH;
@@ -515,9 +553,9 @@ int sz_fft_physical_to_wave(const sz_fft_plan *p, int nfields,
     if (nfields < 0) return -2;
     if (nfields == 0) return 0;
     const fft_geom &g = p->g;
-    const int ntile = (g.Ny + g.yt - 1) / g.yt;
+    const int ntile = (g.Ny + g.yt - 1) / g.yt, ntx = (g.Ny + g.ytx - 1) / g.ytx;
     hipStream_t st = (hipStream_t)stream;
-    SZ_FFT_LAUNCH(k_fft_x_r2c, g.dNx, g.yt, dim3(g.dNz * ntile, nfields), lds_x(p),
+    SZ_FFT_LAUNCH(k_fft_x_r2c, g.dNx, g.ytx, dim3(g.dNz * ntx, nfields), lds_x(p),
                   st, g, p->px, p->twx, phys, sphys,
                   reinterpret_cast<zd *>(mixed), smixed);
     SZ_FFT_LAUNCH(k_fft_z, g.dNz, g.yt, dim3(g.Xw * ntile, nfields), lds_z(p),
