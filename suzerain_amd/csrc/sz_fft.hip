@@ -104,17 +104,25 @@ __device__ zd *lds_fft(zd *a, zd *b, const radix_plan &P, int ntr,
             const int t = it / nr, i = it - t * nr;
             const int k = i % p;
             const zd *in = a + t * n;
+            // radix r <= 4 with constant indices (a register array indexed
+            // by a runtime count lives in scratch memory)
             zd x[4];
             const int tstep = k * (n / (p * r));
-            for (int q = 0; q < r; ++q) {
-                zd v = in[i + q * nr];
-                if (q > 0 && k > 0) v = sz::cmul(v, twiddle(tw, (q * tstep) % n, sign));
-                x[q] = v;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                x[q] = zd{0, 0};
+                if (q < r) {
+                    zd v = in[i + q * nr];
+                    if (q > 0 && k > 0) v = sz::cmul(v, twiddle(tw, (q * tstep) % n, sign));
+                    x[q] = v;
+                }
             }
             dft_small(r, sign, x);
             zd *out = b + t * n;
             const int j = (i - k) * r + k;
-            for (int q = 0; q < r; ++q) out[j + q * p] = x[q];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (q < r) out[j + q * p] = x[q];
         }
         __syncthreads();
         zd *tmp = a; a = b; b = tmp;
