@@ -219,7 +219,7 @@ struct wallbc {
 };
 
 constexpr int PK_NT = 256;
-constexpr int PK_RB = 8;  // wall-normal rows (5 columns each) per workgroup
+constexpr int PK_RB = 4;  // wall-normal rows (5 columns each) per workgroup
 
 // Block and group tables of the operator staged in LDS by the pack kernel,
 // with the band limits of M, D1, D2.
