@@ -19,7 +19,8 @@
 // sends out one row (U row j) and one column (the multipliers of column j).
 //   * k_gbtrf_wave: one wavefront per system, one window column per lane,
 //     the rows in registers; no barriers, about 400 VALU instructions per
-//     step.  Used for the band shapes of the channel operators.
+//     step (760 with the fused U^T solve).  Used for the band shapes of the
+//     channel operators.
 //   * k_gbtrf: one workgroup per system, the window in LDS as a circular
 //     buffer in both directions; wave 0 does the pivot search, the swap and
 //     the scaling, then all waves apply the rank-1 update.  Any band shape.
@@ -684,12 +685,20 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
 
     int c0 = 0, th = 0, ju = 0, first_zero = 0;
     int ob = 0;                                      // jb mod WR
+    const int lane_id = lane;
     for (int jb = 0; jb < n; jb += WV_BS) {
         bool replay = TCOL;                          // tcol is current
 #pragma unroll
         for (int u = 0; u < WV_BS; ++u) {
-            const int j = jb + u;
-            if (j >= n) break;
+            if (jb + u >= n) break;
+            // recompute step-derived values each step: hoisted out of the
+            // unrolled steps they become per-step induction variables and
+            // 64-bit lane masks that overflow the scalar registers (101
+            // spilled to VGPR lanes, 54 with these barriers)
+            int j = jb + u;
+            asm volatile("" : "+s"(j), "+s"(c0), "+s"(th), "+s"(ob));
+            int lane = lane_id;
+            asm volatile("" : "+v"(lane));
             const int j0 = ob + u >= WR ? ob + u - WR : ob + u;   // j mod WR
             const int km = min(kl, n - 1 - j);
             const int d = (lane - c0) & 63;          // this lane: column j + d
