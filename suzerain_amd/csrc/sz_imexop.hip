@@ -186,29 +186,10 @@ __device__ inline zd kappa(const double *k, double km, double kn)
     return zd{re, a + b};
 }
 
-__device__ inline double refv(const refs &rf, int t, int r)
-{
-    return t == ONE ? 1.0 : rf.p[t][(long long)r * rf.ld[t]];
-}
-
-// W_g[r] = sum_t alpha_t ref_t[r], first term a product, then fma.
-__device__ inline zd weight(const table *__restrict__ tab, const zd *alpha,
-                            const refs &rf, int g, int r)
-{
-    const int t0 = tab->gt0[g], nt = tab->gnt[g];
-    const double v0 = refv(rf, tab->tref[t0], r);
-    double wx = alpha[t0].x * v0, wy = alpha[t0].y * v0;
-    for (int t = t0 + 1; t < t0 + nt; ++t) {
-        const double v = refv(rf, tab->tref[t], r);
-        wx = fma(alpha[t].x, v, wx);
-        wy = fma(alpha[t].y, v, wy);
-    }
-    return zd{wx, wy};
-}
-
-// weight() with the term tables staged in LDS: the reference values of up
-// to four terms are loaded together, then combined in weight()'s order (the
-// first term a product, the rest fma), so the result is the same double.
+// W_g[r] = sum_t alpha_t ref_t[r] in term order, the first term a product,
+// the rest fma.  The term tables (gt0, gnt, tref) and the reference columns
+// (rp, rld) are staged in LDS by the caller; the reference values of up to
+// four terms are loaded together before they are combined.
 __device__ inline zd weight_staged(const int *gt0, const int *gnt, const int *tref,
                                    const double *const *rp, const int *rld,
                                    const zd *alpha, int g, int r)
@@ -400,7 +381,9 @@ void k_imexop_pack(const table *__restrict__ tab, bops w, refs rf, zd phi,
     }
 }
 
-constexpr int AC_NT = 256;
+// the x, D x buffers take most of a CU's LDS (one workgroup per CU), so the
+// workgroup is large enough to hide the latency of its own loads
+constexpr int AC_NT = 1024;
 
 // y_i <- (M + phi L) x + beta y_i for one system per workgroup.  Field f of
 // system s starts at in[f] + inoff(s) (complex elements, stride 1 in y).
@@ -423,12 +406,33 @@ void k_imexop_accumulate(const table *__restrict__ tab, bops w, refs rf, zd phi,
     const int s = blockIdx.x, tid = threadIdx.x;
     const zd *in[5] = {in0, in1, in2, in3, in4};
     zd *out[5] = {out0, out1, out2, out3, out4};
+    __shared__ int sbg0[25], sbng[25], sgop[MAXGROUPS], sgt0[MAXGROUPS], sgnt[MAXGROUPS];
+    __shared__ int stref[MAXTERMS], srld[NREF];
+    __shared__ const double *srp[NREF];
     const long long io = inoff ? inoff[s] : (long long)s * instride;
     const long long oo = outoff ? outoff[s] : (long long)s * outstride;
 
     const double kms = km[s], kns = kn[s];
-    for (int t = tid; t < tab->nterms; t += AC_NT)
+    for (int t = tid; t < tab->nterms; t += AC_NT) {
         alpha[t] = sz::cmul(phi, kappa(tab->tk[t], kms, kns));
+        stref[t] = tab->tref[t];
+    }
+    for (int g = tid; g < tab->ngroups; g += AC_NT) {
+        sgop[g] = tab->gop[g];
+        sgt0[g] = tab->gt0[g];
+        sgnt[g] = tab->gnt[g];
+    }
+    if (tid < 25) {
+        sbg0[tid] = tab->bg0[tid];
+        sbng[tid] = tab->bng[tid];
+    }
+    if (tid == 0) {
+#pragma unroll
+        for (int q = 0; q < NREF; ++q) {        // static indices: no private copy
+            srp[q] = rf.p[q];
+            srld[q] = rf.ld[q];
+        }
+    }
     for (int e = tid; e < 5 * n; e += AC_NT) {
         const int f = e / n, y = e - f * n;
         xs[e] = in[f] ? in[f][io + y] : zd{0, 0};
@@ -455,10 +459,10 @@ void k_imexop_accumulate(const table *__restrict__ tab, bops w, refs rf, zd phi,
         zd acc{0, 0};
         for (int j = 0; j < 5; ++j) {
             if (!in[j]) continue;
-            const int b = i * 5 + j, g0 = tab->bg0[b], ng = tab->bng[b];
+            const int b = i * 5 + j, g0 = sbg0[b], ng = sbng[b];
             for (int g = g0; g < g0 + ng; ++g) {
-                const zd W = weight(tab, alpha, rf, g, r);
-                acc = sz::cfma(acc, W, Dx[(tab->gop[g] * 5 + j) * n + r]);
+                const zd W = weight_staged(sgt0, sgnt, stref, srp, srld, alpha, g, r);
+                acc = sz::cfma(acc, W, Dx[(sgop[g] * 5 + j) * n + r]);
             }
         }
         zd *yp = out[i] + oo + r;
@@ -686,7 +690,7 @@ int sz_rholut_imexop_accumulate_batched(
     if (nsys == 0) return 0;
     const bops w = make_bops(op->w);
     const size_t lds = sizeof(zd) * (MAXTERMS + 20 * (size_t)w.n);
-    if (lds > 160 * 1024) return SZ_ERR_LDS;
+    if (lds > 156 * 1024) return SZ_ERR_LDS;   // 160 KB less the staged tables
     const int bz = beta[0] == 0 && beta[1] == 0;
     hipLaunchKernelGGL(k_imexop_accumulate, dim3(nsys), dim3(AC_NT), lds,
                        (hipStream_t)stream, op->dev, w, make_refs(r, ld),
