@@ -150,7 +150,7 @@ def lu_kernel_name(KL, KU, fused_ut=False):
 # scaled by a plain-copy calibration (profiles/gbtrf_4560_hbm_traffic_pmc.md),
 # keyed by (kernel, systems per launch).
 PMC_TRAFFIC = {("k_gbtrf<double, 256>", 4560): 10.15e9,
-               ("k_gbtrf_wave<double, 35, 68, 4, true>", 4560): 15.66e9}
+               ("k_gbtrf_wave<double, 35, 68, 4, true>", 4560): 15.69e9}
 
 RK3_CONFIG = dict(Nx=96, Ny=96, Nz=96, dNx=144, dNz=144, k=8, htdelta=3.0)
 
