@@ -316,10 +316,10 @@ def main_rk3(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the "
-              "implicit solves shard by wavenumber with no communication, so "
-              "whole-job throughput is the one-GPU value times the GPU count.",
-              file=sys.stderr)
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}): this "
+              "build has no multi-GPU path. The implicit solves would shard by "
+              "wavenumber with no communication; the explicit stage would add an "
+              "all-to-all pencil exchange per substep.", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
