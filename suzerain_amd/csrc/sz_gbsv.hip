@@ -599,7 +599,8 @@ __global__ __launch_bounds__(64)
 void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
                   int *__restrict__ ipiv, long long sipiv,
                   int *__restrict__ info, const int *__restrict__ skip,
-                  int nsys, cpx<R> *__restrict__ bv = nullptr, long long sb = 0)
+                  int nsys, cpx<R> *__restrict__ bv = nullptr, long long sb = 0,
+                  int prio = 0)
 {
     using T = cpx<R>;
     constexpr int kl = WR - 1, ku = KV - kl;
@@ -699,6 +700,13 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
             asm volatile("" : "+s"(j), "+s"(c0), "+s"(th), "+s"(ob));
             int lane = lane_id;
             asm volatile("" : "+v"(lane));
+            // with more systems than SIMDs but fewer than two per SIMD, the
+            // SIMDs holding two waves finish last: there the wave in its
+            // pivot chain issues ahead of the other's rank-1 update (the
+            // fused kernel runs the 4560-system substep batch, where the
+            // test alone costs 2%)
+            if constexpr (!UT)
+                if (prio) __builtin_amdgcn_s_setprio(3);
             const int j0 = ob + u >= WR ? ob + u - WR : ob + u;   // j mod WR
             const int km = min(kl, n - 1 - j);
             const int d = (lane - c0) & 63;          // this lane: column j + d
@@ -869,6 +877,8 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
             }
 
             // ---- rank-1 update of rows j+1 .. j+km, columns j+1 .. ju
+            if constexpr (!UT)
+                if (prio) __builtin_amdgcn_s_setprio(0);
             if (nz && km > 0) {
                 if (d >= 1 && d <= ju - j) {
 #pragma unroll
@@ -971,6 +981,23 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
     if (lane == 0 && info) info[sys] = first_zero;
 }
 
+// 1 when a batch of one-wave factorizations leaves some SIMDs with two waves
+// and most with one (between one and two systems per SIMD): 1152 systems
+// take 1.69 ms instead of 1.82 with the pivot chain prioritised; at two or
+// more waves per SIMD it costs 1%.
+int wave_priority(int nsys)
+{
+    static const int simds = [] {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                hipSuccess)
+            return 0;
+        return 4 * cus;
+    }();
+    return nsys > simds && nsys < 2 * simds ? 1 : 0;
+}
+
 // Factorization kernels: k_gbtrf_wave (one wavefront per system, window in
 // registers) for the band shapes it is instantiated for -- kl = ku = 34
 // (five interleaved fields, k = 8 operators), kl = ku = 24 (k = 6) and
@@ -998,7 +1025,8 @@ int launch_gbtrf(int n, int kl, int ku, void *ab, int ldab, long long sab,
 #define SZ_WAVE(WR_, KV_, BS_)                                                  \
         hipLaunchKernelGGL((k_gbtrf_wave<R, WR_, KV_, BS_>), dim3(nsys), dim3(64), 0, \
                            stream, n, reinterpret_cast<cpx<R> *>(ab), ldab, sab, ipiv, \
-                           sipiv, info, skip, nsys)
+                           sipiv, info, skip, nsys, nullptr, 0, prio)
+        const int prio = wave_priority(nsys);
         if (kl == 34 && ku == 34) {
             if (bs == 2) SZ_WAVE(35, 68, 2);
             else if (bs == 4) SZ_WAVE(35, 68, 4);
