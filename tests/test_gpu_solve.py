@@ -117,6 +117,27 @@ def test_gbtrf_gbtrs_match_oracle(gpu, N, kl, ku):
                 np.testing.assert_array_equal(got[s, r], want, err_msg=str((trans, s, r)))
 
 
+def test_gbtrf_large_batch_matches_small_batch(gpu):
+    """1152 systems (between one and two per SIMD on a 256-CU device, the
+    batch shape that runs with wave priority over the pivot chain) factor
+    bit-identically to the same systems in a 6-system batch, whose factors
+    the test above pins to the oracle."""
+    import suzerain_amd as sa
+    N, kl, ku = 480, 34, 34
+    rng = np.random.default_rng(1152)
+    small = _random_bands(rng, 6, N, kl, ku)
+    big = np.tile(small, (192, 1, 1))
+    sd, bd = _t(small, gpu), _t(big, gpu)
+    spiv, sinfo = sa.gbtrf(sd, kl, ku)
+    bpiv, binfo = sa.gbtrf(bd, kl, ku)
+    slu, blu = sd.cpu().numpy(), bd.cpu().numpy()
+    spiv, bpiv = spiv.cpu().numpy(), bpiv.cpu().numpy()
+    assert (sinfo.cpu().numpy() == 0).all() and (binfo.cpu().numpy() == 0).all()
+    for s in range(big.shape[0]):
+        np.testing.assert_array_equal(bpiv[s], spiv[s % 6], err_msg=str(s))
+        np.testing.assert_array_equal(blu[s], slu[s % 6], err_msg=str(s))
+
+
 def test_gbtrf_zero_pivot_info_and_gbsv_skip(gpu):
     import suzerain_amd as sa
     N, kl, ku, nb = 12, 2, 2, 3
