@@ -384,7 +384,9 @@ int sz_channel_apply_operator(sz_channel *h, double phi, int which);
 /* invert_mass_plus_scaled_operator (operator_hybrid_isothermal.cpp:528-689):
  * state <- (M + phi L)^{-1} state with isothermal walls; asynchronous. */
 int sz_channel_invert_operator(sz_channel *h, double phi, int which);
-/* Wait; return the first nonzero per-system solver info, else 0. */
+/* Wait; return the first nonzero per-system solver info recorded by any
+ * invert_operator since the last check (a zero pivot met in substep 1 is
+ * still reported after substep 3), else 0.  Clears the record. */
 int sz_channel_check(sz_channel *h);
 int sz_channel_add_scaled(sz_channel *h, double alpha, int src, int dst);
 int sz_channel_copy(sz_channel *h, int src, int dst);
@@ -410,6 +412,9 @@ void *sz_solver_create(const char *spec, int S, int n, int kl, int ku,
                        int errlen);
 void sz_solver_free(void *h);
 void *sz_solver_storage(void *h, int which, int *ld, long long *stride);
+/* Wait; first nonzero per-system info of any solve since the last call
+ * with clear != 0 (which then resets the record), else 0. */
+int sz_solver_first_info(void *h, int clear);
 void *sz_solver_zcgbsvx_stats(void *h, int which);
 int sz_solver_supply_B(void *h, const void *B, int ldB, int incB,
                        long long strideB);

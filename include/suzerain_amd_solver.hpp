@@ -82,6 +82,7 @@ public:
     long long stride_PB;     // = N * nrhs, leading dimension N
     int *ipiv;
     int *info;               // device, one per system (last solve)
+    int *info_first;         // device: first nonzero info since first_info(true)
 
     // PB(:, j) of every system <- P b_s (b_s at b + s*strideb, stride incb)
     int supply_b(const complex_double *b, int j, int incb, long long strideb);
@@ -102,8 +103,13 @@ public:
     // (suzerain/bsmbsm_solver.cpp:114-142) in last_error().
     int solve(char trans, int nrhs);
     int solve(char trans) { return solve(trans, this->nrhs); }
-    // Enqueue only: no wait, no info check (per-system info stays in `info`).
+    // Enqueue only: no wait, no info check (per-system info stays in `info`,
+    // and a nonzero one is also kept in `info_first`).
     int solve_async(char trans, int nrhs);
+    // Wait; the first nonzero per-system info of any solve since the last
+    // clearing call, else 0 (asynchronous callers check once per batch of
+    // solves, e.g. once per time step).
+    int first_info(bool clear);
 
     // x_s <- P^T PX(:, j) of every system
     int demand_x(complex_double *x, int j, int incx, long long stridex) const;
@@ -131,6 +137,7 @@ protected:
     std::string last_error_;
     std::vector<void *> factor_events_;   // hipEvent_t pairs
     void factor_mark();                    // record one event
+    void record_info();                    // fold info into info_first
 };
 
 class bsmbsm_solver_zgbsv : public bsmbsm_solver {

@@ -28,7 +28,7 @@ import math
 
 import numpy as np
 
-from . import ns
+from . import lowstorage, ns
 
 E, MX, MY, MZ, RHO = range(5)
 AUX = ("e_y e_yy div_grad_e e_x e_z "
@@ -37,14 +37,15 @@ AUX = ("e_y e_yy div_grad_e e_x e_z "
        "mz_y mz_yy mz_x mz_xx mz_xz mz_z mz_zz mz_xy mz_yz "
        "rho_y rho_yy rho_x rho_xx rho_xz rho_z rho_zz rho_xy rho_yz").split()
 
-# SMR91 (suzerain/lowstorage.hpp:1241-1305)
-SMR91_ALPHA = (29 / 96, -3 / 40, 1 / 6)
-SMR91_BETA = (37 / 160, 5 / 24, 1 / 6)
-SMR91_GAMMA = (8 / 15, 5 / 12, 3 / 4)
-SMR91_ZETA = (0.0, -17 / 60, -5 / 12)
-SMR91_ETA = (0.0, 8 / 15, 2 / 3)
-SMR91_EVMAXMAG_REAL = 2.51274532661832862402373
-SMR91_EVMAXMAG_IMAG = math.sqrt(3)
+# SMR91 (suzerain/lowstorage.hpp:1241-1305), pinned in
+# tests/test_oracle_lowstorage.py
+SMR91_ALPHA = lowstorage.SMR91.alpha
+SMR91_BETA = lowstorage.SMR91.beta
+SMR91_GAMMA = lowstorage.SMR91.gamma
+SMR91_ZETA = lowstorage.SMR91.zeta
+SMR91_ETA = tuple(lowstorage.SMR91.eta(i) for i in range(3))
+SMR91_EVMAXMAG_REAL = lowstorage.SMR91.evmaxmag_real
+SMR91_EVMAXMAG_IMAG = lowstorage.SMR91.evmaxmag_imag
 
 
 def inorder_wavenumber(N, i):
@@ -336,23 +337,40 @@ def _systems(g):
                 yield iz, ix
 
 
-def apply_operator(g, phi, scen5, R, x, beta=0.0, y=None):
+def _for_systems(g, fn, threads=1, systems=None):
+    """Run fn(iz, ix) over the retained wavenumbers (or the given list),
+    spread over host threads; the C restatement releases the GIL."""
+    todo = list(_systems(g)) if systems is None else list(systems)
+    if threads <= 1:
+        for iz, ix in todo:
+            fn(iz, ix)
+        return
+    import concurrent.futures
+    chunks = [todo[i::threads] for i in range(threads)]
+    with concurrent.futures.ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda c: [fn(iz, ix) for iz, ix in c], chunks))
+
+
+def apply_operator(g, phi, scen5, R, x, beta=0.0, y=None, threads=1):
     """y <- (M + phi L) x + beta y on retained wavenumbers (oracle imexop);
     x, y (5, dNz, Xw, Ny); non-retained entries of y are left as they are."""
     from . import imexop_accumulate
     out = np.array(x if y is None else y, dtype=complex, copy=True)
-    for iz, ix in _systems(g):
+
+    def one(iz, ix):
         xs = np.ascontiguousarray(x[:, iz, ix, :])
         ys = np.ascontiguousarray(out[:, iz, ix, :])
         imexop_accumulate(phi, g.kx[ix], g.kz[iz], scen5, R, g.ops, xs,
                           beta, ys)
         out[:, iz, ix, :] = ys
+    _for_systems(g, one, threads)
     return out
 
 
-def invert_operator(g, phi, scen5, R, x, bc):
+def invert_operator(g, phi, scen5, R, x, bc, threads=1, systems=None):
     """(M + phi L)^{-1} x with isothermal walls, per retained wavenumber
-    (operator_hybrid_isothermal.cpp:612-689); bc = (lower, upper) (T,u,v,w)."""
+    (operator_hybrid_isothermal.cpp:612-689); bc = (lower, upper) (T,u,v,w).
+    systems: only these (iz, ix) pencils are solved (the others stay 0)."""
     from . import (imexop_pack, isothermal_op, isothermal_rhs, zaPxpby,
                    zgbtrf, zgbtrs)
     S, n = 5, g.Ny
@@ -362,7 +380,8 @@ def invert_operator(g, phi, scen5, R, x, bc):
     N = S * n
     ld = LD + KL
     out = np.zeros_like(x)
-    for iz, ix in _systems(g):
+
+    def one(iz, ix):
         ab = np.zeros(N * ld, dtype=complex)
         imexop_pack(phi, g.kx[ix], g.kz[iz], scen5, R, g.ops, ab, ld, offset=KL)
         isothermal_op(S, n, KL, KU, ab, ld, scen5, bc[0], bc[1], offset=KL)
@@ -377,33 +396,51 @@ def invert_operator(g, phi, scen5, R, x, bc):
         v = np.zeros(N, dtype=complex)
         zaPxpby("T", S, n, 1, p, 0, v)
         out[:, iz, ix, :] = v.reshape(S, n)
+    _for_systems(g, one, threads, systems)
     return out
 
 
-def smr91_step(g, a, scenario, bc, lam, max_dt=0.0):
+class HybridOperator:
+    """The linear operator of the channel step: M + phi L with isothermal
+    walls on every retained wavenumber (operator_hybrid_isothermal.cpp
+    :103-374 apply/accumulate, :528-689 invert), as lowstorage.step uses it."""
+
+    def __init__(self, g, scen5, R, bc, threads=1):
+        self.g, self.scen5, self.R, self.bc = g, scen5, R, bc
+        self.threads = threads
+
+    def apply(self, phi, a):
+        return apply_operator(self.g, phi, self.scen5, self.R, a,
+                              threads=self.threads)
+
+    def accumulate(self, phi, a, beta, b):
+        return apply_operator(self.g, phi, self.scen5, self.R, a, beta=beta,
+                              y=b, threads=self.threads)
+
+    def invert(self, phi, a):
+        return invert_operator(self.g, phi, self.scen5, self.R, a, self.bc,
+                               threads=self.threads)
+
+
+def smr91_step(g, a, scenario, bc, lam, max_dt=0.0, threads=1):
     """lowstorage::step (lowstorage.hpp:1461-1520) for the hybrid operator:
-    returns (new a, dt, refs)."""
+    returns (new a, dt, refs).  The references are collected on substep 0
+    and the implicit operator built from them for the whole step."""
     scen5 = [scenario[k] for k in ("Re", "Pr", "Ma", "alpha", "gamma")]
     chi = 1.0 / (g.dNx * g.dNz)
-    b = a.copy()
-    b, refs, cand = apply_ns(g, b, scenario, "rhome_xyz", zeroth=True, lam=lam)
-    dt = cand[0]
-    for c in cand[1:]:
-        if dt != dt:
-            break
-        if c != c or c < dt:
-            dt = c
-    if max_dt > 0 and dt == dt:
-        dt = min(max_dt, dt)
-    R = ns.imexop_refs(refs)
-    a = apply_operator(g, dt * SMR91_ALPHA[0], scen5, R, a)
-    a = a + chi * dt * SMR91_GAMMA[0] * b
-    a = invert_operator(g, -dt * SMR91_BETA[0], scen5, R, a, bc)
-    for i in (1, 2):
-        b = apply_operator(g, dt * SMR91_ALPHA[i], scen5, R, a,
-                           beta=chi * dt * SMR91_ZETA[i], y=b)
-        a, b = b, a
-        b, _, _ = apply_ns(g, b, scenario, "rhome_xyz", refs=refs, zeroth=False)
-        a = a + chi * dt * SMR91_GAMMA[i] * b
-        a = invert_operator(g, -dt * SMR91_BETA[i], scen5, R, a, bc)
-    return a, dt, refs
+    L = HybridOperator(g, scen5, None, bc, threads)
+    held = {}
+
+    def N(b, i):
+        if i == 0:
+            b, refs, cand = apply_ns(g, b, scenario, "rhome_xyz", zeroth=True,
+                                     lam=lam)
+            held["refs"] = refs
+            L.R = ns.imexop_refs(refs)
+            return b, cand
+        b, _, _ = apply_ns(g, b, scenario, "rhome_xyz", refs=held["refs"],
+                           zeroth=False)
+        return b, None
+
+    a, dt = lowstorage.step(L, chi, N, a, max_dt=max_dt)
+    return a, dt, held["refs"]

@@ -131,6 +131,7 @@ SIGNATURES = {
                               C.c_char_p, i32]),
     "sz_solver_free": (None, [vp]),
     "sz_solver_storage": (vp, [vp, i32, pi, C.POINTER(C.c_longlong)]),
+    "sz_solver_first_info": (i32, [vp, i32]),
     "sz_solver_zcgbsvx_stats": (vp, [vp, i32]),
     "sz_solver_supply_B": (i32, [vp, vp, i32, i32, i64]),
     "sz_solver_demand_X": (i32, [vp, vp, i32, i32, i64]),

@@ -713,18 +713,14 @@ int sz_channel_invert_operator(sz_channel *h, double phi, int which)
 }
 
 // Wait for the stream and report the first system whose factorization met
-// an exact zero pivot (its info), or 0.
+// an exact zero pivot in any solve since the last check (its info), or 0;
+// the solver's record is then cleared.  Every factorization launch rewrites
+// the per-system info, so a zero pivot met in an early substep is carried by
+// the solver's first-status record to the end of the step.
 int sz_channel_check(sz_channel *h)
 {
     if (!h) return -1;
-    std::vector<int> info((size_t)h->nsys);
-    if (hipMemcpyAsync(info.data(), h->solver->info, sizeof(int) * h->nsys,
-                       hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
-        hipStreamSynchronize(h->stream) != hipSuccess)
-        return SZ_ERR_LAUNCH;
-    for (int s = 0; s < h->nsys; ++s)
-        if (info[(size_t)s]) return info[(size_t)s];
-    return 0;
+    return h->solver->first_info(true);
 }
 
 int sz_channel_add_scaled(sz_channel *h, double alpha, int src, int dst)

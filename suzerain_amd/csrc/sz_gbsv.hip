@@ -1204,14 +1204,14 @@ int sz_internal_zgbtrf_ut(int n, int kl, int ku, void *ab, int ldab,
 }
 
 // The rest of ZGBTRS 'T' after sz_internal_zgbtrf_ut: L^T x = y and the
-// interchanges.
+// interchanges; systems with skip[s] != 0 (a zero pivot) are left alone.
 int sz_internal_zgbtrs_lt(int n, int kl, int ku, const void *ab, int ldab,
                           long long strideab, const int *ipiv,
                           long long strideipiv, void *b, long long strideb,
-                          int nbatch, void *stream)
+                          const int *skip, int nbatch, void *stream)
 {
     return launch_gbtrs<double>('T', n, kl, ku, 1, ab, ldab, strideab, ipiv,
-                                strideipiv, b, n, strideb, nullptr, nbatch,
+                                strideipiv, b, n, strideb, skip, nbatch,
                                 (hipStream_t)stream, 1);
 }
 

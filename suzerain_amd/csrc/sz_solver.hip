@@ -230,7 +230,10 @@ bsmbsm_solver::bsmbsm_solver(const sz_bsmbsm &bsmbsm,
     PB = dalloc<complex_double>((size_t)stride_PB * nbatch);
     ipiv = dalloc<int>((size_t)N * nbatch);
     info = dalloc<int>((size_t)nbatch);
+    info_first = dalloc<int>((size_t)nbatch);
     (void)hipMemsetAsync(info, 0, sizeof(int) * (nbatch > 0 ? nbatch : 1),
+                         (hipStream_t)stream);
+    (void)hipMemsetAsync(info_first, 0, sizeof(int) * (nbatch > 0 ? nbatch : 1),
                          (hipStream_t)stream);
     // zgbsv factors in place: PAPT aliases LU + KL, PX aliases PB
     PAPT = LU + KL;
@@ -248,6 +251,45 @@ bsmbsm_solver::~bsmbsm_solver()
     dfree(PB);
     dfree(ipiv);
     dfree(info);
+    dfree(info_first);
+}
+
+namespace {
+
+// info_first[s] keeps the first nonzero info of system s since the last
+// clear (or, with clear_value, is reset).
+__global__ void k_info_first(int *__restrict__ first, const int *__restrict__ info,
+                             int nbatch, int clear)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nbatch) return;
+    if (clear) first[s] = 0;
+    else if (first[s] == 0) first[s] = info[s];
+}
+
+}  // namespace
+
+void bsmbsm_solver::record_info()
+{
+    if (nbatch > 0)
+        hipLaunchKernelGGL(k_info_first, dim3((nbatch + 255) / 256), dim3(256), 0,
+                           (hipStream_t)stream, info_first, info, nbatch, 0);
+}
+
+int bsmbsm_solver::first_info(bool clear)
+{
+    if (nbatch == 0) return 0;
+    std::vector<int> h((size_t)nbatch);
+    if (hipMemcpyAsync(h.data(), info_first, sizeof(int) * nbatch,
+                       hipMemcpyDeviceToHost, (hipStream_t)stream) != hipSuccess ||
+        hipStreamSynchronize((hipStream_t)stream) != hipSuccess)
+        return SZ_ERR_LAUNCH;
+    int first = 0;
+    for (int s = 0; s < nbatch && !first; ++s) first = h[(size_t)s];
+    if (clear && first)
+        hipLaunchKernelGGL(k_info_first, dim3((nbatch + 255) / 256), dim3(256), 0,
+                           (hipStream_t)stream, info_first, info, nbatch, 1);
+    return first;
 }
 
 bsmbsm_solver &bsmbsm_solver::supplied_PAPT()
@@ -313,7 +355,9 @@ int bsmbsm_solver::solve_async(char trans, int nrhs_)
     if (nrhs_ == 0) return 0;
     if (nrhs_ < 1 || nrhs_ > nrhs)
         throw std::invalid_argument("Invalid nrhs supplied to solve()");
-    return solve_hook(trans, nrhs_);
+    const int rc = solve_hook(trans, nrhs_);
+    if (rc == 0) record_info();
+    return rc;
 }
 
 int bsmbsm_solver::solve(char trans, int nrhs_)
@@ -414,8 +458,15 @@ extern "C" int sz_internal_zgbtrf_ut(int n, int kl, int ku, void *ab, int ldab,
 extern "C" int sz_internal_zgbtrs_lt(int n, int kl, int ku, const void *ab,
                                      int ldab, long long strideab,
                                      const int *ipiv, long long strideipiv,
-                                     void *b, long long strideb, int nbatch,
-                                     void *stream);
+                                     void *b, long long strideb,
+                                     const int *skip, int nbatch, void *stream);
+extern "C" int sz_internal_gbtrs_masked(int single, char trans, int n, int kl,
+                                        int ku, int nrhs, const void *ab,
+                                        int ldab, long long strideab,
+                                        const int *ipiv, long long strideipiv,
+                                        void *b, int ldb, long long strideb,
+                                        const int *skip, int nbatch,
+                                        void *stream);
 
 int bsmbsm_solver_zgbsv::solve_hook(char trans, int nrhs_)
 {
@@ -430,8 +481,11 @@ int bsmbsm_solver_zgbsv::solve_hook(char trans, int nrhs_)
         if (time_factor) factor_mark();
         if (rc == 0) {
             fact_ = 'F';
+            // a singular system's PB already holds part of U^T y (the
+            // reference's contract lets solve destroy PB); its L^T sweep is
+            // skipped so no inf/NaN arithmetic follows
             return sz_internal_zgbtrs_lt(N, KL, KU, LU, ld_LU, stride_LU, ipiv, N,
-                                         PB, stride_PB, nbatch, stream);
+                                         PB, stride_PB, info, nbatch, stream);
         }
         if (rc != 1) return rc;
         rc = 0;
@@ -444,11 +498,30 @@ int bsmbsm_solver_zgbsv::solve_hook(char trans, int nrhs_)
         fact_ = 'F';
         if (rc) return rc;
     }
-    return sz_zgbtrs_batched(trans, N, KL, KU, nrhs_, LU, ld_LU, stride_LU, ipiv,
-                             N, PB, N, stride_PB, nbatch, stream);
+    // systems whose factorization met a zero pivot keep PB, as ZGBSV does
+    return sz_internal_gbtrs_masked(0, trans, N, KL, KU, nrhs_, LU, ld_LU,
+                                    stride_LU, ipiv, N, PB, N, stride_PB, info,
+                                    nbatch, stream);
 }
 
 // ---------------------------------------------------------------- zcgbsvx
+
+namespace {
+
+// Per-system refinement state: fact (when set_fact), apprx, and the cached
+// Frobenius norm (-1: not yet known, as the reference constructor's
+// afrob_(-1), bsmbsm_solver.cpp:298).
+__global__ void k_fill_state(char *fact, int *apprx, double *afrob, char f,
+                             int a, int set_fact, int nbatch)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nbatch) return;
+    if (set_fact) fact[s] = f;
+    apprx[s] = a;
+    afrob[s] = -1;
+}
+
+}  // namespace
 
 bsmbsm_solver_zcgbsvx::bsmbsm_solver_zcgbsvx(const sz_bsmbsm &b,
                                              const specification_zgbsv &spec,
@@ -474,9 +547,11 @@ bsmbsm_solver_zcgbsvx::bsmbsm_solver_zcgbsvx(const sz_bsmbsm &b,
     res_dev = dalloc<double>((size_t)nbatch);
     r_dev = dalloc<complex_double>((size_t)N * nbatch);
     work_ = dalloc<char>((size_t)sz_zcgbsvx_workspace_bytes(nbatch));
-    std::vector<char> f((size_t)(nbatch > 0 ? nbatch : 1), 'N');
-    (void)hipMemcpy(fact_dev, f.data(), f.size(), hipMemcpyHostToDevice);
-    (void)hipMemset(apprx_dev, 0, sizeof(int) * (nbatch > 0 ? nbatch : 1));
+    // no factorization yet, nothing approximate, ||A||_F unknown
+    if (nbatch > 0)
+        hipLaunchKernelGGL(k_fill_state, dim3((nbatch + 255) / 256), dim3(256), 0,
+                           (hipStream_t)stream, fact_dev, apprx_dev, afrob_dev,
+                           'N', 0, 1, nbatch);
 }
 
 bsmbsm_solver_zcgbsvx::~bsmbsm_solver_zcgbsvx()
@@ -491,20 +566,6 @@ bsmbsm_solver_zcgbsvx::~bsmbsm_solver_zcgbsvx()
     dfree(r_dev);
     dfree(work_);
 }
-
-namespace {
-
-__global__ void k_fill_state(char *fact, int *apprx, double *afrob, char f,
-                             int a, int set_fact, int nbatch)
-{
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nbatch) return;
-    if (set_fact) fact[s] = f;
-    apprx[s] = a;
-    afrob[s] = -1;
-}
-
-}  // namespace
 
 bsmbsm_solver &bsmbsm_solver_zcgbsvx::supplied_PAPT()
 {
@@ -613,6 +674,11 @@ void *sz_solver_storage(void *h, int which, int *ld, long long *stride)
     case 5: *ld = 1; *stride = 1; return s->info;
     }
     return nullptr;
+}
+
+int sz_solver_first_info(void *h, int clear)
+{
+    return static_cast<bsmbsm_solver *>(h)->first_info(clear != 0);
 }
 
 void *sz_solver_zcgbsvx_stats(void *h, int which)

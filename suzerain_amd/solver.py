@@ -141,6 +141,11 @@ class BsmbsmSolver:
         self.last_error = err.value.decode()
         return rc
 
+    def first_info(self, clear=True):
+        """First nonzero per-system info of any solve since the last clearing
+        call (waits for the device), else 0."""
+        return load().sz_solver_first_info(self._h, int(bool(clear)))
+
     def solve_async(self, trans, nrhs=None):
         return load().sz_solver_solve_async(self._h, trans.encode(),
                                             self.nrhs if nrhs is None else nrhs)

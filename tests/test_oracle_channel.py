@@ -119,6 +119,65 @@ def test_dt_candidates(grid):
     assert c2[0] > 0
 
 
+def _uniform_moving(Ny, nz, nx, rho0, vel, T0):
+    """Physical fields of a uniform state with velocity vel and temperature
+    T0 (p = rho T / gamma, e = p / (gamma - 1) + Ma^2 rho |u|^2 / 2) and
+    vanishing derivatives."""
+    gamma, Ma = SCEN["gamma"], SCEN["Ma"]
+    P = np.empty((5, Ny, nz, nx))
+    p = rho0 * T0 / gamma
+    P[0] = p / (gamma - 1) + Ma * Ma / 2 * rho0 * sum(v * v for v in vel)
+    for f, v in zip((1, 2, 3), vel):
+        P[f] = rho0 * v
+    P[4] = rho0
+    A = {k: np.zeros((Ny, nz, nx)) for k in channel.AUX}
+    return P, A
+
+
+@pytest.mark.parametrize("lin", ["none", "rhome_xyz"])
+def test_dt_candidates_closed_form_uniform_flow(lin):
+    """navier_stokes.hpp:766-892 on a uniform moving state: every point has
+    the same u, T and nu = T^beta / rho, so the 12 candidates reduce to
+    closed forms in the extreme wall-normal scales.  Distinct U, V, W and
+    distinct x, y, z scales catch a swapped direction, a missing sound
+    speed or an evmaxmag mix-up."""
+    Ny, nz, nx = 7, 4, 6
+    rho0, vel, T0 = 1.3, (0.3, -0.2, 0.11), 1.7
+    P, A = _uniform_moving(Ny, nz, nx, rho0, vel, T0)
+    refs = channel.references(None, SCEN, P)
+    l1x, l1z = 3.0, 5.0
+    l1y = np.linspace(2.0, 11.0, Ny)[::-1].copy()
+    l2y = np.linspace(4.0, 130.0, Ny)
+    lam = (l1x, l1x * l1x, l1z, l1z * l1z, l1y, l2y)
+    evr, evi = channel.SMR91_EVMAXMAG_REAL, channel.SMR91_EVMAXMAG_IMAG
+    _, c = channel.rhs_physical(SCEN, P, A, refs, lin, lam, (evr, evi))
+    a = math.sqrt(T0) / SCEN["Ma"] if lin == "none" else 0.0
+    U, V, W = (abs(v) + a for v in vel)
+    want = np.empty(12)
+    tx, ty, tz = U * l1x, V * l1y.max(), W * l1z
+    want[0] = evi / (tx + ty + tz)
+    want[3], want[6], want[9] = evi / tx, evi / ty, evi / tz
+    nu = T0 ** SCEN["beta"] / rho0
+    md = (1 / SCEN["Re"]) * max(SCEN["gamma"] / SCEN["Pr"], max(1.0, SCEN["alpha"]))
+    dx_, dy_, dz_ = nu * md * l1x * l1x, nu * md * l2y.max(), nu * md * l1z * l1z
+    want[2] = evr / (nu * md * (l1x * l1x + l2y.max() + l1z * l1z))
+    want[5], want[8], want[11] = evr / dx_, evr / dy_, evr / dz_
+    conv = [0, 3, 6, 9]
+    np.testing.assert_allclose(c[conv], want[conv], rtol=1e-14)
+    if lin == "none":
+        # no reference: fluctuating candidates equal the total ones and the
+        # whole viscosity is diffusive
+        np.testing.assert_allclose(c[[1, 4, 7, 10]], want[conv], rtol=1e-14)
+        np.testing.assert_allclose(c[[2, 5, 8, 11]], want[[2, 5, 8, 11]],
+                                   rtol=1e-14)
+    else:
+        # the reference profiles equal the state: the fluctuating velocity
+        # and the excess viscosity vanish (to rounding), leaving the
+        # candidates at or near their initial maximum
+        assert (c[[1, 4, 7, 10]] > 1e12).all(), c
+        assert (c[[2, 5, 8, 11]] > 1e12).all(), c
+
+
 def test_smr91_step_keeps_rest_state(grid):
     """A uniform state at rest with T = 1 and isothermal T = 1 walls is a
     steady solution; one SMR91 step must leave it unchanged."""
