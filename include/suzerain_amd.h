@@ -369,7 +369,7 @@ void *sz_channel_state(sz_channel *h, int which);
 /* Internal buffers for inspection: 0 refs (42 x Ny), 1 physical fields
  * (46), 2 wave derivative fields (41), 3 wave values (5), 4 km, 5 kn,
  * 6 per-system pencil offsets, 7 lambda1_y then lambda2_y, 8 the
- * sz_bsplineop, 9 the suzerain_amd::bsmbsm_solver. */
+ * sz_bsplineop, 9 the suzerain_amd::bsmbsm_solver of slot 0. */
 void *sz_channel_buffer(sz_channel *h, int which);
 /* operator_nonlinear::apply_operator (navier_stokes.hpp:107-986): state
  * `which` <- N(state) in place; on zeroth also the references and the 12
@@ -384,6 +384,28 @@ int sz_channel_apply_operator(sz_channel *h, double phi, int which);
 /* invert_mass_plus_scaled_operator (operator_hybrid_isothermal.cpp:528-689):
  * state <- (M + phi L)^{-1} state with isothermal walls; asynchronous. */
 int sz_channel_invert_operator(sz_channel *h, double phi, int which);
+/* As sz_channel_invert_operator with factorization slot `slot`.  A channel
+ * whose solver spec is zcgbsvx with reuse=true keeps three slots, one per
+ * SMR91 substep: the factors made at substep i serve substep i of later steps
+ * as an approximate factorization (bsmbsm_solver::apprx semantics, re-factored
+ * when refinement stalls); other specs keep one slot. */
+int sz_channel_invert_operator_slot(sz_channel *h, double phi, int which,
+                                    int slot);
+int sz_channel_solver_slots(const sz_channel *h);
+/* The slot's suzerain_amd::bsmbsm_solver (C handle of sz_solver_*); NULL
+ * for zcgbsvx, which the channel solves with its own mixed precision solver:
+ * complex float factors of the assembled operator refined with double
+ * precision residuals of the matrix-free operator (dsgbsvx.def:124-313
+ * semantics per system: restart of stalled reused factors, double precision
+ * fallback). */
+void *sz_channel_solver(sz_channel *h, int slot);
+/* zcgbsvx channels: device array of each system's outcome of the last
+ * solve (0 siter int, 1 diter int, 2 tolsc double (res / tol), 3 res
+ * double, 4 fact char, 5 apprx int, 6 info int), else NULL; and the number
+ * of systems factored, restarted and sent to double precision by it. */
+void *sz_channel_solver_stats(sz_channel *h, int which);
+int sz_channel_solver_counts(sz_channel *h, int *factored, int *restarted,
+                             int *fallback);
 /* Wait; return the first nonzero per-system solver info recorded by any
  * invert_operator since the last check (a zero pivot met in substep 1 is
  * still reported after substep 3), else 0.  Clears the record. */
@@ -396,6 +418,12 @@ void sz_channel_exchange(sz_channel *h);
  * switch the bracketing on or off.  HIP events on the solver's stream. */
 int sz_channel_factor_timing(sz_channel *h, int enable, double *total_ms,
                              int *count);
+/* Device time of each substep of the steps taken since the last call (HIP
+ * events on the channel's stream at the substep boundaries; substep 0
+ * includes the wait for the step candidates): up to max values into ms,
+ * their number in count; then (enable >= 0) switch the recording on/off. */
+int sz_channel_substep_timing(sz_channel *h, int enable, double *ms, int max,
+                              int *count);
 /* lowstorage::step (lowstorage.hpp:1461-1520) with SMR91 and the
  * delta_t_reducer; dt_out receives the step taken. */
 int sz_channel_step(sz_channel *h, double max_dt, double *dt_out);

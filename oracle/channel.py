@@ -367,12 +367,18 @@ def apply_operator(g, phi, scen5, R, x, beta=0.0, y=None, threads=1):
     return out
 
 
-def invert_operator(g, phi, scen5, R, x, bc, threads=1, systems=None):
+def invert_operator(g, phi, scen5, R, x, bc, threads=1, systems=None,
+                    lapack=False):
     """(M + phi L)^{-1} x with isothermal walls, per retained wavenumber
     (operator_hybrid_isothermal.cpp:612-689); bc = (lower, upper) (T,u,v,w).
-    systems: only these (iz, ix) pencils are solved (the others stay 0)."""
-    from . import (imexop_pack, isothermal_op, isothermal_rhs, zaPxpby,
-                   zgbtrf, zgbtrs)
+    systems: only these (iz, ix) pencils are solved (the others stay 0).
+    lapack: factor and solve with LAPACK ZGBTRF/ZGBTRS (oracle.lapack, the
+    CPU baseline) instead of the restatement."""
+    from . import imexop_pack, isothermal_op, isothermal_rhs, zaPxpby
+    if lapack:
+        from .lapack import zgbtrf, zgbtrs
+    else:
+        from . import zgbtrf, zgbtrs
     S, n = 5, g.Ny
     KL = S * (g.ops.max_kl + 1) - 1
     KU = S * (g.ops.max_ku + 1) - 1
@@ -405,9 +411,9 @@ class HybridOperator:
     walls on every retained wavenumber (operator_hybrid_isothermal.cpp
     :103-374 apply/accumulate, :528-689 invert), as lowstorage.step uses it."""
 
-    def __init__(self, g, scen5, R, bc, threads=1):
+    def __init__(self, g, scen5, R, bc, threads=1, lapack=False):
         self.g, self.scen5, self.R, self.bc = g, scen5, R, bc
-        self.threads = threads
+        self.threads, self.lapack = threads, lapack
 
     def apply(self, phi, a):
         return apply_operator(self.g, phi, self.scen5, self.R, a,
@@ -419,16 +425,16 @@ class HybridOperator:
 
     def invert(self, phi, a):
         return invert_operator(self.g, phi, self.scen5, self.R, a, self.bc,
-                               threads=self.threads)
+                               threads=self.threads, lapack=self.lapack)
 
 
-def smr91_step(g, a, scenario, bc, lam, max_dt=0.0, threads=1):
+def smr91_step(g, a, scenario, bc, lam, max_dt=0.0, threads=1, lapack=False):
     """lowstorage::step (lowstorage.hpp:1461-1520) for the hybrid operator:
     returns (new a, dt, refs).  The references are collected on substep 0
     and the implicit operator built from them for the whole step."""
     scen5 = [scenario[k] for k in ("Re", "Pr", "Ma", "alpha", "gamma")]
     chi = 1.0 / (g.dNx * g.dNz)
-    L = HybridOperator(g, scen5, None, bc, threads)
+    L = HybridOperator(g, scen5, None, bc, threads, lapack)
     held = {}
 
     def N(b, i):

@@ -43,7 +43,8 @@ def build(verbose=False):
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
     cmd = ["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs,
-           "-o", tmp]
+           "-o", tmp, "-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx",
+           "-Wl,-rpath,/opt/rocm/lib", "-Wl,-z,defs"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr}")

@@ -58,6 +58,13 @@ _SIGS = {
     "sz_channel_apply_operator": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
     "sz_channel_invert_operator": (C.c_int, [C.c_void_p, C.c_double,
                                              C.c_int]),
+    "sz_channel_invert_operator_slot": (C.c_int, [C.c_void_p, C.c_double,
+                                                  C.c_int, C.c_int]),
+    "sz_channel_solver_slots": (C.c_int, [C.c_void_p]),
+    "sz_channel_solver": (C.c_void_p, [C.c_void_p, C.c_int]),
+    "sz_channel_solver_stats": (C.c_void_p, [C.c_void_p, C.c_int]),
+    "sz_channel_solver_counts": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_void_p]),
     "sz_channel_check": (C.c_int, [C.c_void_p]),
     "sz_channel_add_scaled": (C.c_int, [C.c_void_p, C.c_double, C.c_int,
                                         C.c_int]),
@@ -66,6 +73,8 @@ _SIGS = {
     "sz_channel_step": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p]),
     "sz_channel_factor_timing": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p,
                                            C.c_void_p]),
+    "sz_channel_substep_timing": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p,
+                                            C.c_int, C.c_void_p]),
 }
 
 
@@ -187,12 +196,40 @@ class Channel:
         check(_lib().sz_channel_apply_operator(self._h, phi, which),
               "sz_channel_apply_operator")
 
-    def invert_operator(self, phi, which=0):
-        check(_lib().sz_channel_invert_operator(self._h, phi, which),
-              "sz_channel_invert_operator")
+    def invert_operator(self, phi, which=0, slot=0):
+        check(_lib().sz_channel_invert_operator_slot(self._h, phi, which, slot),
+              "sz_channel_invert_operator_slot")
         rc = _lib().sz_channel_check(self._h)
         if rc:
             raise ArithmeticError(f"implicit solve: zero pivot, info {rc}")
+
+    def solver_slots(self):
+        """Factorization slots: 3 for zcgbsvx with reuse (one per substep)."""
+        return _lib().sz_channel_solver_slots(self._h)
+
+    def solver_stats(self, which):
+        """Per-system outcome of the last mixed precision (zcgbsvx) solve:
+        "siter", "diter", "tolsc", "res", "fact", "apprx" or "info" (None
+        for zgbsv)."""
+        from .solver import device_to_host
+        k = {"siter": 0, "diter": 1, "tolsc": 2, "res": 3, "fact": 4,
+             "apprx": 5, "info": 6}[which]
+        p = _lib().sz_channel_solver_stats(self._h, k)
+        if not p:
+            return None
+        dtype = {"siter": np.int32, "diter": np.int32, "tolsc": np.float64,
+                 "res": np.float64, "fact": np.int8, "apprx": np.int32,
+                 "info": np.int32}[which]
+        a = device_to_host(p, self.nsys * np.dtype(dtype).itemsize, dtype)
+        return a.view("S1") if which == "fact" else a
+
+    def solver_counts(self):
+        """(systems factored, restarted, sent to double precision) by the
+        last mixed precision solve."""
+        f, r, d = C.c_int(0), C.c_int(0), C.c_int(0)
+        rc = _lib().sz_channel_solver_counts(self._h, C.byref(f), C.byref(r),
+                                             C.byref(d))
+        return None if rc else (f.value, r.value, d.value)
 
     def add_scaled(self, alpha, src, dst):
         check(_lib().sz_channel_add_scaled(self._h, alpha, src, dst),
@@ -218,6 +255,17 @@ class Channel:
         ms, n = C.c_double(0), C.c_int(0)
         _lib().sz_channel_factor_timing(self._h, enable, C.byref(ms), C.byref(n))
         return ms.value, n.value
+
+    def substep_timing(self, enable=-1, max_count=100000):
+        """Device milliseconds of each substep recorded since the last call
+        (numpy array); enable 1/0 switches recording on/off."""
+        buf = np.zeros(max_count)
+        n = C.c_int(0)
+        check(_lib().sz_channel_substep_timing(self._h, enable,
+                                               buf.ctypes.data, max_count,
+                                               C.byref(n)),
+              "sz_channel_substep_timing")
+        return buf[:n.value].copy()
 
     def __del__(self):
         try:

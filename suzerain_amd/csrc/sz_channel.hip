@@ -19,6 +19,7 @@
 #include "sz_common.hpp"
 #include "../../include/suzerain_amd.h"
 #include "../../include/suzerain_amd_solver.hpp"
+#include "sz_precision.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -294,7 +295,14 @@ struct sz_channel {
     sz_bsplineop *bop = nullptr;
     sz_fft_plan *fft = nullptr;
     sz_imexop *imx = nullptr;
-    suzerain_amd::bsmbsm_solver *solver = nullptr;
+    // one solver per factorization slot: a single slot refactors every
+    // substep; zcgbsvx with reuse keeps one slot per SMR91 substep, so each
+    // substep's factors serve the same substep of later steps
+    std::vector<suzerain_amd::bsmbsm_solver *> solvers;
+    // zcgbsvx: the mixed precision solve with matrix-free residuals instead
+    // (its own factor slots)
+    sz_prec *prec = nullptr;
+    int prec_slots = 0;
     zd *buf[2] = {nullptr, nullptr};   // state a = buf[ia], b = buf[1-ia]
     int ia = 0;
     zd *vals = nullptr, *auxw = nullptr, *mixed = nullptr;
@@ -306,6 +314,27 @@ struct sz_channel {
     std::vector<double> lambda1_y, lambda2_y;
     double lxz[4] = {0, 0, 0, 0};
     std::vector<void *> owned;
+    // substep latency: when on, sz_channel_step records a HIP event on the
+    // channel's stream at each substep boundary
+    bool time_factor = false;             // mixed precision solve bracketing
+    std::vector<hipEvent_t> fac_events;
+    void factor_mark()
+    {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        hipEventRecord(e, stream);
+        fac_events.push_back(e);
+    }
+    bool time_substeps = false;
+    std::vector<hipEvent_t> sub_events;   // 4 per step: start, after 0, 1, 2
+    void substep_mark()
+    {
+        if (!time_substeps) return;
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        hipEventRecord(e, stream);
+        sub_events.push_back(e);
+    }
 
     template <typename T>
     T *dalloc(size_t count)
@@ -317,8 +346,11 @@ struct sz_channel {
     }
     ~sz_channel()
     {
+        for (hipEvent_t e : sub_events) hipEventDestroy(e);
+        for (hipEvent_t e : fac_events) hipEventDestroy(e);
         for (void *p : owned) hipFree(p);
-        delete solver;
+        for (auto *sv : solvers) delete sv;
+        sz_prec_free(prec);
         sz_imexop_free(imx);
         sz_fft_plan_free(fft);
         sz_bsplineop_free(bop);
@@ -422,9 +454,28 @@ sz_channel *sz_channel_create(const sz_channel_spec *spec, void *stream,
 
         const sz_bsmbsm g = sz_bsmbsm_construct(5, sp.Ny, sz_bsplineop_max_kl(h->bop),
                                                 sz_bsplineop_max_ku(h->bop));
-        h->solver = suzerain_amd::bsmbsm_solver::build(
-            g, suzerain_amd::specification_zgbsv(h->solver_spec), 1, h->nsys,
-            stream);
+        const suzerain_amd::specification_zgbsv zspec(h->solver_spec);
+        if (zspec.method() == suzerain_amd::specification_zgbsv::zcgbsvx) {
+            sz_prec_params pp{};
+            pp.nsys = h->nsys;
+            pp.n = sp.Ny;
+            pp.KL = g.KL;
+            pp.KU = g.KU;
+            pp.nslots = zspec.reuse() ? 3 : 1;
+            pp.smax = zspec.siter();
+            pp.dmax = zspec.diter();
+            pp.aiter = zspec.aiter();
+            pp.tolsc = zspec.tolsc();
+            pp.reuse = zspec.reuse() ? 1 : 0;
+            extern int sz_internal_imexop_pack_chunks(const sz_imexop *op);
+            pp.nchunks = sz_internal_imexop_pack_chunks(h->imx);
+            h->prec = sz_prec_create(pp, h->stream);
+            if (!h->prec) throw std::runtime_error("mixed precision solver setup");
+            h->prec_slots = pp.nslots;
+        } else {
+            h->solvers.push_back(
+                suzerain_amd::bsmbsm_solver::build(g, zspec, 1, h->nsys, stream));
+        }
 
         // device storage
         h->buf[0] = h->dalloc<zd>((size_t)h->F * 5);
@@ -518,7 +569,7 @@ void *sz_channel_buffer(sz_channel *h, int which)
     case 6: return h->sysoff;
     case 7: return h->dbuf + 16;              // lambda1_y then lambda2_y
     case 8: return h->bop;
-    case 9: return h->solver;
+    case 9: return h->solvers.empty() ? nullptr : h->solvers[0];
     default: return nullptr;
     }
 }
@@ -528,6 +579,7 @@ int sz_channel_nonlinear(sz_channel *h, int which, int zeroth,
                          double *candidates)
 {
     if (!h) return -1;
+    sz::scope_range range("apply_navier_stokes_spatial_operator");
     if (which != 0 && which != 1) return -2;
     const sz_channel_spec &sp = h->spec;
     zd *s = h->state(which);
@@ -542,6 +594,7 @@ int sz_channel_nonlinear(sz_channel *h, int which, int zeroth,
     const int pb = n >= NT ? 1 : NT / n;
     const bool fused = !(dsel && dsel[0] == 's') && (size_t)pb * n * sizeof(zd) <= 65536;
     if (fused) {
+        sz::scope_range r2("operator_tools::bop_apply+diffwave_accumulate");
         deriv_args a;
         a.s = s;
         a.vals = h->vals;
@@ -618,25 +671,35 @@ int sz_channel_nonlinear(sz_channel *h, int which, int zeroth,
     if (rc) return rc;
     // to physical space: 5 state values then 41 derivatives
     const long long smix = sz_fft_mixed_elements(h->fft);
-    rc = sz_fft_wave_to_physical(h->fft, 5, h->vals, F, h->mixed, smix, h->phys,
-                                 h->Fp, st);
-    if (!rc)
-        rc = sz_fft_wave_to_physical(h->fft, 41, h->auxw, F, h->mixed + 5 * smix,
-                                     smix, h->phys + 5 * h->Fp, h->Fp, st);
+    {
+        sz::scope_range r2("transform_wave_to_physical");
+        rc = sz_fft_wave_to_physical(h->fft, 5, h->vals, F, h->mixed, smix,
+                                     h->phys, h->Fp, st);
+        if (!rc)
+            rc = sz_fft_wave_to_physical(h->fft, 41, h->auxw, F,
+                                         h->mixed + 5 * smix, smix,
+                                         h->phys + 5 * h->Fp, h->Fp, st);
+    }
     if (rc) return rc;
     if (zeroth) {
+        sz::scope_range r2("collect_references");
         rc = sz_ns_references(&sp.scenario, sp.Ny, sp.dNz, sp.dNx, h->phys, h->Fp,
                               h->refs, h->work, st);
         if (rc) return rc;
     }
     double *cand = h->dbuf;
+    sz::scope_range r3("nonlinear right hand sides");
     rc = sz_ns_rhs(&sp.scenario, SZ_LINEARIZE_RHOME_XYZ, zeroth, sp.Ny, sp.dNz,
                    sp.dNx, h->phys, h->Fp, h->phys + 5 * h->Fp, h->Fp, h->phys,
                    h->Fp, h->refs, h->dbuf + 16, h->dbuf + 16 + sp.Ny,
                    h->dbuf + 12, cand, h->work, st);
     if (rc) return rc;
     // back to wave space; rhome_xyz has no explicit density equation
-    rc = sz_fft_physical_to_wave(h->fft, 4, h->phys, h->Fp, h->mixed, smix, s, F, st);
+    {
+        sz::scope_range r2("transform_physical_to_wave");
+        rc = sz_fft_physical_to_wave(h->fft, 4, h->phys, h->Fp, h->mixed, smix,
+                                     s, F, st);
+    }
     if (rc) return rc;
     hipLaunchKernelGGL(k_fill0, dim3(nblocks(F)), dim3(NT), 0, st,
                        s + 4 * (size_t)F, F);
@@ -656,6 +719,8 @@ int sz_channel_accumulate_operator(sz_channel *h, double phi, int in,
                                    double beta, int out)
 {
     if (!h) return -1;
+    sz::scope_range range(beta == 0.0 ? "apply_mass_plus_scaled_operator"
+                                      : "accumulate_mass_plus_scaled_operator");
     if (in != 0 && in != 1) return -3;
     if (out != 0 && out != 1) return -5;
     const zd *x = h->state(in);
@@ -679,28 +744,62 @@ int sz_channel_apply_operator(sz_channel *h, double phi, int which)
 
 // Solve (M + phi L) x = state with isothermal walls for every retained
 // wavenumber, in place; other wavenumbers are zeroed
-// (operator_hybrid_isothermal.cpp:528-689).  Asynchronous: per-system info
-// stays in the solver's info array (checked by sz_channel_check).
-int sz_channel_invert_operator(sz_channel *h, double phi, int which)
+// (operator_hybrid_isothermal.cpp:528-689), with the factorization slot
+// `slot`.  Asynchronous: per-system info stays in the solver's info array
+// (checked by sz_channel_check).
+int sz_channel_invert_operator_slot(sz_channel *h, double phi, int which, int slot)
 {
     if (!h) return -1;
     if (which != 0 && which != 1) return -3;
-    suzerain_amd::bsmbsm_solver &sv = *h->solver;
+    if (slot < 0 || slot >= sz_channel_solver_slots(h)) return -4;
+    sz::scope_range range("invert_mass_plus_scaled_operator");
+    if (h->prec) {
+        sz::scope_range r2("bsmbsm_solver_zcgbsvx::suzerain_lapackext_zcgbsvx");
+        sz_prec_problem pr{};
+        pr.op = h->imx;
+        pr.ref = &h->iref;
+        pr.ld = &h->ild;
+        pr.bc = &h->spec.bc;
+        pr.km = h->km;
+        pr.kn = h->kn;
+        pr.phi = phi;
+        pr.state = h->state(which);
+        pr.F = h->F;
+        pr.sysoff = h->sysoff;
+        if (h->time_factor) h->factor_mark();
+        const int rc = sz_prec_solve(h->prec, slot, pr);
+        if (h->time_factor) h->factor_mark();
+        if (rc) return rc;
+        return h->zero_dealias(h->state(which), 5);
+    }
+    suzerain_amd::bsmbsm_solver &sv = *h->solvers[(size_t)slot];
     zd *s = h->state(which);
     const double p[2] = {phi, 0};
-    int rc = sz_rholut_imexop_packc_batched(
-        h->imx, p, h->km, h->kn, &h->iref, &h->ild, 0, 1, 2, 3, 4, nullptr,
-        sv.PAPT, sv.ld_PAPT, sv.stride_PAPT, h->nsys, &h->spec.bc, h->stream);
+    int rc;
+    {
+        // the isothermal wall rows are written by the assembly kernel itself
+        sz::scope_range r2("implicit operator assembly (packc)");
+        rc = sz_rholut_imexop_packc_batched(
+            h->imx, p, h->km, h->kn, &h->iref, &h->ild, 0, 1, 2, 3, 4, nullptr,
+            sv.PAPT, sv.ld_PAPT, sv.stride_PAPT, h->nsys, &h->spec.bc, h->stream);
+    }
     if (rc) return rc;
     sv.supplied_PAPT();
     const int N = sv.N;
-    hipLaunchKernelGGL(k_gather, dim3(nblocks((long long)N * h->nsys)), dim3(NT), 0,
-                       h->stream, s, h->F, h->sysoff, 5, h->Ny,
-                       reinterpret_cast<zd *>(sv.PB), sv.stride_PB, h->nsys);
+    {
+        sz::scope_range r2("suzerain_bsmbsm_zaPxpby");
+        hipLaunchKernelGGL(k_gather, dim3(nblocks((long long)N * h->nsys)), dim3(NT),
+                           0, h->stream, s, h->F, h->sysoff, 5, h->Ny,
+                           reinterpret_cast<zd *>(sv.PB), sv.stride_PB, h->nsys);
+    }
     if (!ok()) return SZ_ERR_LAUNCH;
-    rc = sz_isothermal_rhs_batched(5, h->Ny, 0, 1, 2, 3, h->spec.bc.enforce_lower,
-                                   h->spec.bc.enforce_upper, sv.PB, sv.stride_PB,
-                                   h->nsys, h->stream);
+    {
+        sz::scope_range r2("implicit right hand side BCs");
+        rc = sz_isothermal_rhs_batched(5, h->Ny, 0, 1, 2, 3,
+                                       h->spec.bc.enforce_lower,
+                                       h->spec.bc.enforce_upper, sv.PB,
+                                       sv.stride_PB, h->nsys, h->stream);
+    }
     if (rc) return rc;
     rc = sv.solve_async('T', 1);
     if (rc) return rc;
@@ -712,6 +811,38 @@ int sz_channel_invert_operator(sz_channel *h, double phi, int which)
     return ok() ? 0 : SZ_ERR_LAUNCH;
 }
 
+int sz_channel_invert_operator(sz_channel *h, double phi, int which)
+{
+    return sz_channel_invert_operator_slot(h, phi, which, 0);
+}
+
+int sz_channel_solver_slots(const sz_channel *h)
+{
+    if (!h) return -1;
+    return h->prec ? h->prec_slots : (int)h->solvers.size();
+}
+
+void *sz_channel_solver(sz_channel *h, int slot)
+{
+    if (!h || slot < 0 || slot >= (int)h->solvers.size()) return nullptr;
+    return h->solvers[(size_t)slot];
+}
+
+void *sz_channel_solver_stats(sz_channel *h, int which)
+{
+    if (!h) return nullptr;
+    if (h->prec) return sz_prec_stats(h->prec, which);
+    return nullptr;
+}
+
+int sz_channel_solver_counts(sz_channel *h, int *factored, int *restarted,
+                             int *fallback)
+{
+    if (!h || !h->prec) return -1;
+    sz_prec_counts(h->prec, factored, restarted, fallback);
+    return 0;
+}
+
 // Wait for the stream and report the first system whose factorization met
 // an exact zero pivot in any solve since the last check (its info), or 0;
 // the solver's record is then cleared.  Every factorization launch rewrites
@@ -720,7 +851,13 @@ int sz_channel_invert_operator(sz_channel *h, double phi, int which)
 int sz_channel_check(sz_channel *h)
 {
     if (!h) return -1;
-    return h->solver->first_info(true);
+    if (h->prec) return sz_prec_first_info(h->prec, true);
+    int first = 0;
+    for (auto *sv : h->solvers) {
+        const int inf = sv->first_info(true);
+        if (!first) first = inf;
+    }
+    return first;
 }
 
 int sz_channel_add_scaled(sz_channel *h, double alpha, int src, int dst)
@@ -741,15 +878,59 @@ int sz_channel_copy(sz_channel *h, int src, int dst)
     return ok() ? 0 : SZ_ERR_LAUNCH;
 }
 
+int sz_channel_substep_timing(sz_channel *h, int enable, double *ms, int max,
+                              int *count)
+{
+    if (!h) return -1;
+    int n = 0;
+    const size_t ne = h->sub_events.size();
+    for (size_t i = 0; i + 4 <= ne; i += 4) {
+        for (int k = 0; k < 3; ++k) {
+            float t = 0;
+            if (hipEventSynchronize(h->sub_events[i + k + 1]) != hipSuccess ||
+                hipEventElapsedTime(&t, h->sub_events[i + k],
+                                    h->sub_events[i + k + 1]) != hipSuccess)
+                return SZ_ERR_LAUNCH;
+            if (ms && n < max) ms[n] = t;
+            ++n;
+        }
+    }
+    for (hipEvent_t e : h->sub_events) hipEventDestroy(e);
+    h->sub_events.clear();
+    if (count) *count = n < max ? n : max;
+    if (enable >= 0) h->time_substeps = enable != 0;
+    return 0;
+}
+
 int sz_channel_factor_timing(sz_channel *h, int enable, double *total_ms,
                              int *count)
 {
     if (!h) return -1;
-    int n = 0;
-    const double ms = h->solver->factor_time_ms(&n);
+    int total = 0;
+    double ms = 0;
+    for (auto *sv : h->solvers) {
+        int n = 0;
+        ms += sv->factor_time_ms(&n);
+        total += n;
+        if (enable >= 0) sv->time_factor = enable != 0;
+    }
+    if (h->prec) {
+        // the whole mixed precision solve is bracketed
+        for (size_t i = 0; i + 1 < h->fac_events.size(); i += 2) {
+            float t = 0;
+            if (hipEventSynchronize(h->fac_events[i + 1]) == hipSuccess &&
+                hipEventElapsedTime(&t, h->fac_events[i], h->fac_events[i + 1]) ==
+                    hipSuccess) {
+                ms += t;
+                ++total;
+            }
+        }
+        for (hipEvent_t e : h->fac_events) hipEventDestroy(e);
+        h->fac_events.clear();
+        if (enable >= 0) h->time_factor = enable != 0;
+    }
     if (total_ms) *total_ms = ms;
-    if (count) *count = n;
-    if (enable >= 0) h->solver->time_factor = enable != 0;
+    if (count) *count = total;
     return 0;
 }
 
@@ -764,8 +945,11 @@ void sz_channel_exchange(sz_channel *h)
 int sz_channel_step(sz_channel *h, double max_dt, double *dt_out)
 {
     if (!h) return -1;
+    sz::scope_range range("lowstorage::step");
     const double chi = 1.0 / ((double)h->spec.dNx * h->spec.dNz);
     double cand[12];
+    const size_t mark0 = h->sub_events.size();
+    h->substep_mark();
     int rc = sz_channel_copy(h, 0, 1);
     if (!rc) rc = sz_channel_nonlinear(h, 1, 1, cand);
     if (rc) return rc;
@@ -779,7 +963,11 @@ int sz_channel_step(sz_channel *h, double max_dt, double *dt_out)
     if (!(dt > 0)) return -2;
     rc = sz_channel_apply_operator(h, dt * smr91_alpha[0], 0);
     if (!rc) rc = sz_channel_add_scaled(h, chi * dt * smr91_gamma[0], 1, 0);
-    if (!rc) rc = sz_channel_invert_operator(h, -dt * smr91_beta[0], 0);
+    // substep i uses factorization slot i when the solver keeps one per
+    // substep (factor reuse across steps), else the single slot
+    const int per_substep = sz_channel_solver_slots(h) == 3 ? 1 : 0;
+    if (!rc) rc = sz_channel_invert_operator_slot(h, -dt * smr91_beta[0], 0, 0);
+    if (!rc) h->substep_mark();
     for (int i = 1; i < 3 && !rc; ++i) {
         rc = sz_channel_accumulate_operator(h, dt * smr91_alpha[i], 0,
                                             chi * dt * smr91_zeta[i], 1);
@@ -787,9 +975,17 @@ int sz_channel_step(sz_channel *h, double max_dt, double *dt_out)
         sz_channel_exchange(h);
         rc = sz_channel_nonlinear(h, 1, 0, nullptr);
         if (!rc) rc = sz_channel_add_scaled(h, chi * dt * smr91_gamma[i], 1, 0);
-        if (!rc) rc = sz_channel_invert_operator(h, -dt * smr91_beta[i], 0);
+        if (!rc) rc = sz_channel_invert_operator_slot(h, -dt * smr91_beta[i], 0,
+                                                      i * per_substep);
+        if (!rc) h->substep_mark();
     }
-    if (rc) return rc;
+    if (rc) {
+        // a step that failed part way leaves no partial substep record
+        for (size_t i = mark0; i < h->sub_events.size(); ++i)
+            hipEventDestroy(h->sub_events[i]);
+        h->sub_events.resize(mark0);
+        return rc;
+    }
     return sz_channel_check(h);
 }
 

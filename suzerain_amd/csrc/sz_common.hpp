@@ -1,9 +1,20 @@
 // Shared device helpers for the suzerain_amd HIP kernels (gfx950, wave64).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 #include <stdint.h>
 
 namespace sz {
+
+// A roctx range over a scope, named after the reference's timer scope for the
+// same stage (SUZERAIN_TIMER_SCOPED names), so rocprofv3 --marker-trace lines
+// up with the reference's GRVY timings.  Ranges mark host-side enqueueing.
+struct scope_range {
+    explicit scope_range(const char *name) { roctxRangePushA(name); }
+    ~scope_range() { roctxRangePop(); }
+    scope_range(const scope_range &) = delete;
+    scope_range &operator=(const scope_range &) = delete;
+};
 
 // Interleaved complex scalar, identical in memory to C99 `double complex`,
 // std::complex<double>, and (for R = float) `float complex`.

@@ -269,16 +269,35 @@ __device__ inline zd element(const pack_tab &pt, const bops &w, const zd *Wr,
     return zd{ex, ey};
 }
 
+// Extra outputs of the single precision assembly (factor storage of the
+// mixed precision solve): systems with skip[s] != 0 are left alone; for the
+// others the wall-row scalings as stored (rounded to float), indexed
+// s*8 + wall*4 + equation, and per-workgroup sums of |a|^2 over the
+// assembled elements (frob[s*nchunks + chunk]).
+struct pack_extra {
+    const int *skip;
+    zd *bcs;
+    double *frob;
+};
+
+template <typename OUT> __device__ inline OUT to_out(zd v);
+template <> __device__ inline zd to_out<zd>(zd v) { return v; }
+template <> __device__ inline sz::zf to_out<sz::zf>(zd v)
+{
+    return sz::zf{(float)v.x, (float)v.y};
+}
+
 // One workgroup assembles the 5*PK_RB columns of P (M + phi L)^T P^T whose
 // equation rows are r0 .. r0+PK_RB-1 (column C = r*S + p holds equation
-// field_of[p] at wall-normal row r) for one system.
-template <int SS>
+// field_of[p] at wall-normal row r) for one system.  OUT: zd (double) or
+// zf (complex float factor storage, with pack_extra outputs).
+template <int SS, typename OUT = zd>
 __global__ __launch_bounds__(PK_NT)
 void k_imexop_pack(const table *__restrict__ tab, bops w, refs rf, zd phi,
                    const double *__restrict__ km, const double *__restrict__ kn,
                    int S_, int4 fof_lo, int fof_hi, int rho_pos, int KL, int KU,
-                   zd *__restrict__ papt, int ldpapt, long long stride,
-                   int nchunks, wallbc bc)
+                   OUT *__restrict__ papt, int ldpapt, long long stride,
+                   int nchunks, wallbc bc, pack_extra ex)
 {
     extern __shared__ double smem[];
     zd *alpha = reinterpret_cast<zd *>(smem);                   // MAXTERMS
@@ -293,6 +312,7 @@ void k_imexop_pack(const table *__restrict__ tab, bops w, refs rf, zd phi,
     const int S = SS ? SS : S_;                 // fields per row (5)
     const int s = blockIdx.x / nchunks;
     const int chunk = blockIdx.x - s * nchunks;
+    if (ex.skip && ex.skip[s]) return;          // uniform over the block
     const int n = w.n, N = S * n;
     const int r0 = chunk * PK_RB;
     const int nr = min(PK_RB, n - r0);
@@ -343,8 +363,9 @@ void k_imexop_pack(const table *__restrict__ tab, bops w, refs rf, zd phi,
     __syncthreads();
     const pack_tab pt{sbg0, sbng, sgop, skl, sku};
 
-    zd *P = papt + (size_t)s * stride;
+    OUT *P = papt + (size_t)s * stride;
     const int LD = KL + 1 + KU;
+    double fsum = 0;
     const int total = S * nr * LD;
     // element e = cl*LD + rr, stepped by PK_NT without dividing in the loop
     int cl = tid / LD, rr = tid - cl * LD;
@@ -369,14 +390,30 @@ void k_imexop_pack(const table *__restrict__ tab, bops w, refs rf, zd phi,
                 for (int q = 0; q < 4; ++q)
                     if (q == fi) f = lo ? bc.fac[0][q] : bc.fac[1][q];
                 v = R == C ? sc : (R == rhoR ? zd{-sc.x * f, -sc.y * f} : zd{0, 0});
+                if (ex.bcs && R == C) {
+                    const OUT o = to_out<OUT>(sc);
+                    ex.bcs[(size_t)s * 8 + (lo ? 0 : 4) + fi] = zd{(double)o.x, (double)o.y};
+                }
             }
-            P[(size_t)C * ldpapt + KU + R - C] = v;
+            P[(size_t)C * ldpapt + KU + R - C] = to_out<OUT>(v);
+            if (ex.frob) fsum = fma(v.x, v.x, fma(v.y, v.y, fsum));
         }
         rr += drr;
         cl += dcl;
         if (rr >= LD) {
             rr -= LD;
             ++cl;
+        }
+    }
+    if (ex.frob) {
+        __shared__ double red[PK_NT / 64];
+        for (int off = 32; off >= 1; off >>= 1) fsum += __shfl_xor(fsum, off);
+        if ((tid & 63) == 0) red[tid >> 6] = fsum;
+        __syncthreads();
+        if (tid == 0) {
+            double t = 0;
+            for (int i = 0; i < PK_NT / 64; ++i) t += red[i];
+            ex.frob[(size_t)s * nchunks + chunk] = t;
         }
     }
 }
@@ -396,9 +433,10 @@ void k_imexop_accumulate(const table *__restrict__ tab, bops w, refs rf, zd phi,
                          long long instride, zd beta, int beta_zero, zd *out0,
                          zd *out1, zd *out2, zd *out3, zd *out4,
                          const long long *__restrict__ outoff,
-                         long long outstride)
+                         long long outstride, const int *__restrict__ skip)
 {
     extern __shared__ double smem[];
+    if (skip && skip[blockIdx.x]) return;       // uniform over the block
     const int n = w.n;
     zd *alpha = reinterpret_cast<zd *>(smem);       // MAXTERMS
     zd *xs = alpha + MAXTERMS;                      // 5 x n
@@ -636,14 +674,14 @@ int sz_rholut_imexop_packc_batched(
                            zd{phi[0], phi[1]}, km, kn, S,
                            make_int4(field_of[0], field_of[1], field_of[2], field_of[3]),
                            field_of[4], rho < 0 ? 0 : rho, g.KL, g.KU, (zd *)patpt,
-                           ldpatpt, stride, nchunks, wb);
+                           ldpatpt, stride, nchunks, wb, pack_extra{});
     else
         hipLaunchKernelGGL(k_imexop_pack<0>, dim3(nsys * nchunks), dim3(PK_NT), lds,
                            (hipStream_t)stream, op->dev, w, make_refs(r, ld),
                            zd{phi[0], phi[1]}, km, kn, S,
                            make_int4(field_of[0], field_of[1], field_of[2], field_of[3]),
                            field_of[4], rho < 0 ? 0 : rho, g.KL, g.KU, (zd *)patpt,
-                           ldpatpt, stride, nchunks, wb);
+                           ldpatpt, stride, nchunks, wb, pack_extra{});
     return launch_ok() ? 0 : SZ_ERR_LAUNCH;
 }
 
@@ -668,12 +706,33 @@ int sz_rholut_imexop_packf_batched(
                                           stride, nsys, bc, stream);
 }
 
+int sz_internal_imexop_accumulate_masked(
+    const sz_imexop *op, const double *phi, const double *km, const double *kn,
+    const sz_imexop_ref *r, const sz_imexop_refld *ld,
+    const void *const *in, const long long *inoff, long long instride,
+    const double *beta, void *const *out, const long long *outoff,
+    long long outstride, int nsys, const int *skip, void *stream);
+
 int sz_rholut_imexop_accumulate_batched(
     const sz_imexop *op, const double *phi, const double *km, const double *kn,
     const sz_imexop_ref *r, const sz_imexop_refld *ld,
     const void *const *in, const long long *inoff, long long instride,
     const double *beta, void *const *out, const long long *outoff,
     long long outstride, int nsys, void *stream)
+{
+    return sz_internal_imexop_accumulate_masked(op, phi, km, kn, r, ld, in, inoff,
+                                                instride, beta, out, outoff,
+                                                outstride, nsys, nullptr, stream);
+}
+
+// As sz_rholut_imexop_accumulate_batched; systems with skip[s] != 0 are
+// left alone (skip may be NULL).
+int sz_internal_imexop_accumulate_masked(
+    const sz_imexop *op, const double *phi, const double *km, const double *kn,
+    const sz_imexop_ref *r, const sz_imexop_refld *ld,
+    const void *const *in, const long long *inoff, long long instride,
+    const double *beta, void *const *out, const long long *outoff,
+    long long outstride, int nsys, const int *skip, void *stream)
 {
     if (!op) return -1;
     if (!phi) return -2;
@@ -698,8 +757,79 @@ int sz_rholut_imexop_accumulate_batched(
                        (const zd *)in[1], (const zd *)in[2], (const zd *)in[3],
                        (const zd *)in[4], inoff, instride, zd{beta[0], beta[1]},
                        bz, (zd *)out[0], (zd *)out[1], (zd *)out[2],
-                       (zd *)out[3], (zd *)out[4], outoff, outstride);
+                       (zd *)out[3], (zd *)out[4], outoff, outstride, skip);
     return launch_ok() ? 0 : SZ_ERR_LAUNCH;
+}
+
+// Single precision packf of the five-field operator (positions 0..4 in
+// (rho E, rho u, rho v, rho w, rho) order) with isothermal walls into
+// complex float LU storage: element (R, C) of system s at
+// lu32 + s*stride + C*ldlu + KL + KU + R - C.  Systems with skip[s] != 0 are
+// left alone; bcs receives the 8 wall-row scalings per system as stored and
+// frob the per-workgroup sums of |a|^2 (nchunks per system, returned by
+// sz_internal_imexop_pack_chunks).
+int sz_internal_imexop_pack_chunks(const sz_imexop *op)
+{
+    return op ? (sz_bsplineop_n(op->w) + PK_RB - 1) / PK_RB : -1;
+}
+
+int sz_internal_imexop_pack32(const sz_imexop *op, const double *phi,
+                              const double *km, const double *kn,
+                              const sz_imexop_ref *r, const sz_imexop_refld *ld,
+                              void *lu32, int ldlu, long long stride, int nsys,
+                              const sz_isothermal_bc *bc, const int *skip,
+                              void *bcs, double *frob, void *stream)
+{
+    if (!op || !phi || !km || !kn || !r || !ld || !bc) return -1;
+    const bops w = make_bops(op->w);
+    const sz_bsmbsm g = sz_bsmbsm_construct(5, w.n, w.max_kl, w.max_ku);
+    if (ldlu < g.LD + g.KL) return -9;
+    if (nsys < 0) return -11;
+    if (nsys == 0) return 0;
+    wallbc wb{};
+    const double Ma = op->scen[2], gamma = op->scen[4];
+    const double *spec[2] = {&bc->lower_T, &bc->upper_T};
+    wb.enforce[0] = bc->enforce_lower;
+    wb.enforce[1] = bc->enforce_upper;
+    for (int k = 0; k < 2; ++k) {
+        const double *p = spec[k];
+        wb.fac[k][0] = p[0] / (gamma * (gamma - 1))
+                     + Ma * Ma / 2 * (p[1] * p[1] + p[2] * p[2] + p[3] * p[3]);
+        wb.fac[k][1] = p[1];
+        wb.fac[k][2] = p[2];
+        wb.fac[k][3] = p[3];
+    }
+    const int nchunks = (w.n + PK_RB - 1) / PK_RB;
+    const size_t lds = sizeof(zd) * (MAXTERMS + PK_RB * MAXGROUPS)
+                     + sizeof(double) * 3 * PK_RB * w.ld;
+    if (lds > 64 * 1024) return SZ_ERR_LDS;
+    pack_extra ex{skip, (zd *)bcs, frob};
+    hipLaunchKernelGGL((k_imexop_pack<5, sz::zf>), dim3(nsys * nchunks), dim3(PK_NT),
+                       lds, (hipStream_t)stream, op->dev, w, make_refs(r, ld),
+                       zd{phi[0], phi[1]}, km, kn, 5, make_int4(0, 1, 2, 3), 4, 4,
+                       g.KL, g.KU, (sz::zf *)lu32 + g.KL, ldlu, stride, nchunks,
+                       wb, ex);
+    return launch_ok() ? 0 : SZ_ERR_LAUNCH;
+}
+
+// The isothermal wall constants of IsothermalPATPTEnforcer
+// (apps/perfect/operator_hybrid_isothermal.cpp:443-459): fac[wall*4 + 0] the
+// energy factor, fac[wall*4 + 1..3] the wall velocities.
+int sz_internal_imexop_wall_factors(const sz_imexop *op,
+                                    const sz_isothermal_bc *bc, double *fac)
+{
+    if (!op || !bc || !fac) return -1;
+    const double Ma = op->scen[2], gamma = op->scen[4];
+    const double *spec[2] = {&bc->lower_T, &bc->upper_T};
+    for (int k = 0; k < 2; ++k) {
+        const double *p = spec[k];
+        fac[k * 4 + 0] = p[0] / (gamma * (gamma - 1))
+                       + Ma * Ma / 2 * (p[1] * p[1] + p[2] * p[2] + p[3] * p[3]);
+        fac[k * 4 + 1] = p[1];
+        fac[k * 4 + 2] = p[2];
+        fac[k * 4 + 3] = p[3];
+    }
+    return 0;
 }
 
 int sz_isothermal_rhs_batched(int S, int n, int rho_E, int rho_u, int rho_v,

@@ -7,6 +7,7 @@
 // methods, same reuse/apprx bookkeeping, same error reporting text; the
 // storage is device memory holding every system of the batch.
 #include "../../include/suzerain_amd_solver.hpp"
+#include "sz_common.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -476,14 +477,18 @@ int bsmbsm_solver_zgbsv::solve_hook(char trans, int nrhs_)
     if (std::toupper((unsigned char)fact_) == 'N' && nrhs_ == 1 &&
         std::toupper((unsigned char)trans) == 'T') {
         if (time_factor) factor_mark();
-        rc = sz_internal_zgbtrf_ut(N, KL, KU, LU, ld_LU, stride_LU, ipiv, N, info,
-                                   PB, stride_PB, nbatch, stream);
+        {
+            sz::scope_range r("bsmbsm_solver_zgbsv::suzerain_lapack_zgbtrf");
+            rc = sz_internal_zgbtrf_ut(N, KL, KU, LU, ld_LU, stride_LU, ipiv, N,
+                                       info, PB, stride_PB, nbatch, stream);
+        }
         if (time_factor) factor_mark();
         if (rc == 0) {
             fact_ = 'F';
             // a singular system's PB already holds part of U^T y (the
             // reference's contract lets solve destroy PB); its L^T sweep is
             // skipped so no inf/NaN arithmetic follows
+            sz::scope_range r("bsmbsm_solver_zgbsv::suzerain_lapack_zgbtrs");
             return sz_internal_zgbtrs_lt(N, KL, KU, LU, ld_LU, stride_LU, ipiv, N,
                                          PB, stride_PB, info, nbatch, stream);
         }
@@ -492,13 +497,17 @@ int bsmbsm_solver_zgbsv::solve_hook(char trans, int nrhs_)
     }
     if (std::toupper((unsigned char)fact_) == 'N') {
         if (time_factor) factor_mark();
-        rc = sz_zgbtrf_batched(N, KL, KU, LU, ld_LU, stride_LU, ipiv, N, info,
-                               nbatch, stream);
+        {
+            sz::scope_range r("bsmbsm_solver_zgbsv::suzerain_lapack_zgbtrf");
+            rc = sz_zgbtrf_batched(N, KL, KU, LU, ld_LU, stride_LU, ipiv, N, info,
+                                   nbatch, stream);
+        }
         if (time_factor) factor_mark();
         fact_ = 'F';
         if (rc) return rc;
     }
     // systems whose factorization met a zero pivot keep PB, as ZGBSV does
+    sz::scope_range r("bsmbsm_solver_zgbsv::suzerain_lapack_zgbtrs");
     return sz_internal_gbtrs_masked(0, trans, N, KL, KU, nrhs_, LU, ld_LU,
                                     stride_LU, ipiv, N, PB, N, stride_PB, info,
                                     nbatch, stream);
@@ -581,6 +590,7 @@ bsmbsm_solver &bsmbsm_solver_zcgbsvx::supplied_PAPT()
 
 int bsmbsm_solver_zcgbsvx::solve_hook(char trans, int nrhs_)
 {
+    sz::scope_range r("bsmbsm_solver_zcgbsvx::suzerain_lapackext_zcgbsvx");
     int rc = 0;
     for (int j = 0; !rc && j < nrhs_; ++j) {
         rc = sz_zcgbsvx_batched(fact_dev, apprx_dev, spec.aiter(), trans, N, KL,

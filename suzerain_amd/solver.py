@@ -57,6 +57,18 @@ def host_to_device(ptr, arr):
         raise RuntimeError(f"hipMemcpy failed: {rc}")
 
 
+def stats_of(handle, which, nbatch):
+    """Per-system zcgbsvx outcome arrays of a solver handle (None when the
+    solver is not zcgbsvx)."""
+    p = load().sz_solver_zcgbsvx_stats(C.c_void_p(handle), _STATS[which])
+    if not p:
+        return None
+    dtype = {"siter": np.int32, "diter": np.int32, "tolsc": np.float64,
+             "res": np.float64, "fact": np.int8, "apprx": np.int32}[which]
+    a = device_to_host(p, nbatch * np.dtype(dtype).itemsize, dtype)
+    return a.view("S1") if which == "fact" else a
+
+
 class BsmbsmSolver:
     def __init__(self, spec, S, n, kl, ku, nrhs=1, nbatch=1, stream=None):
         lib = load()
@@ -101,13 +113,7 @@ class BsmbsmSolver:
         host_to_device(p, arr)
 
     def stats(self, which):
-        p = load().sz_solver_zcgbsvx_stats(self._h, _STATS[which])
-        if not p:
-            return None
-        dtype = {"siter": np.int32, "diter": np.int32, "tolsc": np.float64,
-                 "res": np.float64, "fact": np.int8, "apprx": np.int32}[which]
-        a = device_to_host(p, self.nbatch * np.dtype(dtype).itemsize, dtype)
-        return a.view("S1") if which == "fact" else a
+        return stats_of(self._h, which, self.nbatch)
 
     def supplied_PAPT(self):
         load().sz_solver_supplied_PAPT(self._h)

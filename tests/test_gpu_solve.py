@@ -252,7 +252,8 @@ def test_fused_factor_and_transposed_solve_match_oracle(gpu, N, kl, ku, dom):
     f_ut.argtypes = [i32, i32, i32, vp, i32, i64, vp, i64, vp, vp, i64, i32, vp]
     f_lt = lib.sz_internal_zgbtrs_lt
     f_lt.restype = i32
-    f_lt.argtypes = [i32, i32, i32, vp, i32, i64, vp, i64, vp, i64, i32, vp]
+    f_lt.argtypes = [i32, i32, i32, vp, i32, i64, vp, i64, vp, i64, vp, i32,
+                     vp]
     rng = np.random.default_rng(N + 3 * kl + ku + int(dom))
     nb = 6
     ab = _random_bands(rng, nb, N, kl, ku, dom)
@@ -265,7 +266,7 @@ def test_fused_factor_and_transposed_solve_match_oracle(gpu, N, kl, ku, dom):
     assert f_ut(N, kl, ku, abd.data_ptr(), ldab, ldab * N, ipiv.data_ptr(), N,
                 info.data_ptr(), bd.data_ptr(), N, nb, stream) == 0
     assert f_lt(N, kl, ku, abd.data_ptr(), ldab, ldab * N, ipiv.data_ptr(), N,
-                bd.data_ptr(), N, nb, stream) == 0
+                bd.data_ptr(), N, info.data_ptr(), nb, stream) == 0
     torch.cuda.synchronize()
     lu, piv, x = abd.cpu().numpy(), ipiv.cpu().numpy(), bd.cpu().numpy()
     assert (info.cpu().numpy() == 0).all()
