@@ -369,7 +369,8 @@ void *sz_channel_state(sz_channel *h, int which);
 /* Internal buffers for inspection: 0 refs (42 x Ny), 1 physical fields
  * (46), 2 wave derivative fields (41), 3 wave values (5), 4 km, 5 kn,
  * 6 per-system pencil offsets, 7 lambda1_y then lambda2_y, 8 the
- * sz_bsplineop, 9 the suzerain_amd::bsmbsm_solver of slot 0. */
+ * sz_bsplineop, 9 the suzerain_amd::bsmbsm_solver of slot 0, 10 the 12
+ * step candidates of the last zeroth substep. */
 void *sz_channel_buffer(sz_channel *h, int which);
 /* operator_nonlinear::apply_operator (navier_stokes.hpp:107-986): state
  * `which` <- N(state) in place; on zeroth also the references and the 12
@@ -427,6 +428,58 @@ int sz_channel_substep_timing(sz_channel *h, int enable, double *ms, int max,
 /* lowstorage::step (lowstorage.hpp:1461-1520) with SMR91 and the
  * delta_t_reducer; dt_out receives the step taken. */
 int sz_channel_step(sz_channel *h, double max_dt, double *dt_out);
+
+/* ------------------------------------------------------------ ensembles
+ * No single counterpart in the reference, which advances one case per process
+ * (apps/perfect/main_advance.cpp, lowstorage::step per member): an ensemble
+ * of channels (an uncertainty sweep over Re, Ma and resolution) advanced
+ * together on one GPU, each member with its own fixed time step.  The
+ * members' explicit stages run on a few streams sharing scratch per stream;
+ * the implicit solves of all members run in one grouped mixed precision
+ * solver (float factors kept per SMR91 substep and reused across steps, double
+ * residuals of the matrix-free operator, the zcgbsvx specification's
+ * restart and fallback rules).  With graph set, the first step runs eagerly
+ * and the next is captured in a hipGraph and replayed thereafter (the
+ * restart pass then is always enqueued and the double fallback is off). */
+typedef struct sz_ensemble sz_ensemble;
+typedef struct sz_ensemble_opts {
+    const char *solver;   /* zcgbsvx specification; NULL: "zcgbsvx,reuse=true" */
+    int nstreams;         /* explicit-stage streams (1..max_streams)        */
+    int max_streams;      /* streams (and scratch sets) created, <= 8       */
+    int graph;            /* capture steps in a hipGraph                    */
+    int lu_bs;            /* factorization unroll, 4 or 8 (0: 8)            */
+} sz_ensemble_opts;
+sz_ensemble *sz_ensemble_create(int nmembers, const sz_channel_spec *specs,
+                                const double *dt, const sz_ensemble_opts *opts,
+                                void *stream, char *err, int errlen);
+void sz_ensemble_free(sz_ensemble *e);
+/* Members join and leave between steps (continuous batching): add returns
+ * the new member's id (ids are never reused; the initial members are 0..) or
+ * a negative error with a message (SZ_ERR_ALLOC: out of device memory);
+ * remove frees the member's storage (waits for the device). */
+int sz_ensemble_add(sz_ensemble *e, const sz_channel_spec *spec, double dt,
+                    char *err, int errlen);
+int sz_ensemble_remove(sz_ensemble *e, int id);
+/* Live members, and the position of member id among them (-1 if gone). */
+int sz_ensemble_size(const sz_ensemble *e);
+int sz_ensemble_find(const sz_ensemble *e, int id);
+/* The channel of the live member at position m (its states:
+ * sz_channel_state; no solver of its own). */
+sz_channel *sz_ensemble_member(sz_ensemble *e, int m);
+long long sz_ensemble_bytes(const sz_ensemble *e);
+/* nsteps SMR91 steps of every member; asynchronous. */
+int sz_ensemble_step(sz_ensemble *e, int nsteps);
+/* Wait; first nonzero per-system info since the last check, else 0. */
+int sz_ensemble_check(sz_ensemble *e);
+/* Per-system outcome of the last solve (sz_channel_solver_stats codes),
+ * members' systems in the solver's order. */
+void *sz_ensemble_solver_stats(sz_ensemble *e, int which);
+/* Autotune the explicit-stage stream count and the factorization unroll by
+ * timing `steps` eager steps per candidate from the same saved states; the
+ * states are restored afterwards.  Call before the first graph step. */
+int sz_ensemble_autotune(sz_ensemble *e, int steps);
+int sz_ensemble_tuning(const sz_ensemble *e, int *nstreams, int *lu_bs,
+                       double *trial_ms, int max, int *ntrials);
 
 /* ------------------------------------------------- solver handle (C view)
  * C handle over suzerain_amd::bsmbsm_solver (include/suzerain_amd_solver.hpp),

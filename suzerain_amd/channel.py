@@ -124,6 +124,42 @@ class Channel:
         self.Xw, self.nsys, self.F, self.Fp = Xw.value, nsys.value, F.value, Fp.value
         self.shape = (5, dNz, self.Xw, Ny)
 
+    @classmethod
+    def view(cls, handle, Nx, Ny, Nz, dNx=None, dNz=None, k=8, htdelta=3.0,
+             Lx=4 * math.pi, Ly=2.0, Lz=4 * math.pi / 3, scenario=None,
+             lower=(1.0, 0.0, 0.0, 0.0), upper=(1.0, 0.0, 0.0, 0.0),
+             stream=None):
+        """A non-owning Channel over an existing sz_channel handle (an
+        ensemble member) made from the same arguments."""
+        import torch
+        self = cls.__new__(cls)
+        self.scenario = dict(DEFAULT_SCENARIO, **(scenario or {}))
+        dNx = dNx or 3 * Nx // 2
+        dNz = dNz or 3 * Nz // 2
+        self.Nx, self.Ny, self.Nz, self.dNx, self.dNz, self.k = (
+            Nx, Ny, Nz, dNx, dNz, k)
+        self.htdelta, self.Lx, self.Ly, self.Lz = htdelta, Lx, Ly, Lz
+        self.lower, self.upper = lower, upper
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        self._stream = stream
+        self._h = handle
+        self._owned = False
+        Xw, nsys = C.c_int(), C.c_int()
+        F, Fp = C.c_longlong(), C.c_longlong()
+        _lib().sz_channel_geometry(self._h, C.byref(Xw), C.byref(nsys),
+                                   C.byref(F), C.byref(Fp))
+        self.Xw, self.nsys, self.F, self.Fp = Xw.value, nsys.value, F.value, Fp.value
+        self.shape = (5, dNz, self.Xw, Ny)
+        return self
+
+    def candidates(self):
+        """The 12 stable step candidates of the last zeroth substep."""
+        out = np.empty(12)
+        p = _lib().sz_channel_buffer(self._h, 10)
+        self._copy(out.ctypes.data, p, out.nbytes, 2)
+        return out
+
     # -------------------------------------------------------------- access
     def _copy(self, dst, src, nbytes, kind):
         rc = _hip().hipMemcpyAsync(C.c_void_p(dst), C.c_void_p(src),
@@ -269,7 +305,7 @@ class Channel:
 
     def __del__(self):
         try:
-            if getattr(self, "_h", None):
+            if getattr(self, "_h", None) and getattr(self, "_owned", True):
                 _lib().sz_channel_free(self._h)
                 self._h = None
         except Exception:

@@ -380,8 +380,21 @@ const double TWO_PI = 6.283185307179586476925286766559005768394338798750211642;
 
 extern "C" {
 
+sz_channel *sz_internal_channel_create(const sz_channel_spec *spec, void *stream,
+                                       int defer_scratch, char *err, int errlen);
+
 sz_channel *sz_channel_create(const sz_channel_spec *spec, void *stream,
                               char *err, int errlen)
+{
+    return sz_internal_channel_create(spec, stream, 0, err, errlen);
+}
+
+// As sz_channel_create; with defer_scratch the explicit operator's scratch
+// (derivative, transform and flux buffers) is not allocated: the caller
+// (an ensemble sharing one set among members run one after another) hands
+// it in with sz_internal_channel_set_scratch before the first step.
+sz_channel *sz_internal_channel_create(const sz_channel_spec *spec, void *stream,
+                                       int defer_scratch, char *err, int errlen)
 {
     if (!spec) { set_err(err, errlen, "null spec"); return nullptr; }
     const sz_channel_spec &sp = *spec;
@@ -454,22 +467,23 @@ sz_channel *sz_channel_create(const sz_channel_spec *spec, void *stream,
 
         const sz_bsmbsm g = sz_bsmbsm_construct(5, sp.Ny, sz_bsplineop_max_kl(h->bop),
                                                 sz_bsplineop_max_ku(h->bop));
-        const suzerain_amd::specification_zgbsv zspec(h->solver_spec);
-        if (zspec.method() == suzerain_amd::specification_zgbsv::zcgbsvx) {
+        // "none": no implicit solver of its own (an ensemble solves its
+        // members' operators together)
+        const bool no_solver = h->solver_spec == "none";
+        const suzerain_amd::specification_zgbsv zspec(no_solver ? "zgbsv"
+                                                                : h->solver_spec);
+        if (no_solver) {
+            // nothing
+        } else if (zspec.method() == suzerain_amd::specification_zgbsv::zcgbsvx) {
             sz_prec_params pp{};
-            pp.nsys = h->nsys;
-            pp.n = sp.Ny;
-            pp.KL = g.KL;
-            pp.KU = g.KU;
             pp.nslots = zspec.reuse() ? 3 : 1;
             pp.smax = zspec.siter();
             pp.dmax = zspec.diter();
             pp.aiter = zspec.aiter();
             pp.tolsc = zspec.tolsc();
             pp.reuse = zspec.reuse() ? 1 : 0;
-            extern int sz_internal_imexop_pack_chunks(const sz_imexop *op);
-            pp.nchunks = sz_internal_imexop_pack_chunks(h->imx);
-            h->prec = sz_prec_create(pp, h->stream);
+            const std::vector<sz_prec_member> mem{{sp.Ny, g.KL, g.KU, h->nsys}};
+            h->prec = sz_prec_create(mem, pp, h->stream);
             if (!h->prec) throw std::runtime_error("mixed precision solver setup");
             h->prec_slots = pp.nslots;
         } else {
@@ -480,13 +494,15 @@ sz_channel *sz_channel_create(const sz_channel_spec *spec, void *stream,
         // device storage
         h->buf[0] = h->dalloc<zd>((size_t)h->F * 5);
         h->buf[1] = h->dalloc<zd>((size_t)h->F * 5);
-        h->vals = h->dalloc<zd>((size_t)h->F * 5);
-        h->auxw = h->dalloc<zd>((size_t)h->F * 41);
-        h->phys = h->dalloc<double>((size_t)h->Fp * 46);
-        h->mixed = h->dalloc<zd>((size_t)sz_fft_mixed_elements(h->fft) * 46);
+        if (!defer_scratch) {
+            h->vals = h->dalloc<zd>((size_t)h->F * 5);
+            h->auxw = h->dalloc<zd>((size_t)h->F * 41);
+            h->phys = h->dalloc<double>((size_t)h->Fp * 46);
+            h->mixed = h->dalloc<zd>((size_t)sz_fft_mixed_elements(h->fft) * 46);
+            h->work = h->dalloc<double>(
+                (size_t)(sz_ns_workspace_bytes(sp.Ny, sp.dNz, sp.dNx) / 8 + 1));
+        }
         h->refs = h->dalloc<double>((size_t)SZ_NS_NREF * sp.Ny);
-        h->work = h->dalloc<double>(
-            (size_t)(sz_ns_workspace_bytes(sp.Ny, sp.dNz, sp.dNx) / 8 + 1));
         h->dbuf = h->dalloc<double>(12 + 4 + 2 * (size_t)sp.Ny);
         hipMemsetAsync(h->buf[0], 0, (size_t)h->F * 5 * sizeof(zd), h->stream);
         hipMemsetAsync(h->buf[1], 0, (size_t)h->F * 5 * sizeof(zd), h->stream);
@@ -570,6 +586,7 @@ void *sz_channel_buffer(sz_channel *h, int which)
     case 7: return h->dbuf + 16;              // lambda1_y then lambda2_y
     case 8: return h->bop;
     case 9: return h->solvers.empty() ? nullptr : h->solvers[0];
+    case 10: return h->dbuf;                  // the 12 step candidates
     default: return nullptr;
     }
 }
@@ -581,6 +598,7 @@ int sz_channel_nonlinear(sz_channel *h, int which, int zeroth,
     if (!h) return -1;
     sz::scope_range range("apply_navier_stokes_spatial_operator");
     if (which != 0 && which != 1) return -2;
+    if (!h->vals) return SZ_ERR_ALLOC;          // deferred scratch not handed in
     const sz_channel_spec &sp = h->spec;
     zd *s = h->state(which);
     const long long F = h->F;
@@ -751,7 +769,7 @@ int sz_channel_invert_operator_slot(sz_channel *h, double phi, int which, int sl
 {
     if (!h) return -1;
     if (which != 0 && which != 1) return -3;
-    if (slot < 0 || slot >= sz_channel_solver_slots(h)) return -4;
+    if (slot < 0 || slot >= sz_channel_solver_slots(h)) return -4;   // or "none"
     sz::scope_range range("invert_mass_plus_scaled_operator");
     if (h->prec) {
         sz::scope_range r2("bsmbsm_solver_zcgbsvx::suzerain_lapackext_zcgbsvx");
@@ -767,7 +785,7 @@ int sz_channel_invert_operator_slot(sz_channel *h, double phi, int which, int sl
         pr.F = h->F;
         pr.sysoff = h->sysoff;
         if (h->time_factor) h->factor_mark();
-        const int rc = sz_prec_solve(h->prec, slot, pr);
+        const int rc = sz_prec_solve(h->prec, slot, &pr);
         if (h->time_factor) h->factor_mark();
         if (rc) return rc;
         return h->zero_dealias(h->state(which), 5);
@@ -937,6 +955,66 @@ int sz_channel_factor_timing(sz_channel *h, int enable, double *total_ms,
 void sz_channel_exchange(sz_channel *h)
 {
     if (h) h->ia = 1 - h->ia;
+}
+
+// Bytes of the explicit operator's scratch: vals, auxw, phys, mixed, work.
+void sz_internal_channel_scratch_bytes(const sz_channel *h, long long out[5])
+{
+    const sz_channel_spec &sp = h->spec;
+    out[0] = h->F * 5 * (long long)sizeof(zd);
+    out[1] = h->F * 41 * (long long)sizeof(zd);
+    out[2] = h->Fp * 46 * (long long)sizeof(double);
+    out[3] = sz_fft_mixed_elements(h->fft) * 46 * (long long)sizeof(zd);
+    out[4] = (sz_ns_workspace_bytes(sp.Ny, sp.dNz, sp.dNx) / 8 + 1) * 8;
+}
+
+void sz_internal_channel_set_scratch(sz_channel *h, void *const p[5])
+{
+    h->vals = static_cast<zd *>(p[0]);
+    h->auxw = static_cast<zd *>(p[1]);
+    h->phys = static_cast<double *>(p[2]);
+    h->mixed = static_cast<zd *>(p[3]);
+    h->work = static_cast<double *>(p[4]);
+}
+
+// The mixed precision solver's view of this channel's implicit problem on
+// state `which` with phi: operator, wavenumbers, references, walls, state.
+int sz_internal_channel_problem(sz_channel *h, double phi, int which, void *out,
+                                int *n, int *KL, int *KU, int *nsys)
+{
+    if (!h) return -1;
+    sz_prec_problem pr{};
+    pr.op = h->imx;
+    pr.ref = &h->iref;
+    pr.ld = &h->ild;
+    pr.bc = &h->spec.bc;
+    pr.km = h->km;
+    pr.kn = h->kn;
+    pr.phi = phi;
+    pr.state = h->state(which);
+    pr.F = h->F;
+    pr.sysoff = h->sysoff;
+    if (out) *static_cast<sz_prec_problem *>(out) = pr;
+    const sz_bsmbsm g = sz_bsmbsm_construct(5, h->Ny, sz_bsplineop_max_kl(h->bop),
+                                            sz_bsplineop_max_ku(h->bop));
+    if (n) *n = h->Ny;
+    if (KL) *KL = g.KL;
+    if (KU) *KU = g.KU;
+    if (nsys) *nsys = h->nsys;
+    return 0;
+}
+
+hipStream_t sz_internal_channel_stream(sz_channel *h) { return h ? h->stream : nullptr; }
+
+void sz_internal_channel_set_stream(sz_channel *h, hipStream_t s)
+{
+    if (h) h->stream = s;
+}
+
+// Bytes of one state (5 wave fields).
+long long sz_internal_channel_state_bytes(const sz_channel *h)
+{
+    return h ? h->F * 5 * (long long)sizeof(zd) : 0;
 }
 
 // One SMR91 step of lowstorage.hpp:1461-1520 advancing state a; b is scratch.

@@ -71,6 +71,14 @@ __device__ inline cpx<double> cadd_rn(cpx<double> a, cpx<double> b)
 template <typename R>
 __device__ inline cpx<R> cconj(cpx<R> a) { return {a.x, -a.y}; }
 
+// One system of a grouped launch: systems of different orders (different
+// ensemble members, each with storage of its own) share one launch; device
+// addresses of the system's band storage, pivots and right hand side.
+struct gsys {
+    int n, pad;
+    unsigned long long ab, ipiv, b;
+};
+
 // a - b*c with fused multiply-adds.
 template <typename R>
 __device__ inline cpx<R> cfnms(cpx<R> a, cpx<R> b, cpx<R> c)

@@ -356,16 +356,25 @@ void k_gbtrs(int trans, int n, int kl, int ku, int nrhs,
              const cpx<R> *__restrict__ ab, int ldab, long long sab,
              const int *__restrict__ ipiv, long long sipiv,
              cpx<R> *__restrict__ b, int ldb, long long sb,
-             const int *__restrict__ skip, int nsys, int skip_ut)
+             const int *__restrict__ skip, int nsys, int skip_ut,
+             const sz::gsys *__restrict__ gs = nullptr)
 {
     using T = cpx<R>;
     extern __shared__ __align__(16) unsigned char smem[];
     const int sys = blockIdx.x;
     if (sys >= nsys) return;
     if (skip && skip[sys] != 0) return;   // singular factor: leave b as is
-    ab += (size_t)sys * (size_t)sab;
-    ipiv += (size_t)sys * (size_t)sipiv;
-    b += (size_t)sys * (size_t)sb;
+    if (gs) {                             // grouped: per-system order, storage
+        const sz::gsys g = gs[sys];
+        n = g.n;
+        ab = reinterpret_cast<const T *>(g.ab);
+        ipiv = reinterpret_cast<const int *>(g.ipiv);
+        b = reinterpret_cast<T *>(g.b);
+    } else {
+        ab += (size_t)sys * (size_t)sab;
+        ipiv += (size_t)sys * (size_t)sipiv;
+        b += (size_t)sys * (size_t)sb;
+    }
     const int tid = threadIdx.x;
     const int kv = kl + ku;
     T *x = reinterpret_cast<T *>(smem);
@@ -600,7 +609,7 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
                   int *__restrict__ ipiv, long long sipiv,
                   int *__restrict__ info, const int *__restrict__ skip,
                   int nsys, cpx<R> *__restrict__ bv = nullptr, long long sb = 0,
-                  int prio = 0)
+                  int prio = 0, const sz::gsys *__restrict__ gs = nullptr)
 {
     using T = cpx<R>;
     constexpr int kl = WR - 1, ku = KV - kl;
@@ -620,9 +629,17 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
     const int sys = blockIdx.x;
     if (sys >= nsys) return;
     if (skip && skip[sys] != 0) return;
-    ab += (size_t)sys * (size_t)sab;
-    ipiv += (size_t)sys * (size_t)sipiv;
-    if constexpr (UT) bv += (size_t)sys * (size_t)sb;
+    if (gs) {                             // grouped: per-system order, storage
+        const sz::gsys g = gs[sys];
+        n = g.n;
+        ab = reinterpret_cast<T *>(g.ab);
+        ipiv = reinterpret_cast<int *>(g.ipiv);
+        if constexpr (UT) bv = reinterpret_cast<T *>(g.b);
+    } else {
+        ab += (size_t)sys * (size_t)sab;
+        ipiv += (size_t)sys * (size_t)sipiv;
+        if constexpr (UT) bv += (size_t)sys * (size_t)sb;
+    }
     const int lane = threadIdx.x;
 
     // component-wise select (a select of whole structs goes through memory)
@@ -1152,9 +1169,83 @@ int launch_gbtrs(char trans, int n, int kl, int ku, int nrhs, const void *ab,
     return hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH;
 }
 
+// Grouped launches: every system of gs (orders up to nmax, band storage at
+// gs[s].ab with leading dimension ldab = 2 kl + ku + 1) in one launch.
+// The wave factorization kernel takes the channel band shapes (kl = ku = 34
+// and 24); bs selects its unrolled block (4 or 8 steps).
+template <typename R>
+int launch_gbtrf_grouped(int kl, int ku, void *ab, int ldab, int *ipiv, int *info,
+                         const int *skip, const sz::gsys *gs, int nsys, int bs,
+                         hipStream_t stream)
+{
+    if (nsys <= 0) return nsys < 0 ? -10 : 0;
+    if (ldab != 2 * kl + ku + 1) return -5;
+    const int prio = 0;
+    cpx<R> *a = reinterpret_cast<cpx<R> *>(ab);
+#define SZ_WAVE_G(WR_, KV_, BS_)                                                 \
+    hipLaunchKernelGGL((k_gbtrf_wave<R, WR_, KV_, BS_>), dim3(nsys), dim3(64), 0,  \
+                       stream, 0, a, ldab, 0LL, ipiv, 0LL, info, skip, nsys,     \
+                       nullptr, 0LL, prio, gs)
+    if (kl == 34 && ku == 34) {
+        if (bs == 4) SZ_WAVE_G(35, 68, 4);
+        else SZ_WAVE_G(35, 68, 8);
+    } else if (kl == 24 && ku == 24) {
+        if (bs == 4) SZ_WAVE_G(25, 48, 4);
+        else SZ_WAVE_G(25, 48, 8);
+    } else {
+        return SZ_ERR_LDS;
+    }
+#undef SZ_WAVE_G
+    return hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH;
+}
+
+template <typename R>
+int launch_gbtrs_grouped(char trans, int kl, int ku, const void *ab, int ldab,
+                         const int *ipiv, void *b, const int *skip,
+                         const sz::gsys *gs, int nsys, int nmax,
+                         hipStream_t stream)
+{
+    const int tc = trans_code(trans);
+    if (tc <= 0) return -1;              // the transposed solves only
+    if (nsys <= 0) return nsys < 0 ? -10 : 0;
+    if (kl + ku + 1 > TRS_PRE * TRS_NT) return SZ_ERR_LDS;
+    const size_t lds = (size_t)nmax * sizeof(cpx<R>);
+    if (lds > LDS_MAX) return SZ_ERR_LDS;
+    hipLaunchKernelGGL((k_gbtrs<R, true>), dim3(nsys), dim3(TRS_NT), lds, stream, tc,
+                       nmax, kl, ku, 1, reinterpret_cast<const cpx<R> *>(ab), ldab,
+                       0LL, ipiv, 0LL, reinterpret_cast<cpx<R> *>(b), nmax, 0LL, skip,
+                       nsys, 0, gs);
+    return hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH;
+}
+
 }  // namespace
 
 extern "C" {
+
+int sz_internal_gbtrf_grouped(int single, int kl, int ku, void *ab, int ldab,
+                              int *ipiv, int *info, const int *skip,
+                              const void *gs, int nsys, int bs, void *stream)
+{
+    const sz::gsys *g = static_cast<const sz::gsys *>(gs);
+    if (single)
+        return launch_gbtrf_grouped<float>(kl, ku, ab, ldab, ipiv, info, skip, g,
+                                           nsys, bs, (hipStream_t)stream);
+    return launch_gbtrf_grouped<double>(kl, ku, ab, ldab, ipiv, info, skip, g, nsys,
+                                        bs, (hipStream_t)stream);
+}
+
+int sz_internal_gbtrs_grouped(int single, char trans, int kl, int ku,
+                              const void *ab, int ldab, const int *ipiv, void *b,
+                              const int *skip, const void *gs, int nsys, int nmax,
+                              void *stream)
+{
+    const sz::gsys *g = static_cast<const sz::gsys *>(gs);
+    if (single)
+        return launch_gbtrs_grouped<float>(trans, kl, ku, ab, ldab, ipiv, b, skip, g,
+                                           nsys, nmax, (hipStream_t)stream);
+    return launch_gbtrs_grouped<double>(trans, kl, ku, ab, ldab, ipiv, b, skip, g,
+                                        nsys, nmax, (hipStream_t)stream);
+}
 
 int sz_zgbtrf_batched(int n, int kl, int ku, void *ab, int ldab,
                       long long strideab, int *ipiv, long long strideipiv,

@@ -291,13 +291,15 @@ template <> __device__ inline sz::zf to_out<sz::zf>(zd v)
 // equation rows are r0 .. r0+PK_RB-1 (column C = r*S + p holds equation
 // field_of[p] at wall-normal row r) for one system.  OUT: zd (double) or
 // zf (complex float factor storage, with pack_extra outputs).
-template <int SS, typename OUT = zd>
-__global__ __launch_bounds__(PK_NT)
-void k_imexop_pack(const table *__restrict__ tab, bops w, refs rf, zd phi,
-                   const double *__restrict__ km, const double *__restrict__ kn,
-                   int S_, int4 fof_lo, int fof_hi, int rho_pos, int KL, int KU,
-                   OUT *__restrict__ papt, int ldpapt, long long stride,
-                   int nchunks, wallbc bc, pack_extra ex)
+// The body of one pack workgroup: rows chunk*PK_RB.. of the system whose
+// band storage starts at P, wavenumbers kms, kns; bcs (8 values) and frob
+// (this workgroup's |a|^2 sum) are this system's extra outputs or null.
+template <int SS, typename OUT>
+__device__ __forceinline__ void
+pack_body(const table *__restrict__ tab, const bops &w, const refs &rf, zd phi,
+          double kms, double kns, int S_, int4 fof_lo, int fof_hi, int rho_pos,
+          int KL, int KU, OUT *__restrict__ P, int ldpapt, int chunk,
+          const wallbc &bc, zd *bcs, double *frob)
 {
     extern __shared__ double smem[];
     zd *alpha = reinterpret_cast<zd *>(smem);                   // MAXTERMS
@@ -310,9 +312,6 @@ void k_imexop_pack(const table *__restrict__ tab, bops w, refs rf, zd phi,
     __shared__ const double *srp[NREF];
 
     const int S = SS ? SS : S_;                 // fields per row (5)
-    const int s = blockIdx.x / nchunks;
-    const int chunk = blockIdx.x - s * nchunks;
-    if (ex.skip && ex.skip[s]) return;          // uniform over the block
     const int n = w.n, N = S * n;
     const int r0 = chunk * PK_RB;
     const int nr = min(PK_RB, n - r0);
@@ -324,7 +323,6 @@ void k_imexop_pack(const table *__restrict__ tab, bops w, refs rf, zd phi,
              : p == 3 ? fof_lo.w : fof_hi;
     };
 
-    const double kms = km[s], kns = kn[s];
     for (int t = tid; t < tab->nterms; t += PK_NT)
         alpha[t] = sz::cmul(phi, kappa(tab->tk[t], kms, kns));
     if (tid < 25) {
@@ -363,7 +361,6 @@ void k_imexop_pack(const table *__restrict__ tab, bops w, refs rf, zd phi,
     __syncthreads();
     const pack_tab pt{sbg0, sbng, sgop, skl, sku};
 
-    OUT *P = papt + (size_t)s * stride;
     const int LD = KL + 1 + KU;
     double fsum = 0;
     const int total = S * nr * LD;
@@ -390,13 +387,13 @@ void k_imexop_pack(const table *__restrict__ tab, bops w, refs rf, zd phi,
                 for (int q = 0; q < 4; ++q)
                     if (q == fi) f = lo ? bc.fac[0][q] : bc.fac[1][q];
                 v = R == C ? sc : (R == rhoR ? zd{-sc.x * f, -sc.y * f} : zd{0, 0});
-                if (ex.bcs && R == C) {
+                if (bcs && R == C) {
                     const OUT o = to_out<OUT>(sc);
-                    ex.bcs[(size_t)s * 8 + (lo ? 0 : 4) + fi] = zd{(double)o.x, (double)o.y};
+                    bcs[(lo ? 0 : 4) + fi] = zd{(double)o.x, (double)o.y};
                 }
             }
             P[(size_t)C * ldpapt + KU + R - C] = to_out<OUT>(v);
-            if (ex.frob) fsum = fma(v.x, v.x, fma(v.y, v.y, fsum));
+            if (frob) fsum = fma(v.x, v.x, fma(v.y, v.y, fsum));
         }
         rr += drr;
         cl += dcl;
@@ -405,7 +402,7 @@ void k_imexop_pack(const table *__restrict__ tab, bops w, refs rf, zd phi,
             ++cl;
         }
     }
-    if (ex.frob) {
+    if (frob) {
         __shared__ double red[PK_NT / 64];
         for (int off = 32; off >= 1; off >>= 1) fsum += __shfl_xor(fsum, off);
         if ((tid & 63) == 0) red[tid >> 6] = fsum;
@@ -413,9 +410,75 @@ void k_imexop_pack(const table *__restrict__ tab, bops w, refs rf, zd phi,
         if (tid == 0) {
             double t = 0;
             for (int i = 0; i < PK_NT / 64; ++i) t += red[i];
-            ex.frob[(size_t)s * nchunks + chunk] = t;
+            *frob = t;
         }
     }
+}
+
+template <int SS, typename OUT = zd>
+__global__ __launch_bounds__(PK_NT)
+void k_imexop_pack(const table *__restrict__ tab, bops w, refs rf, zd phi,
+                   const double *__restrict__ km, const double *__restrict__ kn,
+                   int S_, int4 fof_lo, int fof_hi, int rho_pos, int KL, int KU,
+                   OUT *__restrict__ papt, int ldpapt, long long stride,
+                   int nchunks, wallbc bc, pack_extra ex)
+{
+    const int s = blockIdx.x / nchunks;
+    const int chunk = blockIdx.x - s * nchunks;
+    if (ex.skip && ex.skip[s]) return;          // uniform over the block
+    pack_body<SS, OUT>(tab, w, rf, phi, km[s], kn[s], S_, fof_lo, fof_hi, rho_pos,
+                       KL, KU, papt + (size_t)s * stride, ldpapt, chunk, bc,
+                       ex.bcs ? ex.bcs + (size_t)s * 8 : nullptr,
+                       ex.frob ? ex.frob + (size_t)s * nchunks + chunk : nullptr);
+}
+
+// One member (ensemble case) of a grouped launch: its operator, wavenumbers,
+// walls, and where its systems' data sit.  Launch blocks blk0 .. of the
+// member cover its systems (nchunks pack blocks or one accumulate block
+// each); sys0 is its first system in the concatenated per-system arrays.
+struct gmember {
+    const table *tab;
+    bops w;
+    refs rf;
+    zd phi;
+    const double *km, *kn;
+    wallbc bc;
+    int blk0, sys0, nsys, nchunks, KL, KU, ldlu;
+    long long v0, frob0;         // offsets: vectors, frob partial sums
+    sz::zf *lu;                  // the member's float factor storage (a slot)
+    zd *bcs;                     // its wall-row scalings (8 per system)
+};
+
+// Member of launch block b: the last member whose first block is <= b.
+__device__ __forceinline__ int member_of(const gmember *__restrict__ m, int nm,
+                                         int b)
+{
+    int lo = 0, hi = nm - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (m[mid].blk0 <= b) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Grouped single precision packf of every member's five-field operator with
+// isothermal walls (systems with skip[s] != 0 left alone).
+__global__ __launch_bounds__(PK_NT)
+void k_imexop_pack_grouped(const gmember *__restrict__ mem, int nm,
+                           const int *__restrict__ skip,
+                           double *__restrict__ frob)
+{
+    const gmember &M = mem[member_of(mem, nm, blockIdx.x)];
+    const int b = blockIdx.x - M.blk0;
+    const int ls = b / M.nchunks, chunk = b - ls * M.nchunks;
+    const int s = M.sys0 + ls;
+    if (skip && skip[s]) return;                 // uniform over the block
+    const int N = 5 * M.w.n;
+    pack_body<5, sz::zf>(M.tab, M.w, M.rf, M.phi, M.km[ls], M.kn[ls], 5,
+                         make_int4(0, 1, 2, 3), 4, 4, M.KL, M.KU,
+                         M.lu + (size_t)ls * M.ldlu * N + M.KL, M.ldlu, chunk,
+                         M.bc, M.bcs + (size_t)ls * 8,
+                         frob + M.frob0 + (size_t)ls * M.nchunks + chunk);
 }
 
 // the x, D x buffers take most of a CU's LDS (one workgroup per CU), so the
@@ -424,33 +487,24 @@ constexpr int AC_NT = 1024;
 
 // y_i <- (M + phi L) x + beta y_i for one system per workgroup.  Field f of
 // system s starts at in[f] + inoff(s) (complex elements, stride 1 in y).
-__global__ __launch_bounds__(AC_NT)
-void k_imexop_accumulate(const table *__restrict__ tab, bops w, refs rf, zd phi,
-                         const double *__restrict__ km,
-                         const double *__restrict__ kn, const zd *in0,
-                         const zd *in1, const zd *in2, const zd *in3,
-                         const zd *in4, const long long *__restrict__ inoff,
-                         long long instride, zd beta, int beta_zero, zd *out0,
-                         zd *out1, zd *out2, zd *out3, zd *out4,
-                         const long long *__restrict__ outoff,
-                         long long outstride, const int *__restrict__ skip)
+// The body of one accumulate workgroup: field j of x at in[j] (null: absent),
+// field i of y at out[i], wavenumbers kms, kns.
+__device__ __forceinline__ void
+acc_body(const table *__restrict__ tab, const bops &w, const refs &rf, zd phi,
+         double kms, double kns, const zd *const (&in)[5], zd *const (&out)[5],
+         zd beta, int beta_zero)
 {
     extern __shared__ double smem[];
-    if (skip && skip[blockIdx.x]) return;       // uniform over the block
     const int n = w.n;
     zd *alpha = reinterpret_cast<zd *>(smem);       // MAXTERMS
     zd *xs = alpha + MAXTERMS;                      // 5 x n
     zd *Dx = xs + 5 * n;                            // 3 x 5 x n
-    const int s = blockIdx.x, tid = threadIdx.x;
-    const zd *in[5] = {in0, in1, in2, in3, in4};
-    zd *out[5] = {out0, out1, out2, out3, out4};
+    const int tid = threadIdx.x;
     __shared__ int sbg0[25], sbng[25], sgop[MAXGROUPS], sgt0[MAXGROUPS], sgnt[MAXGROUPS];
     __shared__ int stref[MAXTERMS], srld[NREF];
     __shared__ const double *srp[NREF];
-    const long long io = inoff ? inoff[s] : (long long)s * instride;
-    const long long oo = outoff ? outoff[s] : (long long)s * outstride;
+    const long long io = 0, oo = 0;
 
-    const double kms = km[s], kns = kn[s];
     for (int t = tid; t < tab->nterms; t += AC_NT) {
         alpha[t] = sz::cmul(phi, kappa(tab->tk[t], kms, kns));
         stref[t] = tab->tref[t];
@@ -509,6 +563,51 @@ void k_imexop_accumulate(const table *__restrict__ tab, bops w, refs rf, zd phi,
         const zd m = Dx[(M * 5 + i) * n + r];
         *yp = zd{sm.x + m.x, sm.y + m.y};
     }
+}
+
+__global__ __launch_bounds__(AC_NT)
+void k_imexop_accumulate(const table *__restrict__ tab, bops w, refs rf, zd phi,
+                         const double *__restrict__ km,
+                         const double *__restrict__ kn, const zd *in0,
+                         const zd *in1, const zd *in2, const zd *in3,
+                         const zd *in4, const long long *__restrict__ inoff,
+                         long long instride, zd beta, int beta_zero, zd *out0,
+                         zd *out1, zd *out2, zd *out3, zd *out4,
+                         const long long *__restrict__ outoff,
+                         long long outstride, const int *__restrict__ skip)
+{
+    const int s = blockIdx.x;
+    if (skip && skip[s]) return;                // uniform over the block
+    const long long io = inoff ? inoff[s] : (long long)s * instride;
+    const long long oo = outoff ? outoff[s] : (long long)s * outstride;
+    const zd *const in[5] = {in0 ? in0 + io : nullptr, in1 ? in1 + io : nullptr,
+                             in2 ? in2 + io : nullptr, in3 ? in3 + io : nullptr,
+                             in4 ? in4 + io : nullptr};
+    zd *const out[5] = {out0 ? out0 + oo : nullptr, out1 ? out1 + oo : nullptr,
+                        out2 ? out2 + oo : nullptr, out3 ? out3 + oo : nullptr,
+                        out4 ? out4 + oo : nullptr};
+    acc_body(tab, w, rf, phi, km[s], kn[s], in, out, beta, beta_zero);
+}
+
+// Grouped y = (M + phi L) x of every member's systems (one block each) on
+// the compact (5, n) vectors at x + v0 + ls*5n, y likewise; systems with
+// skip[s] != 0 left alone.
+__global__ __launch_bounds__(AC_NT)
+void k_imexop_apply_grouped(const gmember *__restrict__ mem, int nm,
+                            const zd *__restrict__ x, zd *__restrict__ y,
+                            const int *__restrict__ skip)
+{
+    const gmember &M = mem[member_of(mem, nm, blockIdx.x)];
+    const int ls = blockIdx.x - M.blk0;
+    const int s = M.sys0 + ls;
+    if (skip && skip[s]) return;                 // uniform over the block
+    const int n = M.w.n;
+    const long long o = M.v0 + (long long)ls * 5 * n;
+    const zd *const in[5] = {x + o, x + o + n, x + o + 2 * n, x + o + 3 * n,
+                             x + o + 4 * n};
+    zd *const out[5] = {y + o, y + o + n, y + o + 2 * n, y + o + 3 * n,
+                        y + o + 4 * n};
+    acc_body(M.tab, M.w, M.rf, M.phi, M.km[ls], M.kn[ls], in, out, zd{0, 0}, 1);
 }
 
 // Zero the permuted right hand side entries of the replaced wall equations.
@@ -622,6 +721,9 @@ int positions(const int *pos, int field_of[5])
 }  // namespace
 
 extern "C" {
+
+int sz_internal_imexop_wall_factors(const sz_imexop *op,
+                                    const sz_isothermal_bc *bc, double *fac);
 
 int sz_rholut_imexop_packc_batched(
     const sz_imexop *op, const double *phi, const double *km, const double *kn,
@@ -809,6 +911,78 @@ int sz_internal_imexop_pack32(const sz_imexop *op, const double *phi,
                        zd{phi[0], phi[1]}, km, kn, 5, make_int4(0, 1, 2, 3), 4, 4,
                        g.KL, g.KU, (sz::zf *)lu32 + g.KL, ldlu, stride, nchunks,
                        wb, ex);
+    return launch_ok() ? 0 : SZ_ERR_LAUNCH;
+}
+
+// Grouped launches over several operators (ensemble members).  A member
+// descriptor is filled on the host (sz_internal_imexop_member) into an array
+// the caller copies to the device; blocks blk0.. of a launch belong to it.
+int sz_internal_imexop_member_bytes(void) { return (int)sizeof(gmember); }
+
+int sz_internal_imexop_member(const sz_imexop *op, const double *phi,
+                              const double *km, const double *kn,
+                              const sz_imexop_ref *r, const sz_imexop_refld *ld,
+                              const sz_isothermal_bc *bc, int blk0, int sys0,
+                              int nsys, int KL, int KU, int ldlu, void *lu,
+                              void *bcs, long long v0, long long frob0, void *out)
+{
+    if (!op || !phi || !r || !ld || !bc || !out) return -1;
+    gmember m{};
+    m.tab = op->dev;
+    m.w = make_bops(op->w);
+    if (!m.w.dt) return SZ_ERR_ALLOC;
+    m.rf = make_refs(r, ld);
+    m.phi = zd{phi[0], phi[1]};
+    m.km = km;
+    m.kn = kn;
+    double fac[8];
+    sz_internal_imexop_wall_factors(op, bc, fac);
+    m.bc.enforce[0] = bc->enforce_lower;
+    m.bc.enforce[1] = bc->enforce_upper;
+    for (int k = 0; k < 2; ++k)
+        for (int q = 0; q < 4; ++q) m.bc.fac[k][q] = fac[k * 4 + q];
+    m.blk0 = blk0;
+    m.sys0 = sys0;
+    m.nsys = nsys;
+    m.nchunks = (m.w.n + PK_RB - 1) / PK_RB;
+    m.KL = KL;
+    m.KU = KU;
+    m.ldlu = ldlu;
+    m.lu = static_cast<sz::zf *>(lu);
+    m.bcs = static_cast<zd *>(bcs);
+    m.v0 = v0;
+    m.frob0 = frob0;
+    std::memcpy(out, &m, sizeof(m));
+    return 0;
+}
+
+// Single precision packf of every member's systems (nblocks = the sum of
+// nsys * nchunks; ldmax = the largest B-spline band leading dimension).
+int sz_internal_imexop_pack_grouped(const void *dmem, int nm, int nblocks,
+                                    int ldmax, const int *skip, double *frob,
+                                    void *stream)
+{
+    if (nblocks <= 0) return 0;
+    const size_t lds = sizeof(zd) * (MAXTERMS + PK_RB * MAXGROUPS)
+                     + sizeof(double) * 3 * PK_RB * (size_t)ldmax;
+    if (lds > 64 * 1024) return SZ_ERR_LDS;
+    hipLaunchKernelGGL(k_imexop_pack_grouped, dim3(nblocks), dim3(PK_NT), lds,
+                       (hipStream_t)stream, (const gmember *)dmem, nm, skip, frob);
+    return launch_ok() ? 0 : SZ_ERR_LAUNCH;
+}
+
+// y = (M + phi L) x for every member's systems (nsys blocks in all; nmax the
+// largest n) on compact vectors.
+int sz_internal_imexop_apply_grouped(const void *dmem, int nm, int nsys,
+                                     int nmax, const void *x, void *y,
+                                     const int *skip, void *stream)
+{
+    if (nsys <= 0) return 0;
+    const size_t lds = sizeof(zd) * (MAXTERMS + 20 * (size_t)nmax);
+    if (lds > 156 * 1024) return SZ_ERR_LDS;
+    hipLaunchKernelGGL(k_imexop_apply_grouped, dim3(nsys), dim3(AC_NT), lds,
+                       (hipStream_t)stream, (const gmember *)dmem, nm,
+                       (const zd *)x, (zd *)y, skip);
     return launch_ok() ? 0 : SZ_ERR_LAUNCH;
 }
 

@@ -1,23 +1,27 @@
 // Mixed precision solve of the channel's implicit operator: complex float
 // factors of P (M + phi L)^T P^T (isothermal walls included) refined with
-// double precision residuals of the matrix-free operator (k_imexop_accumulate
-// for the interior equations, the wall constraints evaluated directly).
-// Semantics: sz_precision.hpp (suzerain_lapackext_zcgbsvx,
+// double precision residuals of the matrix-free operator (the imexop
+// accumulate body for the interior equations, the wall constraints evaluated
+// directly).  Semantics: sz_precision.hpp (suzerain_lapackext_zcgbsvx,
 // suzerain/blas_et_al/dsgbsvx.def:124-313).
 //
-// Layout: b, x and A x of every system are kept compact, (5, n) complex
-// doubles per system in (rho E, rho u, rho v, rho w, rho) order; the float
-// right hand side / correction PB32 is in the permuted order of the factors
-// (index i <-> field i mod 5, row i / 5).  Every system advances through a
-// small state machine in device memory; the host enqueues a fixed sequence
+// Layout: the systems of every member are numbered consecutively, members
+// of one band shape together.  b, x and A x of a system are kept compact,
+// (5, n) complex doubles in (rho E, rho u, rho v, rho w, rho) order at the
+// system's vector offset; the float right hand side / correction PB is in
+// the permuted order of the factors (index i <-> field i mod 5, row i / 5)
+// at the same offset, the pivots too.  Every system advances through a small
+// state machine in device memory; the host enqueues a fixed sequence
 // (gather, init, masked pack + factor, smax + 1 sweeps of {masked float
-// solve, update, masked residual}), then reads two counters once to decide
-// whether a restart pass or the double precision fallback is needed.
+// solve, update, masked residual}), then (outside graph capture) reads the
+// counters once to decide on a restart pass and the double fallback.
 #include "sz_precision.hpp"
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <new>
 #include <stdexcept>
 
@@ -33,6 +37,15 @@ extern "C" int sz_internal_gbtrs_masked(int single, char trans, int n, int kl,
                                         void *b, int ldb, long long strideb,
                                         const int *skip, int nbatch,
                                         void *stream);
+extern "C" int sz_internal_gbtrf_grouped(int single, int kl, int ku, void *ab,
+                                         int ldab, int *ipiv, int *info,
+                                         const int *skip, const void *gs,
+                                         int nsys, int bs, void *stream);
+extern "C" int sz_internal_gbtrs_grouped(int single, char trans, int kl, int ku,
+                                         const void *ab, int ldab,
+                                         const int *ipiv, void *b,
+                                         const int *skip, const void *gs,
+                                         int nsys, int nmax, void *stream);
 extern "C" int sz_internal_zgbtrf_ut(int n, int kl, int ku, void *ab, int ldab,
                                      long long strideab, int *ipiv,
                                      long long strideipiv, int *info, void *b,
@@ -42,23 +55,23 @@ extern "C" int sz_internal_zgbtrs_lt(int n, int kl, int ku, const void *ab,
                                      const int *ipiv, long long strideipiv,
                                      void *b, long long strideb,
                                      const int *skip, int nbatch, void *stream);
-extern "C" int sz_internal_imexop_pack32(const sz_imexop *op, const double *phi,
-                                         const double *km, const double *kn,
-                                         const sz_imexop_ref *r,
-                                         const sz_imexop_refld *ld, void *lu32,
-                                         int ldlu, long long stride, int nsys,
-                                         const sz_isothermal_bc *bc,
-                                         const int *skip, void *bcs,
-                                         double *frob, void *stream);
 extern "C" int sz_internal_imexop_wall_factors(const sz_imexop *op,
                                                const sz_isothermal_bc *bc,
                                                double *fac);
-extern "C" int sz_internal_imexop_accumulate_masked(
+extern "C" int sz_internal_imexop_member_bytes(void);
+extern "C" int sz_internal_imexop_member(
     const sz_imexop *op, const double *phi, const double *km, const double *kn,
-    const sz_imexop_ref *r, const sz_imexop_refld *ld,
-    const void *const *in, const long long *inoff, long long instride,
-    const double *beta, void *const *out, const long long *outoff,
-    long long outstride, int nsys, const int *skip, void *stream);
+    const sz_imexop_ref *r, const sz_imexop_refld *ld, const sz_isothermal_bc *bc,
+    int blk0, int sys0, int nsys, int KL, int KU, int ldlu, void *lu, void *bcs,
+    long long v0, long long frob0, void *out);
+extern "C" int sz_internal_imexop_pack_grouped(const void *dmem, int nm,
+                                               int nblocks, int ldmax,
+                                               const int *skip, double *frob,
+                                               void *stream);
+extern "C" int sz_internal_imexop_apply_grouped(const void *dmem, int nm,
+                                                int nsys, int nmax, const void *x,
+                                                void *y, const int *skip,
+                                                void *stream);
 
 namespace {
 
@@ -73,9 +86,30 @@ struct psys {
     double lastres, res, tol, tolconst;
 };
 
+// per system: order n, member, offset of its vectors (and pivots)
+struct pdesc {
+    int n, m;
+    long long v;
+};
+
 struct wall_consts {
     int enforce[2];
     double fac[2][4];     // E_factor, u, v, w per wall
+};
+
+// per member and slot: where its state lives this call, its walls, and its
+// factor state in this slot
+struct pmem {
+    zd *state;
+    long long F;
+    const long long *sysoff;
+    int sys0, nchunks;
+    long long frob0;
+    wall_consts wc;
+    char *fact;
+    int *apprx;
+    double *afrob;
+    const zd *bcs;
 };
 
 __device__ double block_sum(double v, double *red)
@@ -90,44 +124,40 @@ __device__ double block_sum(double v, double *red)
     return s;
 }
 
-// B <- the state's pencils of every system with the wall rows of the energy
-// and momentum equations zeroed (IsothermalPATPTEnforcer::rhs,
-// apps/perfect/operator_hybrid_isothermal.cpp:513-524); X <- 0.
-__global__ __launch_bounds__(PT)
-void k_p_gather(const zd *__restrict__ st, long long F,
-                const long long *__restrict__ sysoff, int n, int lower,
-                int upper, zd *__restrict__ B, zd *__restrict__ X, int nsys)
-{
-    const long long N = 5LL * n;
-    const long long e = (long long)blockIdx.x * PT + threadIdx.x;
-    if (e >= N * nsys) return;
-    const int s = (int)(e / N);
-    const int a = (int)(e - (long long)s * N);
-    const int f = a / n, y = a - f * n;
-    zd v = st[f * F + sysoff[s] + y];
-    if (f < 4 && ((y == 0 && lower) || (y == n - 1 && upper))) v = zd{0, 0};
-    B[e] = v;
-    X[e] = zd{0, 0};
-}
-
 struct p_args {
-    int n, N, smax, aiter, pass;
+    int smax, aiter, pass;
     double tolsc;
-    const zd *B;
-    zd *X;
+    const pdesc *D;
+    const pmem *PM;
+    zd *B, *X;
     const zd *T;
     zf *PB;
     psys *S;
     int *skip, *fskip;
-    char *fact;
-    int *apprx;
-    double *afrob;
     const double *frob;
-    int nchunks;
     const int *finfo;
-    const zd *bcs;
-    wall_consts wc;
 };
+
+// B <- the state's pencils with the wall rows of the energy and momentum
+// equations zeroed (IsothermalPATPTEnforcer::rhs,
+// apps/perfect/operator_hybrid_isothermal.cpp:513-524); X <- 0.  One block
+// per system.
+__global__ __launch_bounds__(PT) void k_p_gather(p_args A)
+{
+    const int s = blockIdx.x;
+    const pdesc d = A.D[s];
+    const pmem &M = A.PM[d.m];
+    const int n = d.n, N = 5 * n;
+    const zd *st = M.state + M.sysoff[s - M.sys0];
+    const int lower = M.wc.enforce[0], upper = M.wc.enforce[1];
+    for (int a = threadIdx.x; a < N; a += PT) {
+        const int f = a / n, y = a - f * n;
+        zd v = st[f * M.F + y];
+        if (f < 4 && ((y == 0 && lower) || (y == n - 1 && upper))) v = zd{0, 0};
+        A.B[d.v + a] = v;
+        A.X[d.v + a] = zd{0, 0};
+    }
+}
 
 // Start (pass 0: every system) or restart (pass 1: systems marked
 // P_RESTART, refactored from scratch) the single precision attempt.
@@ -140,11 +170,14 @@ __global__ __launch_bounds__(PT) void k_p_init(p_args A)
         if (tid == 0) A.fskip[s] = 1;
         return;
     }
-    const int N = A.N, n = A.n;
-    const zd *b = A.B + (size_t)s * N;
-    zd *x = A.X + (size_t)s * N;
+    const pdesc d = A.D[s];
+    const pmem &M = A.PM[d.m];
+    const int ls = s - M.sys0;
+    const int n = d.n, N = 5 * n;
+    const zd *b = A.B + d.v;
+    zd *x = A.X + d.v;
     // read before the reduction's barriers: thread 0 rewrites it at the end
-    const char f = A.pass == 1 ? 'N' : A.fact[s];
+    const char f = A.pass == 1 ? 'N' : M.fact[ls];
     double ss = 0;
     for (int i = tid; i < N; i += PT) {
         ss += b[i].x * b[i].x + b[i].y * b[i].y;
@@ -160,7 +193,7 @@ __global__ __launch_bounds__(PT) void k_p_init(p_args A)
     const bool active = above && A.smax >= 0;
     const bool need = active && f != 'S';
     if (active) {
-        zf *pb = A.PB + (size_t)s * N;
+        zf *pb = A.PB + d.v;
         for (int i = tid; i < N; i += PT) {
             const zd v = b[(i % 5) * n + i / 5];      // (P b)_i = b_{q(i)}
             pb[i] = zf{(float)v.x, (float)v.y};
@@ -177,12 +210,12 @@ __global__ __launch_bounds__(PT) void k_p_init(p_args A)
         A.skip[s] = active ? 0 : 1;
         A.fskip[s] = need ? 0 : 1;
         if (need) {
-            A.fact[s] = 'N';
-            A.apprx[s] = 0;
+            M.fact[ls] = 'N';
+            M.apprx[ls] = 0;
         } else if (active) {
             // factors of an earlier operator: approximate for this one
             // (bsmbsm_solver::supplied_PAPT with reuse, bsmbsm_solver.cpp:80-92)
-            A.apprx[s] = 1;
+            M.apprx[ls] = 1;
         }
     }
 }
@@ -195,36 +228,41 @@ __global__ void k_p_afterfactor(p_args A, int nsys)
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsys) return;
     psys &S = A.S[s];
+    const pdesc d = A.D[s];
     const bool use_eps = A.tolsc == 0.0;
+    const pmem &M = A.PM[d.m];
+    const int ls = s - M.sys0;
     if (A.fskip[s] == 0) {
+        const double *fr = A.frob + M.frob0 + (long long)ls * M.nchunks;
         double t = 0;
-        for (int c = 0; c < A.nchunks; ++c) t += A.frob[(size_t)s * A.nchunks + c];
-        A.afrob[s] = sqrt(t);
+        for (int c = 0; c < M.nchunks; ++c) t += fr[c];
+        M.afrob[ls] = sqrt(t);
         if (A.finfo[s] > 0) {
-            A.fact[s] = 'N';
+            M.fact[ls] = 'N';
             S.st = P_FAIL;
             A.skip[s] = 1;
         } else {
-            A.fact[s] = 'S';
+            M.fact[ls] = 'S';
         }
     }
     const double eps = DBL_EPSILON * 0.5;
-    S.tolconst = use_eps ? 1.0 : A.afrob[s] * eps * sqrt((double)A.N) * A.tolsc;
+    S.tolconst = use_eps ? 1.0 : M.afrob[ls] * eps * sqrt(5.0 * d.n) * A.tolsc;
 }
 
-// x += P^T (float correction) for the active systems.
-__global__ __launch_bounds__(PT) void k_p_update(p_args A, int nsys)
+// x += P^T (float correction) for the active systems; one block per system.
+__global__ __launch_bounds__(PT) void k_p_update(p_args A)
 {
-    const long long N = A.N;
-    const long long e = (long long)blockIdx.x * PT + threadIdx.x;
-    if (e >= N * nsys) return;
-    const int s = (int)(e / N);
+    const int s = blockIdx.x;
     if (A.skip[s]) return;
-    const int a = (int)(e - (long long)s * N);
-    const int f = a / A.n, y = a - f * A.n;
-    const zf d = A.PB[(size_t)s * N + y * 5 + f];     // (P^T d)_a = d_{q^-1(a)}
-    zd &x = A.X[e];
-    x = zd{x.x + (double)d.x, x.y + (double)d.y};
+    const pdesc d = A.D[s];
+    const int n = d.n, N = 5 * n;
+    const zf *pb = A.PB + d.v;
+    zd *x = A.X + d.v;
+    for (int a = threadIdx.x; a < N; a += PT) {
+        const int f = a / n, y = a - f * n;
+        const zf c = pb[y * 5 + f];                    // (P^T c)_a = c_{q^-1(a)}
+        x[a] = zd{x[a].x + (double)c.x, x[a].y + (double)c.y};
+    }
 }
 
 // r = b - A x (T holds the matrix-free (M + phi L) x; the wall rows are the
@@ -242,19 +280,24 @@ __global__ __launch_bounds__(PT) void k_p_post(p_args A)
     // read before the reductions' barriers: thread 0 rewrites S at the end
     const int siter0 = S.siter, info0 = S.info;
     const double lastres0 = S.lastres, tolconst = S.tolconst;
-    const int N = A.N, n = A.n;
-    const zd *b = A.B + (size_t)s * N;
-    const zd *x = A.X + (size_t)s * N;
-    const zd *t = A.T + (size_t)s * N;
+    const pdesc d = A.D[s];
+    const pmem &M = A.PM[d.m];
+    const int ls = s - M.sys0;
+    const wall_consts &wc = M.wc;
+    const int apprx = M.apprx[ls];
+    const int n = d.n, N = 5 * n;
+    const zd *b = A.B + d.v;
+    const zd *x = A.X + d.v;
+    const zd *t = A.T + d.v;
     zd *r = reinterpret_cast<zd *>(psm);
     double ss = 0, xs = 0;
     for (int a = tid; a < N; a += PT) {
         const int f = a / n, y = a - f * n;
         zd v;
         const int wall = (y == 0) ? 0 : (y == n - 1 ? 1 : -1);
-        if (f < 4 && wall >= 0 && A.wc.enforce[wall]) {
-            const zd sc = A.bcs[(size_t)s * 8 + wall * 4 + f];
-            const double fac = A.wc.fac[wall][f];
+        if (f < 4 && wall >= 0 && wc.enforce[wall]) {
+            const zd sc = M.bcs[(size_t)ls * 8 + wall * 4 + f];
+            const double fac = wc.fac[wall][f];
             const zd xr = x[4 * n + y];
             const zd c = zd{x[a].x - fac * xr.x, x[a].y - fac * xr.y};
             const zd p = sz::cmul(sc, c);
@@ -279,13 +322,13 @@ __global__ __launch_bounds__(PT) void k_p_post(p_args A)
         info = -100;                       // NaN residual from a finite start
     } else if (iter >= A.aiter && lastres0 < res * 2.0) {
         lastres = lastres0;                // residual failed to halve
-        st = A.apprx[s] ? P_RESTART : (res <= tol ? P_DONE : P_FAIL);
+        st = apprx ? P_RESTART : (res <= tol ? P_DONE : P_FAIL);
     } else if (!(iter < A.smax && res > tol)) {
         st = res <= tol ? P_DONE : P_FAIL;
     }
     if (st == P_ACTIVE) {
         __syncthreads();
-        zf *pb = A.PB + (size_t)s * N;
+        zf *pb = A.PB + d.v;
         for (int i = tid; i < N; i += PT) {
             const zd v = r[(i % 5) * n + i / 5];
             pb[i] = zf{(float)v.x, (float)v.y};
@@ -328,9 +371,9 @@ __global__ __launch_bounds__(PT) void k_p_count(const psys *S,
     }
 }
 
-// The failed systems in ascending order (one block): list, and their
-// wavenumbers and state offsets for the double precision pass.
-__global__ __launch_bounds__(PT) void k_p_compact(const psys *S, int nsys,
+// The failed systems of [s0, s0 + ns) in ascending order (one block):
+// member-local indices, their wavenumbers.
+__global__ __launch_bounds__(PT) void k_p_compact(const psys *S, int s0, int ns,
                                                   const double *km,
                                                   const double *kn, int *list,
                                                   double *skm, double *skn)
@@ -339,10 +382,9 @@ __global__ __launch_bounds__(PT) void k_p_compact(const psys *S, int nsys,
     __shared__ int wsum[PT / 64];
     if (threadIdx.x == 0) base = 0;
     __syncthreads();
-    for (int s0 = 0; s0 < nsys; s0 += PT) {
-        const int s = s0 + threadIdx.x;
-        const int flag = (s < nsys && S[s].st == P_FAIL) ? 1 : 0;
-        // exclusive prefix over the block
+    for (int l0 = 0; l0 < ns; l0 += PT) {
+        const int l = l0 + threadIdx.x;
+        const int flag = (l < ns && S[s0 + l].st == P_FAIL) ? 1 : 0;
         const unsigned long long m = __ballot(flag);
         const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
         const int before = __popcll(m & ((1ULL << lane) - 1ULL));
@@ -352,9 +394,9 @@ __global__ __launch_bounds__(PT) void k_p_compact(const psys *S, int nsys,
         for (int i = 0; i < w; ++i) off += wsum[i];
         if (flag) {
             const int j = off + before;
-            list[j] = s;
-            skm[j] = km[s];
-            skn[j] = kn[s];
+            list[j] = l;
+            skm[j] = km[l];
+            skn[j] = kn[l];
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -366,58 +408,62 @@ __global__ __launch_bounds__(PT) void k_p_compact(const psys *S, int nsys,
     }
 }
 
-// Double precision pass over failed systems list[j0 .. j0+m): permuted right
-// hand side in, solution out.
+// Double precision pass over one member's failed systems list[j0 .. j0+m)
+// (member-local indices; vectors at v0 + l * 5n).
 __global__ __launch_bounds__(PT)
-void k_p_sub_gather(const zd *__restrict__ B, const int *__restrict__ list,
+void k_p_sub_gather(const zd *__restrict__ B, long long v0, const int *list,
                     int j0, int m, int n, zd *__restrict__ PB)
 {
     const long long N = 5LL * n;
     const long long e = (long long)blockIdx.x * PT + threadIdx.x;
     if (e >= N * m) return;
     const int j = (int)(e / N), i = (int)(e - (long long)j * N);
-    PB[e] = B[(size_t)list[j0 + j] * N + (i % 5) * n + i / 5];
+    PB[e] = B[v0 + (long long)list[j0 + j] * N + (i % 5) * n + i / 5];
 }
 
 __global__ __launch_bounds__(PT)
-void k_p_sub_scatter(const zd *__restrict__ PB, const int *__restrict__ list,
-                     const int *__restrict__ finfo, int j0, int m, int n,
+void k_p_sub_scatter(const zd *__restrict__ PB, long long v0, int s0,
+                     const int *list, const int *finfo, int j0, int m, int n,
                      zd *__restrict__ X, psys *S)
 {
     const long long N = 5LL * n;
     const long long e = (long long)blockIdx.x * PT + threadIdx.x;
     if (e >= N * m) return;
     const int j = (int)(e / N), a = (int)(e - (long long)j * N);
-    const int s = list[j0 + j];
+    const int l = list[j0 + j];
     const int f = a / n, y = a - f * n;
-    if (finfo[j] == 0) X[(size_t)s * N + a] = PB[(size_t)j * N + y * 5 + f];
+    if (finfo[j] == 0) X[v0 + (long long)l * N + a] = PB[(size_t)j * N + y * 5 + f];
     if (a == 0) {
-        S[s].st = P_DONE;
-        S[s].diter = 1;
-        S[s].info = finfo[j];
+        S[s0 + l].st = P_DONE;
+        S[s0 + l].diter = 1;
+        S[s0 + l].info = finfo[j];
     }
 }
 
-// x -> state pencils of every system.
-__global__ __launch_bounds__(PT)
-void k_p_scatter(const zd *__restrict__ X, int n, zd *__restrict__ st,
-                 long long F, const long long *__restrict__ sysoff, int nsys)
+// x -> state pencils of every system; one block per system.
+__global__ __launch_bounds__(PT) void k_p_scatter(p_args A)
 {
-    const long long N = 5LL * n;
-    const long long e = (long long)blockIdx.x * PT + threadIdx.x;
-    if (e >= N * nsys) return;
-    const int s = (int)(e / N);
-    const int a = (int)(e - (long long)s * N);
-    const int f = a / n, y = a - f * n;
-    st[f * F + sysoff[s] + y] = X[e];
+    const int s = blockIdx.x;
+    const pdesc d = A.D[s];
+    const pmem &M = A.PM[d.m];
+    const int n = d.n, N = 5 * n;
+    zd *st = M.state + M.sysoff[s - M.sys0];
+    for (int a = threadIdx.x; a < N; a += PT) {
+        const int f = a / n, y = a - f * n;
+        st[f * M.F + y] = A.X[d.v + a];
+    }
 }
 
 // Per-system outcome arrays and the first-info record.
-__global__ void k_p_stats(const psys *S, int nsys, int *siter, int *diter,
-                          double *tolsc, double *res, int *info, int *first)
+__global__ void k_p_stats(p_args A, const psys *S, int nsys, int *siter,
+                          int *diter, double *tolsc, double *res, int *info,
+                          int *first, char *fact, int *apprx)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsys) return;
+    const pmem &M = A.PM[A.D[s].m];
+    fact[s] = M.fact[s - M.sys0];
+    apprx[s] = M.apprx[s - M.sys0];
     siter[s] = S[s].siter;
     diter[s] = S[s].diter;
     res[s] = S[s].res;
@@ -446,8 +492,8 @@ inline bool ok() { return hipGetLastError() == hipSuccess; }
 struct sz_prec {
     sz_prec_params p{};
     hipStream_t stream = nullptr;
-    int N = 0, ldlu = 0;
-    struct slot {
+    // members by id; the live ones are solved in id order
+    struct slotdata {
         zf *lu = nullptr;
         int *ipiv = nullptr;
         char *fact = nullptr;
@@ -455,8 +501,40 @@ struct sz_prec {
         double *afrob = nullptr;
         zd *bcs = nullptr;
     };
+    struct mstore {
+        sz_prec_member g{};
+        bool alive = false;
+        std::vector<slotdata> sl;
+        std::vector<void *> owned;
+        long long bytes = 0;
+    };
+    std::vector<mstore> ms;
+    std::vector<int> live;                // ids of live members, id order
+    // layout of the live members (rebuilt when membership changes)
+    bool dirty = true;
+    std::vector<int> order;               // internal position -> live index
+    std::vector<int> sys0, nchunks;       // per internal position
+    std::vector<long long> v0, frob0;
+    int nsys = 0, nmax = 0, ldmax = 0, nblocks_pack = 0;
+    long long nvec = 0, nfrob = 0;
+    struct lu_class {
+        int KL, KU, ldlu, s0, ns, nmax;
+        std::vector<sz::gsys *> gs;       // per slot, device, ns entries
+    };
+    std::vector<lu_class> classes;
+    struct slot {
+        unsigned char *gmem = nullptr;    // imexop member descriptors (one
+                                          // block per system)
+        unsigned char *gpack = nullptr;   // the same with pack block offsets
+        pmem *pm = nullptr;               // member descriptors
+        std::vector<unsigned char> last;  // host copy of what was uploaded
+    };
     std::vector<slot> slots;
     int last_slot = 0;
+    // pools (grow-only), sized for the live members
+    long long cap_vec = 0, cap_frob = 0;
+    int cap_sys = 0, cap_mem = 0;
+    pdesc *D = nullptr;
     zd *B = nullptr, *X = nullptr, *T = nullptr;
     zf *PB = nullptr;
     psys *S = nullptr;
@@ -466,78 +544,274 @@ struct sz_prec {
     int *list = nullptr;
     double *skm = nullptr, *skn = nullptr;
     int *o_siter = nullptr, *o_diter = nullptr, *o_info = nullptr,
-        *o_first = nullptr;
+        *o_first = nullptr, *o_apprx = nullptr;
+    char *o_fact = nullptr;
     double *o_tolsc = nullptr, *o_res = nullptr;
     // double precision fallback, allocated on first use
     int cap = 0;
+    long long cap_elems = 0;
     zd *lu64 = nullptr, *pb64 = nullptr;
     int *ipiv64 = nullptr, *info64 = nullptr;
-    long long bytes = 0;
+    long long pool_bytes = 0;
     int n_factored = 0, n_restarted = 0, n_fallback = 0;
-    std::vector<void *> owned;
 
-    template <typename T>
-    T *dalloc(size_t count)
+    static void *raw_alloc(size_t bytes)
     {
         void *q = nullptr;
-        if (count == 0) count = 1;
-        if (hipMalloc(&q, count * sizeof(T)) != hipSuccess) throw std::bad_alloc();
-        owned.push_back(q);
-        bytes += (long long)(count * sizeof(T));
-        return static_cast<T *>(q);
+        if (hipMalloc(&q, bytes ? bytes : 1) != hipSuccess) throw std::bad_alloc();
+        return q;
+    }
+    template <typename T>
+    void realloc_pool(T *&ptr, size_t count)
+    {
+        if (ptr) (void)hipFree(ptr);
+        ptr = static_cast<T *>(raw_alloc(count * sizeof(T)));
+    }
+    void free_pools()
+    {
+        void *ps[] = {D, B, X, T, PB, S, skip, fskip, finfo, frob, list, skm, skn,
+                      o_siter, o_diter, o_info, o_first, o_apprx, o_fact, o_tolsc,
+                      o_res, lu64, pb64, ipiv64, info64, counts};
+        for (void *q : ps) if (q) (void)hipFree(q);
+        for (auto &sl : slots) {
+            if (sl.gmem) (void)hipFree(sl.gmem);
+            if (sl.gpack) (void)hipFree(sl.gpack);
+            if (sl.pm) (void)hipFree(sl.pm);
+        }
+        for (auto &c : classes)
+            for (auto *g : c.gs) if (g) (void)hipFree(g);
     }
     ~sz_prec()
     {
-        for (void *q : owned) (void)hipFree(q);
+        (void)hipStreamSynchronize(stream);
+        free_pools();
+        for (auto &m : ms)
+            for (void *q : m.owned) (void)hipFree(q);
         if (hcounts) (void)hipHostFree(hcounts);
     }
 };
 
-sz_prec *sz_prec_create(const sz_prec_params &p, hipStream_t stream)
+namespace {
+
+int nchunks_of(int n) { return (n + 3) / 4; }
+
+void rebuild(sz_prec *h)
 {
+    if (!h->dirty) return;
+    (void)hipStreamSynchronize(h->stream);     // pools may be reallocated
+    // order: live members of one band shape together
+    const int nl = (int)h->live.size();
+    h->order.resize((size_t)nl);
+    for (int i = 0; i < nl; ++i) h->order[(size_t)i] = i;
+    std::stable_sort(h->order.begin(), h->order.end(), [&](int a, int b) {
+        const auto &x = h->ms[(size_t)h->live[(size_t)a]].g;
+        const auto &y = h->ms[(size_t)h->live[(size_t)b]].g;
+        return x.KL != y.KL ? x.KL > y.KL : x.KU > y.KU;
+    });
+    for (auto &c : h->classes)
+        for (auto *g : c.gs) if (g) (void)hipFree(g);
+    h->classes.clear();
+    h->sys0.clear();
+    h->nchunks.clear();
+    h->v0.clear();
+    h->frob0.clear();
+    h->nsys = h->nmax = h->ldmax = h->nblocks_pack = 0;
+    h->nvec = h->nfrob = 0;
+    std::vector<pdesc> D;
+    for (int i = 0; i < nl; ++i) {
+        const sz_prec::mstore &m = h->ms[(size_t)h->live[(size_t)h->order[(size_t)i]]];
+        const int N = 5 * m.g.n, ldlu = 2 * m.g.KL + m.g.KU + 1;
+        if (h->classes.empty() || h->classes.back().KL != m.g.KL ||
+            h->classes.back().KU != m.g.KU)
+            h->classes.push_back({m.g.KL, m.g.KU, ldlu, h->nsys, 0, 0, {}});
+        auto &c = h->classes.back();
+        h->sys0.push_back(h->nsys);
+        h->v0.push_back(h->nvec);
+        h->frob0.push_back(h->nfrob);
+        const int nch = nchunks_of(m.g.n);
+        h->nchunks.push_back(nch);
+        for (int l = 0; l < m.g.nsys; ++l)
+            D.push_back(pdesc{m.g.n, i, h->nvec + (long long)l * N});
+        c.ns += m.g.nsys;
+        c.nmax = std::max(c.nmax, N);
+        h->nsys += m.g.nsys;
+        h->nvec += (long long)m.g.nsys * N;
+        h->nfrob += (long long)m.g.nsys * nch;
+        h->nblocks_pack += m.g.nsys * nch;
+        h->nmax = std::max(h->nmax, m.g.n);
+        // B-spline band leading dimension: KL = 5 (kl + 1) - 1
+        h->ldmax = std::max(h->ldmax, (m.g.KL + 1) / 5 + (m.g.KU + 1) / 5 - 1);
+    }
+    const int ns = std::max(h->nsys, 1);
+    if (h->nsys > h->cap_sys || !h->S) {
+        const int c = std::max(ns, h->cap_sys * 5 / 4);
+        h->realloc_pool(h->D, (size_t)c);
+        h->realloc_pool(h->S, (size_t)c);
+        h->realloc_pool(h->skip, (size_t)c);
+        h->realloc_pool(h->fskip, (size_t)c);
+        h->realloc_pool(h->finfo, (size_t)c);
+        h->realloc_pool(h->list, (size_t)c);
+        h->realloc_pool(h->skm, (size_t)c);
+        h->realloc_pool(h->skn, (size_t)c);
+        h->realloc_pool(h->o_siter, (size_t)c);
+        h->realloc_pool(h->o_diter, (size_t)c);
+        h->realloc_pool(h->o_info, (size_t)c);
+        h->realloc_pool(h->o_first, (size_t)c);
+        h->realloc_pool(h->o_apprx, (size_t)c);
+        h->realloc_pool(h->o_fact, (size_t)c);
+        h->realloc_pool(h->o_tolsc, (size_t)c);
+        h->realloc_pool(h->o_res, (size_t)c);
+        hipLaunchKernelGGL(k_p_fill_int, dim3(nb(c)), dim3(PT), 0, h->stream,
+                           h->o_first, 0, c);
+        h->cap_sys = c;
+    }
+    if (h->nvec > h->cap_vec || !h->B) {
+        const long long c = std::max(std::max(h->nvec, 1LL), h->cap_vec * 5 / 4);
+        h->realloc_pool(h->B, (size_t)c);
+        h->realloc_pool(h->X, (size_t)c);
+        h->realloc_pool(h->T, (size_t)c);
+        h->realloc_pool(h->PB, (size_t)c);
+        h->cap_vec = c;
+    }
+    if (h->nfrob > h->cap_frob || !h->frob) {
+        const long long c = std::max(std::max(h->nfrob, 1LL), h->cap_frob * 5 / 4);
+        h->realloc_pool(h->frob, (size_t)c);
+        h->cap_frob = c;
+    }
+    if (nl > h->cap_mem || !h->slots[0].pm) {
+        const int c = std::max(std::max(nl, 1), h->cap_mem * 5 / 4);
+        const int gbytes = sz_internal_imexop_member_bytes();
+        for (auto &sl : h->slots) {
+            h->realloc_pool(sl.gmem, (size_t)gbytes * c);
+            h->realloc_pool(sl.gpack, (size_t)gbytes * c);
+            h->realloc_pool(sl.pm, (size_t)c);
+        }
+        h->cap_mem = c;
+    }
+    if (!h->counts) h->realloc_pool(h->counts, 3);
+    h->pool_bytes = (long long)h->cap_sys * (sizeof(pdesc) + sizeof(psys) + 64) +
+                    h->cap_vec * (3 * sizeof(zd) + sizeof(zf)) +
+                    h->cap_frob * (long long)sizeof(double);
+    if (!D.empty())
+        (void)hipMemcpy(h->D, D.data(), D.size() * sizeof(pdesc),
+                        hipMemcpyHostToDevice);
+    // grouped factorization / solve descriptors per slot and band shape
+    for (auto &c : h->classes) {
+        c.gs.assign(h->slots.size(), nullptr);
+        for (size_t k = 0; k < h->slots.size(); ++k) {
+            std::vector<sz::gsys> G;
+            G.reserve((size_t)c.ns);
+            for (int i = 0; i < nl; ++i) {
+                const sz_prec::mstore &m =
+                    h->ms[(size_t)h->live[(size_t)h->order[(size_t)i]]];
+                if (m.g.KL != c.KL || m.g.KU != c.KU) continue;
+                const int N = 5 * m.g.n;
+                const sz_prec::slotdata &sd = m.sl[k];
+                for (int l = 0; l < m.g.nsys; ++l) {
+                    sz::gsys g{};
+                    g.n = N;
+                    g.ab = (unsigned long long)(sd.lu + (size_t)l * c.ldlu * N);
+                    g.ipiv = (unsigned long long)(sd.ipiv + (size_t)l * N);
+                    g.b = (unsigned long long)(h->PB + h->v0[(size_t)i] +
+                                               (long long)l * N);
+                    G.push_back(g);
+                }
+            }
+            c.gs[k] = static_cast<sz::gsys *>(
+                sz_prec::raw_alloc(std::max<size_t>(G.size(), 1) * sizeof(sz::gsys)));
+            if (!G.empty())
+                (void)hipMemcpy(c.gs[k], G.data(), G.size() * sizeof(sz::gsys),
+                                hipMemcpyHostToDevice);
+        }
+    }
+    for (auto &sl : h->slots) sl.last.clear();
+    h->dirty = false;
+}
+
+}  // namespace
+
+int sz_prec_add(sz_prec *h, const sz_prec_member &g)
+{
+    if (!h) return -1;
+    if (g.n < 2 || g.nsys < 0 || g.KL < 0 || g.KU < 0) return -2;
+    sz_prec::mstore m;
+    m.g = g;
+    m.alive = true;
+    try {
+        const size_t N = 5 * (size_t)g.n, ns = (size_t)g.nsys;
+        const size_t ldlu = 2 * (size_t)g.KL + g.KU + 1;
+        auto al = [&](size_t bytes) {
+            void *q = sz_prec::raw_alloc(bytes);
+            m.owned.push_back(q);
+            m.bytes += (long long)bytes;
+            return q;
+        };
+        for (int k = 0; k < h->p.nslots; ++k) {
+            sz_prec::slotdata sd;
+            sd.lu = static_cast<zf *>(al(ns * N * ldlu * sizeof(zf)));
+            sd.ipiv = static_cast<int *>(al(ns * N * sizeof(int)));
+            sd.fact = static_cast<char *>(al(ns));
+            sd.apprx = static_cast<int *>(al(ns * sizeof(int)));
+            sd.afrob = static_cast<double *>(al(ns * sizeof(double)));
+            sd.bcs = static_cast<zd *>(al(ns * 8 * sizeof(zd)));
+            if (ns) {
+                hipLaunchKernelGGL(k_p_fill_char, dim3(nb((long long)ns)), dim3(PT), 0,
+                                   h->stream, sd.fact, 'N', (int)ns);
+                hipLaunchKernelGGL(k_p_fill_int, dim3(nb((long long)ns)), dim3(PT), 0,
+                                   h->stream, sd.apprx, 0, (int)ns);
+            }
+            m.sl.push_back(sd);
+        }
+    } catch (...) {
+        for (void *q : m.owned) (void)hipFree(q);
+        return SZ_ERR_ALLOC;
+    }
+    h->ms.push_back(std::move(m));
+    const int id = (int)h->ms.size() - 1;
+    h->live.push_back(id);
+    h->dirty = true;
+    return id;
+}
+
+int sz_prec_remove(sz_prec *h, int id)
+{
+    if (!h || id < 0 || id >= (int)h->ms.size() || !h->ms[(size_t)id].alive)
+        return -1;
+    (void)hipStreamSynchronize(h->stream);
+    sz_prec::mstore &m = h->ms[(size_t)id];
+    for (void *q : m.owned) (void)hipFree(q);
+    m.owned.clear();
+    m.sl.clear();
+    m.bytes = 0;
+    m.alive = false;
+    h->live.erase(std::find(h->live.begin(), h->live.end(), id));
+    h->dirty = true;
+    return 0;
+}
+
+int sz_prec_live(const sz_prec *h) { return h ? (int)h->live.size() : 0; }
+
+long long sz_prec_member_bytes(const sz_prec *h, const sz_prec_member &g)
+{
+    const long long N = 5LL * g.n, ldlu = 2LL * g.KL + g.KU + 1;
+    const long long per = N * ldlu * 8 + N * 4 + 1 + 4 + 8 + 8 * 16;
+    return (long long)(h ? h->p.nslots : 1) * g.nsys * per;
+}
+
+sz_prec *sz_prec_create(const std::vector<sz_prec_member> &members,
+                        const sz_prec_params &p, hipStream_t stream)
+{
+    if (p.nslots < 1) return nullptr;
     sz_prec *h = new sz_prec();
     try {
         h->p = p;
         h->stream = stream;
-        h->N = 5 * p.n;
-        h->ldlu = 2 * p.KL + p.KU + 1;
-        const size_t ns = (size_t)p.nsys, N = (size_t)h->N;
         h->slots.resize((size_t)p.nslots);
-        for (auto &sl : h->slots) {
-            sl.lu = h->dalloc<zf>(ns * N * (size_t)h->ldlu);
-            sl.ipiv = h->dalloc<int>(ns * N);
-            sl.fact = h->dalloc<char>(ns);
-            sl.apprx = h->dalloc<int>(ns);
-            sl.afrob = h->dalloc<double>(ns);
-            sl.bcs = h->dalloc<zd>(ns * 8);
-            hipLaunchKernelGGL(k_p_fill_char, dim3(nb(p.nsys)), dim3(PT), 0, stream,
-                               sl.fact, 'N', p.nsys);
-            hipLaunchKernelGGL(k_p_fill_int, dim3(nb(p.nsys)), dim3(PT), 0, stream,
-                               sl.apprx, 0, p.nsys);
-        }
-        h->B = h->dalloc<zd>(ns * N);
-        h->X = h->dalloc<zd>(ns * N);
-        h->T = h->dalloc<zd>(ns * N);
-        h->PB = h->dalloc<zf>(ns * N);
-        h->S = h->dalloc<psys>(ns);
-        h->skip = h->dalloc<int>(ns);
-        h->fskip = h->dalloc<int>(ns);
-        h->finfo = h->dalloc<int>(ns);
-        h->frob = h->dalloc<double>(ns * (size_t)p.nchunks);
-        h->counts = h->dalloc<int>(3);
-        h->list = h->dalloc<int>(ns);
-        h->skm = h->dalloc<double>(ns);
-        h->skn = h->dalloc<double>(ns);
-        h->o_siter = h->dalloc<int>(ns);
-        h->o_diter = h->dalloc<int>(ns);
-        h->o_info = h->dalloc<int>(ns);
-        h->o_first = h->dalloc<int>(ns);
-        h->o_tolsc = h->dalloc<double>(ns);
-        h->o_res = h->dalloc<double>(ns);
-        hipLaunchKernelGGL(k_p_fill_int, dim3(nb(p.nsys)), dim3(PT), 0, stream,
-                           h->o_first, 0, p.nsys);
+        for (const auto &m : members)
+            if (sz_prec_add(h, m) < 0) throw std::bad_alloc();
         if (hipHostMalloc(&h->hcounts, 3 * sizeof(int)) != hipSuccess)
             throw std::bad_alloc();
+        rebuild(h);
         if (!ok() || hipStreamSynchronize(stream) != hipSuccess)
             throw std::runtime_error("setup");
     } catch (...) {
@@ -549,15 +823,52 @@ sz_prec *sz_prec_create(const sz_prec_params &p, hipStream_t stream)
 
 void sz_prec_free(sz_prec *h) { delete h; }
 
+int sz_prec_nsys(const sz_prec *h)
+{
+    if (!h) return 0;
+    int n = 0;
+    for (int id : h->live) n += h->ms[(size_t)id].g.nsys;
+    return n;
+}
+
+int sz_prec_sys0(const sz_prec *h, int k)
+{
+    if (!h || k < 0 || k >= (int)h->live.size()) return -1;
+    const_cast<sz_prec *>(h)->dirty ? rebuild(const_cast<sz_prec *>(h)) : (void)0;
+    for (size_t i = 0; i < h->order.size(); ++i)
+        if (h->order[i] == k) return h->sys0[i];
+    return -1;
+}
+
+void sz_prec_set_lu_bs(sz_prec *h, int bs)
+{
+    if (h) h->p.lu_bs = bs;
+}
+
+void sz_prec_set_graph(sz_prec *h, int graph)
+{
+    if (h) h->p.graph = graph ? 1 : 0;
+}
+
 void sz_prec_invalidate(sz_prec *h)
 {
     if (!h) return;
-    for (auto &sl : h->slots)
-        hipLaunchKernelGGL(k_p_fill_char, dim3(nb(h->p.nsys)), dim3(PT), 0, h->stream,
-                           sl.fact, 'N', h->p.nsys);
+    for (int id : h->live) {
+        const auto &m = h->ms[(size_t)id];
+        if (m.g.nsys == 0) continue;
+        for (const auto &sd : m.sl)
+            hipLaunchKernelGGL(k_p_fill_char, dim3(nb(m.g.nsys)), dim3(PT), 0,
+                               h->stream, sd.fact, 'N', m.g.nsys);
+    }
 }
 
-long long sz_prec_bytes(const sz_prec *h) { return h ? h->bytes : 0; }
+long long sz_prec_bytes(const sz_prec *h)
+{
+    if (!h) return 0;
+    long long b = h->pool_bytes;
+    for (const auto &m : h->ms) b += m.bytes;
+    return b;
+}
 
 void sz_prec_counts(const sz_prec *h, int *factored, int *restarted,
                     int *fallback)
@@ -575,8 +886,8 @@ void *sz_prec_stats(sz_prec *h, int which)
     case 1: return h->o_diter;
     case 2: return h->o_tolsc;
     case 3: return h->o_res;
-    case 4: return h->slots[(size_t)h->last_slot].fact;
-    case 5: return h->slots[(size_t)h->last_slot].apprx;
+    case 4: return h->o_fact;
+    case 5: return h->o_apprx;
     case 6: return h->o_info;
     }
     return nullptr;
@@ -584,8 +895,10 @@ void *sz_prec_stats(sz_prec *h, int which)
 
 int sz_prec_first_info(sz_prec *h, bool clear)
 {
-    if (!h || h->p.nsys == 0) return 0;
-    std::vector<int> v((size_t)h->p.nsys);
+    if (!h) return 0;
+    rebuild(h);
+    if (h->nsys == 0) return 0;
+    std::vector<int> v((size_t)h->nsys);
     if (hipMemcpyAsync(v.data(), h->o_first, sizeof(int) * v.size(),
                        hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
         hipStreamSynchronize(h->stream) != hipSuccess)
@@ -593,57 +906,115 @@ int sz_prec_first_info(sz_prec *h, bool clear)
     int first = 0;
     for (int x : v) if (!first) first = x;
     if (clear && first)
-        hipLaunchKernelGGL(k_p_fill_int, dim3(nb(h->p.nsys)), dim3(PT), 0, h->stream,
-                           h->o_first, 0, h->p.nsys);
+        hipLaunchKernelGGL(k_p_fill_int, dim3(nb(h->nsys)), dim3(PT), 0, h->stream,
+                           h->o_first, 0, h->nsys);
     return first;
+}
+
+int sz_prec_prepare(sz_prec *h, int slot, const sz_prec_problem *pr)
+{
+    if (!h || !pr) return -1;
+    if (slot < 0 || slot >= h->p.nslots) return -2;
+    try {
+        rebuild(h);
+    } catch (...) {
+        return SZ_ERR_ALLOC;
+    }
+    sz_prec::slot &sl = h->slots[(size_t)slot];
+    const int nl = (int)h->live.size();
+    const int gbytes = sz_internal_imexop_member_bytes();
+    // [apply descriptors | pack descriptors | pmem]
+    const size_t gsz = (size_t)gbytes * nl;
+    std::vector<unsigned char> buf(2 * gsz + sizeof(pmem) * nl);
+    int blk = 0, pblk = 0;
+    for (int i = 0; i < nl; ++i) {
+        const int k = h->order[(size_t)i];
+        const sz_prec::mstore &m = h->ms[(size_t)h->live[(size_t)k]];
+        const sz_prec::slotdata &sd = m.sl[(size_t)slot];
+        const sz_prec_problem &q = pr[k];
+        const double phi2[2] = {q.phi, 0.0};
+        const int ldlu = 2 * m.g.KL + m.g.KU + 1;
+        int rc = sz_internal_imexop_member(
+            q.op, phi2, q.km, q.kn, q.ref, q.ld, q.bc, blk, h->sys0[(size_t)i],
+            m.g.nsys, m.g.KL, m.g.KU, ldlu, sd.lu, sd.bcs, h->v0[(size_t)i],
+            h->frob0[(size_t)i], buf.data() + (size_t)gbytes * i);
+        if (rc) return rc;
+        rc = sz_internal_imexop_member(
+            q.op, phi2, q.km, q.kn, q.ref, q.ld, q.bc, pblk, h->sys0[(size_t)i],
+            m.g.nsys, m.g.KL, m.g.KU, ldlu, sd.lu, sd.bcs, h->v0[(size_t)i],
+            h->frob0[(size_t)i], buf.data() + gsz + (size_t)gbytes * i);
+        if (rc) return rc;
+        blk += m.g.nsys;                          // one accumulate block a system
+        pblk += m.g.nsys * h->nchunks[(size_t)i]; // nchunks pack blocks a system
+        pmem pm{};
+        pm.state = q.state;
+        pm.F = q.F;
+        pm.sysoff = q.sysoff;
+        pm.sys0 = h->sys0[(size_t)i];
+        pm.nchunks = h->nchunks[(size_t)i];
+        pm.frob0 = h->frob0[(size_t)i];
+        double fac[8];
+        sz_internal_imexop_wall_factors(q.op, q.bc, fac);
+        pm.wc.enforce[0] = q.bc->enforce_lower;
+        pm.wc.enforce[1] = q.bc->enforce_upper;
+        for (int w = 0; w < 2; ++w)
+            for (int t = 0; t < 4; ++t) pm.wc.fac[w][t] = fac[w * 4 + t];
+        pm.fact = sd.fact;
+        pm.apprx = sd.apprx;
+        pm.afrob = sd.afrob;
+        pm.bcs = sd.bcs;
+        std::memcpy(buf.data() + 2 * gsz + sizeof(pmem) * i, &pm, sizeof(pm));
+    }
+    if (buf == sl.last) return 0;
+    // the previous contents may still be read by enqueued work
+    if (hipStreamSynchronize(h->stream) != hipSuccess ||
+        (gsz && hipMemcpy(sl.gmem, buf.data(), gsz, hipMemcpyHostToDevice) !=
+                    hipSuccess) ||
+        (gsz && hipMemcpy(sl.gpack, buf.data() + gsz, gsz, hipMemcpyHostToDevice) !=
+                    hipSuccess) ||
+        (nl && hipMemcpy(sl.pm, buf.data() + 2 * gsz, sizeof(pmem) * nl,
+                         hipMemcpyHostToDevice) != hipSuccess))
+        return SZ_ERR_LAUNCH;
+    sl.last.swap(buf);
+    return 0;
 }
 
 namespace {
 
-// One single precision attempt (pass 0) or restart pass (pass 1): init,
-// masked pack + factor of the systems that need factors, then the sweeps.
-int single_pass(sz_prec *h, sz_prec::slot &sl, const sz_prec_problem &pr,
-                const p_args &A0, int pass)
+int single_pass(sz_prec *h, sz_prec::slot &sl, int slot, const p_args &A0, int pass)
 {
-    const sz_prec_params &p = h->p;
     hipStream_t st = h->stream;
     p_args A = A0;
     A.pass = pass;
-    hipLaunchKernelGGL(k_p_init, dim3(p.nsys), dim3(PT), 0, st, A);
+    const int nm = (int)h->live.size();
+    hipLaunchKernelGGL(k_p_init, dim3(h->nsys), dim3(PT), 0, st, A);
     if (!ok()) return SZ_ERR_LAUNCH;
-    const double phi2[2] = {pr.phi, 0.0};
-    int rc = sz_internal_imexop_pack32(pr.op, phi2, pr.km, pr.kn, pr.ref, pr.ld,
-                                       sl.lu, h->ldlu, (long long)h->ldlu * h->N,
-                                       p.nsys, pr.bc, h->fskip, sl.bcs, h->frob, st);
+    int rc = sz_internal_imexop_pack_grouped(sl.gpack, nm, h->nblocks_pack, h->ldmax,
+                                             h->fskip, h->frob, st);
     if (rc) return rc;
-    rc = sz_internal_gbtrf_masked(1, h->N, p.KL, p.KU, sl.lu, h->ldlu,
-                                  (long long)h->ldlu * h->N, sl.ipiv, h->N,
-                                  h->finfo, h->fskip, p.nsys, st);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_p_afterfactor, dim3(nb(p.nsys)), dim3(PT), 0, st, A,
-                       p.nsys);
-    if (!ok()) return SZ_ERR_LAUNCH;
-    const long long N = h->N;
-    const void *in[5];
-    void *out[5];
-    for (int f = 0; f < 5; ++f) {
-        in[f] = h->X + (size_t)f * p.n;
-        out[f] = h->T + (size_t)f * p.n;
+    for (const auto &c : h->classes) {
+        rc = sz_internal_gbtrf_grouped(1, c.KL, c.KU, nullptr, c.ldlu, nullptr,
+                                       h->finfo + c.s0, h->fskip + c.s0,
+                                       c.gs[(size_t)slot], c.ns, h->p.lu_bs, st);
+        if (rc) return rc;
     }
-    const double zero[2] = {0, 0};
-    for (int it = 0; it <= p.smax; ++it) {
-        rc = sz_internal_gbtrs_masked(1, 'T', h->N, p.KL, p.KU, 1, sl.lu, h->ldlu,
-                                      (long long)h->ldlu * h->N, sl.ipiv, h->N,
-                                      h->PB, h->N, N, h->skip, p.nsys, st);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_p_update, dim3(nb(N * p.nsys)), dim3(PT), 0, st, A,
-                           p.nsys);
+    hipLaunchKernelGGL(k_p_afterfactor, dim3(nb(h->nsys)), dim3(PT), 0, st, A,
+                       h->nsys);
+    if (!ok()) return SZ_ERR_LAUNCH;
+    const size_t lds_post = (size_t)5 * h->nmax * sizeof(zd);
+    for (int it = 0; it <= h->p.smax; ++it) {
+        for (const auto &c : h->classes) {
+            rc = sz_internal_gbtrs_grouped(1, 'T', c.KL, c.KU, nullptr, c.ldlu,
+                                           nullptr, nullptr, h->skip + c.s0,
+                                           c.gs[(size_t)slot], c.ns, c.nmax, st);
+            if (rc) return rc;
+        }
+        hipLaunchKernelGGL(k_p_update, dim3(h->nsys), dim3(PT), 0, st, A);
         if (!ok()) return SZ_ERR_LAUNCH;
-        rc = sz_internal_imexop_accumulate_masked(pr.op, phi2, pr.km, pr.kn, pr.ref,
-                                                  pr.ld, in, nullptr, N, zero, out,
-                                                  nullptr, N, p.nsys, h->skip, st);
+        rc = sz_internal_imexop_apply_grouped(sl.gmem, nm, h->nsys, h->nmax, h->X, h->T,
+                                              h->skip, st);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_p_post, dim3(p.nsys), dim3(PT), N * sizeof(zd), st, A);
+        hipLaunchKernelGGL(k_p_post, dim3(h->nsys), dim3(PT), lds_post, st, A);
         if (!ok()) return SZ_ERR_LAUNCH;
     }
     return 0;
@@ -652,7 +1023,7 @@ int single_pass(sz_prec *h, sz_prec::slot &sl, const sz_prec_problem &pr,
 int read_counts(sz_prec *h)
 {
     hipLaunchKernelGGL(k_p_count, dim3(1), dim3(PT), 0, h->stream, h->S, h->fskip,
-                       h->p.nsys, h->counts);
+                       h->nsys, h->counts);
     if (hipMemcpyAsync(h->hcounts, h->counts, 3 * sizeof(int),
                        hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
         hipStreamSynchronize(h->stream) != hipSuccess)
@@ -661,79 +1032,112 @@ int read_counts(sz_prec *h)
 }
 
 // Systems still above tolerance after the single precision attempt: double
-// precision factorization and solve in chunks of h->cap.
-int double_fallback(sz_prec *h, const sz_prec_problem &pr, int nfail)
+// precision factorization and solve, member by member in chunks.
+int double_fallback(sz_prec *h, const sz_prec_problem *pr)
 {
-    const sz_prec_params &p = h->p;
     hipStream_t st = h->stream;
-    const int KL = p.KL, KU = p.KU, N = h->N, ldlu64 = KL + (KL + 1 + KU);
-    if (!h->lu64) {
-        h->cap = p.nsys < 512 ? p.nsys : 512;
-        h->lu64 = h->dalloc<zd>((size_t)h->cap * N * ldlu64);
-        h->pb64 = h->dalloc<zd>((size_t)h->cap * N);
-        h->ipiv64 = h->dalloc<int>((size_t)h->cap * N);
-        h->info64 = h->dalloc<int>((size_t)h->cap);
-    }
-    hipLaunchKernelGGL(k_p_compact, dim3(1), dim3(PT), 0, st, h->S, p.nsys, pr.km,
-                       pr.kn, h->list, h->skm, h->skn);
-    if (!ok()) return SZ_ERR_LAUNCH;
-    const double phi2[2] = {pr.phi, 0.0};
-    for (int j0 = 0; j0 < nfail; j0 += h->cap) {
-        const int m = nfail - j0 < h->cap ? nfail - j0 : h->cap;
-        int rc = sz_rholut_imexop_packc_batched(
-            pr.op, phi2, h->skm + j0, h->skn + j0, pr.ref, pr.ld, 0, 1, 2, 3, 4,
-            nullptr, h->lu64 + KL, ldlu64, (long long)ldlu64 * N, m, pr.bc, st);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_p_sub_gather, dim3(nb((long long)N * m)), dim3(PT), 0, st,
-                           h->B, h->list, j0, m, p.n, h->pb64);
-        if (!ok()) return SZ_ERR_LAUNCH;
-        rc = sz_internal_zgbtrf_ut(N, KL, KU, h->lu64, ldlu64, (long long)ldlu64 * N,
-                                   h->ipiv64, N, h->info64, h->pb64, N, m, st);
-        if (rc == 1) {
-            rc = sz_internal_gbtrf_masked(0, N, KL, KU, h->lu64, ldlu64,
-                                          (long long)ldlu64 * N, h->ipiv64, N,
-                                          h->info64, nullptr, m, st);
-            if (rc) return rc;
-            rc = sz_internal_gbtrs_masked(0, 'T', N, KL, KU, 1, h->lu64, ldlu64,
-                                          (long long)ldlu64 * N, h->ipiv64, N,
-                                          h->pb64, N, N, h->info64, m, st);
-        } else if (rc == 0) {
-            rc = sz_internal_zgbtrs_lt(N, KL, KU, h->lu64, ldlu64,
-                                       (long long)ldlu64 * N, h->ipiv64, N,
-                                       h->pb64, N, h->info64, m, st);
+    const int nl = (int)h->live.size();
+    for (int i = 0; i < nl; ++i) {
+        const int k = h->order[(size_t)i];
+        const sz_prec_member &m = h->ms[(size_t)h->live[(size_t)k]].g;
+        const sz_prec_problem &q = pr[k];
+        const int s0 = h->sys0[(size_t)i];
+        if (m.nsys == 0) continue;
+        // failures of this member
+        hipLaunchKernelGGL(k_p_count, dim3(1), dim3(PT), 0, st, h->S + s0,
+                           h->fskip + s0, m.nsys, h->counts);
+        if (hipMemcpyAsync(h->hcounts, h->counts, 3 * sizeof(int),
+                           hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return SZ_ERR_LAUNCH;
+        const int nfail = h->hcounts[1];
+        if (nfail == 0) continue;
+        const int KL = m.KL, KU = m.KU, N = 5 * m.n, ldlu64 = 2 * KL + KU + 1;
+        const long long per = (long long)N * ldlu64;
+        const long long want = std::min<long long>(nfail, 512) * per;
+        if (want > h->cap_elems || (long long)std::min(nfail, 512) * N >
+                                       (long long)h->cap * 5 * 256) {
+            if (h->lu64) (void)hipFree(h->lu64);
+            if (h->pb64) (void)hipFree(h->pb64);
+            if (h->ipiv64) (void)hipFree(h->ipiv64);
+            if (h->info64) (void)hipFree(h->info64);
+            h->lu64 = h->pb64 = nullptr;
+            h->ipiv64 = h->info64 = nullptr;
+            const int c = std::min(nfail, 512);
+            try {
+                h->lu64 = static_cast<zd *>(sz_prec::raw_alloc((size_t)c * per * sizeof(zd)));
+                h->pb64 = static_cast<zd *>(sz_prec::raw_alloc((size_t)c * 5 * 256 * sizeof(zd)));
+                h->ipiv64 = static_cast<int *>(sz_prec::raw_alloc((size_t)c * 5 * 256 * sizeof(int)));
+                h->info64 = static_cast<int *>(sz_prec::raw_alloc((size_t)c * sizeof(int)));
+            } catch (...) {
+                return SZ_ERR_ALLOC;
+            }
+            h->cap_elems = (long long)c * per;
+            h->cap = c;
         }
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_p_sub_scatter, dim3(nb((long long)N * m)), dim3(PT), 0,
-                           st, h->pb64, h->list, h->info64, j0, m, p.n, h->X, h->S);
+        const int cap = (int)std::min<long long>(h->cap, h->cap_elems / per);
+        hipLaunchKernelGGL(k_p_compact, dim3(1), dim3(PT), 0, st, h->S, s0, m.nsys,
+                           q.km, q.kn, h->list, h->skm, h->skn);
         if (!ok()) return SZ_ERR_LAUNCH;
+        const double phi2[2] = {q.phi, 0.0};
+        for (int j0 = 0; j0 < nfail; j0 += cap) {
+            const int c = std::min(cap, nfail - j0);
+            int rc = sz_rholut_imexop_packc_batched(
+                q.op, phi2, h->skm + j0, h->skn + j0, q.ref, q.ld, 0, 1, 2, 3, 4,
+                nullptr, h->lu64 + KL, ldlu64, per, c, q.bc, st);
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_p_sub_gather, dim3(nb((long long)N * c)), dim3(PT), 0,
+                               st, h->B, h->v0[(size_t)i], h->list, j0, c, m.n,
+                               h->pb64);
+            if (!ok()) return SZ_ERR_LAUNCH;
+            rc = sz_internal_zgbtrf_ut(N, KL, KU, h->lu64, ldlu64, per, h->ipiv64, N,
+                                       h->info64, h->pb64, N, c, st);
+            if (rc == 1) {
+                rc = sz_internal_gbtrf_masked(0, N, KL, KU, h->lu64, ldlu64, per,
+                                              h->ipiv64, N, h->info64, nullptr, c, st);
+                if (rc) return rc;
+                rc = sz_internal_gbtrs_masked(0, 'T', N, KL, KU, 1, h->lu64, ldlu64, per,
+                                              h->ipiv64, N, h->pb64, N, N, h->info64,
+                                              c, st);
+            } else if (rc == 0) {
+                rc = sz_internal_zgbtrs_lt(N, KL, KU, h->lu64, ldlu64, per, h->ipiv64,
+                                           N, h->pb64, N, h->info64, c, st);
+            }
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_p_sub_scatter, dim3(nb((long long)N * c)), dim3(PT), 0,
+                               st, h->pb64, h->v0[(size_t)i], s0, h->list, h->info64,
+                               j0, c, m.n, h->X, h->S);
+            if (!ok()) return SZ_ERR_LAUNCH;
+        }
+        h->n_fallback += nfail;
     }
     return 0;
 }
 
 }  // namespace
 
-int sz_prec_solve(sz_prec *h, int slot, const sz_prec_problem &pr)
+int sz_prec_solve(sz_prec *h, int slot, const sz_prec_problem *pr)
 {
-    if (!h) return -1;
+    if (!h || !pr) return -1;
     if (slot < 0 || slot >= h->p.nslots) return -2;
-    const sz_prec_params &p = h->p;
     hipStream_t st = h->stream;
-    if (p.nsys == 0) return 0;
     sz_prec::slot &sl = h->slots[(size_t)slot];
     h->last_slot = slot;
-    if (!p.reuse) sz_prec_invalidate(h);
-    // 1. right hand sides with the wall rows zeroed, x = 0
-    const long long N = h->N;
-    hipLaunchKernelGGL(k_p_gather, dim3(nb(N * p.nsys)), dim3(PT), 0, st, pr.state,
-                       pr.F, pr.sysoff, p.n, pr.bc->enforce_lower,
-                       pr.bc->enforce_upper, h->B, h->X, p.nsys);
-    if (!ok()) return SZ_ERR_LAUNCH;
+    int rc = 0;
+    if (!h->p.graph) {
+        rc = sz_prec_prepare(h, slot, pr);
+        if (rc) return rc;
+    } else if (h->dirty) {
+        return -3;                        // graph mode: prepare first
+    }
+    if (h->nsys == 0) return 0;
+    if (!h->p.reuse) sz_prec_invalidate(h);
     p_args A{};
-    A.n = p.n;
-    A.N = h->N;
-    A.smax = p.smax;
-    A.aiter = p.aiter;
-    A.tolsc = p.tolsc;
+    A.smax = h->p.smax;
+    A.aiter = h->p.aiter;
+    A.tolsc = h->p.tolsc;
+    A.D = h->D;
+    A.PM = sl.pm;
     A.B = h->B;
     A.X = h->X;
     A.T = h->T;
@@ -741,45 +1145,44 @@ int sz_prec_solve(sz_prec *h, int slot, const sz_prec_problem &pr)
     A.S = h->S;
     A.skip = h->skip;
     A.fskip = h->fskip;
-    A.fact = sl.fact;
-    A.apprx = sl.apprx;
-    A.afrob = sl.afrob;
     A.frob = h->frob;
-    A.nchunks = p.nchunks;
     A.finfo = h->finfo;
-    A.bcs = sl.bcs;
-    A.wc.enforce[0] = pr.bc->enforce_lower;
-    A.wc.enforce[1] = pr.bc->enforce_upper;
-    int rc = sz_internal_imexop_wall_factors(pr.op, pr.bc, &A.wc.fac[0][0]);
-    if (rc) return rc;
+    // 1. right hand sides with the wall rows zeroed, x = 0
+    hipLaunchKernelGGL(k_p_gather, dim3(h->nsys), dim3(PT), 0, st, A);
+    if (!ok()) return SZ_ERR_LAUNCH;
     // 2. single precision attempt with the slot's factors (refactoring the
     //    systems without any)
     h->n_factored = h->n_restarted = h->n_fallback = 0;
-    rc = single_pass(h, sl, pr, A, 0);
+    rc = single_pass(h, sl, slot, A, 0);
     if (rc) return rc;
-    rc = read_counts(h);
-    if (rc) return rc;
-    h->n_factored = h->hcounts[2];
-    // 3. restart the systems whose reused factors stalled
-    if (h->hcounts[0] > 0) {
-        h->n_restarted = h->hcounts[0];
-        rc = single_pass(h, sl, pr, A, 1);
+    if (h->p.graph) {
+        // 3'. restart pass always enqueued (masked: nothing to do when no
+        //     reused factors stalled); no double precision fallback
+        rc = single_pass(h, sl, slot, A, 1);
         if (rc) return rc;
+    } else {
         rc = read_counts(h);
         if (rc) return rc;
-        h->n_factored += h->hcounts[2];
+        h->n_factored = h->hcounts[2];
+        // 3. restart the systems whose reused factors stalled
+        if (h->hcounts[0] > 0) {
+            h->n_restarted = h->hcounts[0];
+            rc = single_pass(h, sl, slot, A, 1);
+            if (rc) return rc;
+            rc = read_counts(h);
+            if (rc) return rc;
+            h->n_factored += h->hcounts[2];
+        }
+        // 4. double precision for the rest
+        if (h->hcounts[1] > 0 && h->p.dmax >= 0) {
+            rc = double_fallback(h, pr);
+            if (rc) return rc;
+        }
     }
-    // 4. double precision for the rest
-    if (h->hcounts[1] > 0 && p.dmax >= 0) {
-        h->n_fallback = h->hcounts[1];
-        rc = double_fallback(h, pr, h->hcounts[1]);
-        if (rc) return rc;
-    }
-    // 5. solutions into the state, outcome arrays
-    hipLaunchKernelGGL(k_p_scatter, dim3(nb(N * p.nsys)), dim3(PT), 0, st, h->X, p.n,
-                       pr.state, pr.F, pr.sysoff, p.nsys);
-    hipLaunchKernelGGL(k_p_stats, dim3(nb(p.nsys)), dim3(PT), 0, st, h->S, p.nsys,
-                       h->o_siter, h->o_diter, h->o_tolsc, h->o_res, h->o_info,
-                       h->o_first);
+    // 5. solutions into the states, outcome arrays
+    hipLaunchKernelGGL(k_p_scatter, dim3(h->nsys), dim3(PT), 0, st, A);
+    hipLaunchKernelGGL(k_p_stats, dim3(nb(h->nsys)), dim3(PT), 0, st, A, h->S,
+                       h->nsys, h->o_siter, h->o_diter, h->o_tolsc, h->o_res,
+                       h->o_info, h->o_first, h->o_fact, h->o_apprx);
     return ok() ? 0 : SZ_ERR_LAUNCH;
 }

@@ -11,9 +11,12 @@
 //   precision attempt; systems still above tolerance get a double precision
 //   factorization and solve.  tol = eps (tolsc = 0) or
 //   ||x||_2 ||A||_F eps sqrt(N) tolsc (Langou et al.).
-// Factor storage lives in slots: a slot's factors stay valid across calls and
-// are reused as approximate factorizations (apprx) of the next operator the
-// slot is asked to invert.
+// One solver serves one or more operators ("members": the channels of an
+// ensemble, of different sizes and band shapes); every stage is one launch
+// over all members' systems (one per band shape for the factorization and
+// the float solve).  Factor storage lives in slots: a slot's factors stay
+// valid across calls and are reused as approximate factorizations (apprx)
+// of the next operators the slot is asked to invert.
 #pragma once
 #include "sz_common.hpp"
 #include "../../include/suzerain_amd.h"
@@ -22,19 +25,25 @@
 
 struct sz_prec;
 
+// One member's systems: n B-spline coefficients per field (N = 5n), the
+// BSMBSM band (KL, KU) of P A^T P^T and the number of (kx, kz) systems.
+struct sz_prec_member {
+    int n, KL, KU, nsys;
+};
+
 struct sz_prec_params {
-    int nsys, n;          // systems, B-spline coefficients per field
-    int KL, KU;           // BSMBSM band of P A^T P^T (S = 5)
     int nslots;           // factorization slots kept
     int smax, dmax, aiter;
     double tolsc;
     int reuse;            // keep factors between calls (apprx semantics)
-    int nchunks;          // sz_internal_imexop_pack_chunks of the operator
+    int graph;            // no host waits: restart pass always enqueued,
+                          // no double fallback (capturable in a hipGraph)
+    int lu_bs;            // factorization unroll (4 or 8; 0 = default)
 };
 
-// The operator and right hand side of one call: the channel's imexop handle,
-// wavenumbers, references and walls, and the state (5 fields at stride F,
-// system s's pencils at + sysoff[s]) solved in place.
+// One member's operator and right hand side for one call: the imexop
+// handle, wavenumbers, references and walls, and the state (5 fields at
+// stride F, system s's pencils at + sysoff[s]) solved in place.
 struct sz_prec_problem {
     const sz_imexop *op;
     const sz_imexop_ref *ref;
@@ -47,22 +56,45 @@ struct sz_prec_problem {
     const long long *sysoff;
 };
 
-sz_prec *sz_prec_create(const sz_prec_params &p, hipStream_t stream);
+sz_prec *sz_prec_create(const std::vector<sz_prec_member> &members,
+                        const sz_prec_params &p, hipStream_t stream);
 void sz_prec_free(sz_prec *h);
-// Solve in place with slot `slot`; returns 0 or an error code.  Waits once
-// for the device (the count of systems needing a restart or the double
-// precision fallback).
-int sz_prec_solve(sz_prec *h, int slot, const sz_prec_problem &pr);
+// Members join and leave (serving): add returns the new member's id (ids
+// are never reused; the live members are solved in id order) or a negative
+// error; its factor storage stays resident until it is removed.  Either
+// makes earlier prepared descriptors stale.
+int sz_prec_add(sz_prec *h, const sz_prec_member &m);
+int sz_prec_remove(sz_prec *h, int id);
+int sz_prec_live(const sz_prec *h);          // live members
+// Device bytes one member of this shape keeps resident (factor slots).
+long long sz_prec_member_bytes(const sz_prec *h, const sz_prec_member &m);
+// Solve every live member in place with slot `slot` (pr: one problem per
+// live member, in id order).  Returns 0 or an error code.  Unless the solver
+// is in graph mode it waits once for the device (the count of systems
+// needing a restart or the double precision fallback) and uploads the
+// members' descriptors when they changed.
+int sz_prec_solve(sz_prec *h, int slot, const sz_prec_problem *pr);
+// Upload the descriptors of slot `slot` for problems pr (graph mode: once,
+// before capture).
+int sz_prec_prepare(sz_prec *h, int slot, const sz_prec_problem *pr);
+// First system of live member `k` (k-th in id order) in the outcome arrays.
+int sz_prec_sys0(const sz_prec *h, int k);
 // Forget every slot's factors (the next call refactors every system).
 void sz_prec_invalidate(sz_prec *h);
-// Per-system outcome of the last solve on device memory: 0 siter, 1 diter,
-// 2 tolsc (res / tol), 3 res, 4 fact of the slot last used, 5 apprx of that
-// slot, 6 info.
+// Per-system outcome of the last solve on device memory (all members'
+// systems in order): 0 siter, 1 diter, 2 tolsc (res / tol), 3 res, 4 fact
+// of the slot last used, 5 apprx of that slot, 6 info.
 void *sz_prec_stats(sz_prec *h, int which);
 // First nonzero info recorded since the last clearing call (waits).
 int sz_prec_first_info(sz_prec *h, bool clear);
 // Device bytes held.
 long long sz_prec_bytes(const sz_prec *h);
-// Counters of the last solve: systems refactored, restarted, sent to double.
+// Counters of the last solve: systems refactored, restarted, sent to double
+// (graph mode: not counted).
 void sz_prec_counts(const sz_prec *h, int *factored, int *restarted,
                     int *fallback);
+// Total systems over all members.
+int sz_prec_nsys(const sz_prec *h);
+// Factorization unroll (autotuner) and graph mode switches.
+void sz_prec_set_lu_bs(sz_prec *h, int bs);
+void sz_prec_set_graph(sz_prec *h, int graph);
