@@ -54,5 +54,31 @@ def build(verbose=False):
     return LIB
 
 
+DROPIN_SRC = os.path.join(os.path.dirname(HERE), "tests", "dropin",
+                          "dropin_loop.cpp")
+DROPIN_BIN = os.path.join(os.path.dirname(HERE), "tests", "dropin",
+                          "dropin_loop")
+
+
+def build_dropin(verbose=False):
+    """Compile the reference-style C++ caller of the per-system solver
+    (tests/dropin/dropin_loop.cpp) with the host C++ compiler against
+    include/ and the library: the boundary as a C++ user links it."""
+    root = os.path.dirname(HERE)
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall",
+           "-I", os.path.join(root, "include"), "-I", "/opt/rocm/include",
+           "-D__HIP_PLATFORM_AMD__", DROPIN_SRC, "-L", HERE, "-lsuzerain_amd",
+           "-L/opt/rocm/lib", "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN/../../suzerain_amd", "-Wl,-rpath,/opt/rocm/lib",
+           "-o", DROPIN_BIN]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"g++ failed for {DROPIN_SRC}:\n{res.stderr}")
+    if verbose:
+        print(f"built {DROPIN_BIN}", file=sys.stderr)
+    return DROPIN_BIN
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_dropin(verbose=True)

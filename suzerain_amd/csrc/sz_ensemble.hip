@@ -56,7 +56,9 @@ struct sz_ensemble {
     std::vector<double> dt, chi;
     std::vector<sz_channel_spec> specs;
     long long scr[5] = {0, 0, 0, 0, 0};   // bytes of each scratch buffer
-    hipStream_t main = nullptr;
+    hipStream_t user = nullptr;           // the caller's stream
+    hipStream_t main = nullptr;           // own stream (capturable), owned
+    hipEvent_t sync_ev = nullptr;
     int maxstreams = 1, nstreams = 1;
     std::vector<hipStream_t> streams;     // explicit stage streams (owned)
     std::vector<hipEvent_t> ev;           // join events, one per stream
@@ -81,7 +83,9 @@ struct sz_ensemble {
         for (void *p : owned) (void)hipFree(p);
         for (hipEvent_t e : ev) hipEventDestroy(e);
         if (fork) hipEventDestroy(fork);
+        if (sync_ev) hipEventDestroy(sync_ev);
         for (hipStream_t s : streams) hipStreamDestroy(s);
+        if (main) hipStreamDestroy(main);
     }
 
     void *dalloc(size_t bytes_)
@@ -293,8 +297,14 @@ sz_ensemble *sz_ensemble_create(int nmembers, const sz_channel_spec *specs,
         return nullptr;
     }
     sz_ensemble *e = new sz_ensemble();
+    (void)hipGetLastError();                  // start from a clean error state
     try {
-        e->main = (hipStream_t)stream;
+        // steps run on a stream of the ensemble's own (the legacy default
+        // stream cannot be captured), ordered after and before the caller's
+        e->user = (hipStream_t)stream;
+        if (hipStreamCreate(&e->main) != hipSuccess ||
+            hipEventCreateWithFlags(&e->sync_ev, hipEventDisableTiming) != hipSuccess)
+            throw std::runtime_error("stream setup");
         e->maxstreams = std::max(1, std::min(opts->max_streams, 8));
         for (int k = 0; k < e->maxstreams; ++k) {
             hipStream_t s = nullptr;
@@ -427,10 +437,24 @@ long long sz_ensemble_bytes(const sz_ensemble *e)
     return e ? e->bytes + sz_prec_bytes(e->prec) : -1;
 }
 
+namespace {
+
+// order stream b after the work enqueued so far on stream a
+int order_after(hipEvent_t ev, hipStream_t a, hipStream_t b)
+{
+    if (hipEventRecord(ev, a) != hipSuccess || hipStreamWaitEvent(b, ev, 0) != hipSuccess)
+        return SZ_ERR_LAUNCH;
+    return 0;
+}
+
+}  // namespace
+
 int sz_ensemble_step(sz_ensemble *e, int nsteps)
 {
     if (!e) return -1;
     if (nsteps < 0) return -2;
+    int rc0 = order_after(e->sync_ev, e->user, e->main);
+    if (rc0) return rc0;
     for (int k = 0; k < nsteps; ++k) {
         int rc;
         if (e->graph) {
@@ -449,7 +473,7 @@ int sz_ensemble_step(sz_ensemble *e, int nsteps)
         }
         if (rc) return rc;
     }
-    return 0;
+    return order_after(e->sync_ev, e->main, e->user);
 }
 
 int sz_ensemble_check(sz_ensemble *e)
@@ -472,6 +496,7 @@ int sz_ensemble_autotune(sz_ensemble *e, int steps)
     if (!e) return -1;
     if (steps < 1) return -2;
     if (e->exec) return -3;            // tune before the first captured step
+    if (order_after(e->sync_ev, e->user, e->main)) return SZ_ERR_LAUNCH;
     const size_t nm = e->ch.size();
     std::vector<void *> saved(nm, nullptr);
     int rc = 0;

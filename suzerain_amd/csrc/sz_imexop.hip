@@ -22,6 +22,7 @@
 //   assembly:   A_ij[r][c] = fma chain of W_g[r] D_g[r][c] over groups whose
 //               band holds c, then + M[r][c] on diagonal blocks
 #include "sz_common.hpp"
+#include "sz_precision.hpp"
 #include "../../include/suzerain_amd.h"
 
 #include <cstring>
@@ -447,6 +448,7 @@ struct gmember {
     long long v0, frob0;         // offsets: vectors, frob partial sums
     sz::zf *lu;                  // the member's float factor storage (a slot)
     zd *bcs;                     // its wall-row scalings (8 per system)
+    int *apprx;                  // its factors are approximate (per system)
 };
 
 // Member of launch block b: the last member whose first block is <= b.
@@ -587,6 +589,142 @@ void k_imexop_accumulate(const table *__restrict__ tab, bops w, refs rf, zd phi,
                         out2 ? out2 + oo : nullptr, out3 ? out3 + oo : nullptr,
                         out4 ? out4 + oo : nullptr};
     acc_body(tab, w, rf, phi, km[s], kn[s], in, out, beta, beta_zero);
+}
+
+__device__ __forceinline__ double block_sum_ac(double v, double *red)
+{
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    double t = 0;
+    for (int i = 0; i < AC_NT / 64; ++i) t += red[i];
+    return t;
+}
+
+// One refinement sweep of the mixed precision solve for every member's
+// active systems (one block each): x += P^T c (c the float correction in
+// a.PB), r = b - (M + phi L) x with the isothermal wall rows evaluated as
+// the constraints s (x_f - fac x_rho) = 0 whose scalings s the factors hold,
+// ||r||_2 and ||x||_2, the decision of dsgbsvx.def:186-227, and for the
+// systems that continue the next right hand side P r in float.  The operator
+// product is the accumulate body's arithmetic.
+__global__ __launch_bounds__(AC_NT)
+void k_imexop_sweep_grouped(const gmember *__restrict__ mem, int nm,
+                            sz_prec_sweep a)
+{
+    extern __shared__ double smem[];
+    const gmember &M = mem[member_of(mem, nm, blockIdx.x)];
+    const int ls = blockIdx.x - M.blk0;
+    const int s = M.sys0 + ls;
+    if (a.skip[s]) return;                       // uniform over the block
+    const table *__restrict__ tab = M.tab;
+    const bops &w = M.w;
+    const refs &rf = M.rf;
+    const int n = w.n, N = 5 * n, tid = threadIdx.x;
+    zd *alpha = reinterpret_cast<zd *>(smem);       // MAXTERMS
+    zd *xs = alpha + MAXTERMS;                      // 5 x n
+    zd *Dx = xs + 5 * n;                            // 3 x 5 x n
+    zd *rr = Dx + 15 * n;                           // 5 x n
+    __shared__ int sbg0[25], sbng[25], sgop[MAXGROUPS], sgt0[MAXGROUPS], sgnt[MAXGROUPS];
+    __shared__ int stref[MAXTERMS], srld[NREF];
+    __shared__ const double *srp[NREF];
+    __shared__ double red[AC_NT / 64];
+    const long long o = M.v0 + (long long)ls * N;
+    zd *x = a.X + o;
+    const zd *b = a.B + o;
+    sz::zf *pb = a.PB + o;
+    // read before the barriers: thread 0 rewrites the state at the end
+    sz_psys S = a.S[s];
+    const int apprx = M.apprx[ls];
+    const double kms = M.km[ls], kns = M.kn[ls];
+    for (int t = tid; t < tab->nterms; t += AC_NT) {
+        alpha[t] = sz::cmul(M.phi, kappa(tab->tk[t], kms, kns));
+        stref[t] = tab->tref[t];
+    }
+    for (int g = tid; g < tab->ngroups; g += AC_NT) {
+        sgop[g] = tab->gop[g];
+        sgt0[g] = tab->gt0[g];
+        sgnt[g] = tab->gnt[g];
+    }
+    if (tid < 25) {
+        sbg0[tid] = tab->bg0[tid];
+        sbng[tid] = tab->bng[tid];
+    }
+    if (tid == 0) {
+#pragma unroll
+        for (int q = 0; q < NREF; ++q) {        // static indices: no private copy
+            srp[q] = rf.p[q];
+            srld[q] = rf.ld[q];
+        }
+    }
+    for (int e = tid; e < N; e += AC_NT) {
+        const int f = e / n, y = e - f * n;
+        const sz::zf c = pb[y * 5 + f];          // (P^T c)_e = c_{q^-1(e)}
+        const zd v = zd{x[e].x + (double)c.x, x[e].y + (double)c.y};
+        xs[e] = v;
+        x[e] = v;
+    }
+    __syncthreads();
+    for (int e = tid; e < 15 * n; e += AC_NT) {
+        const int op = e / (5 * n), rem = e - op * 5 * n;
+        const int f = rem / n, r = rem - f * n;
+        const zd *xf = xs + f * n;
+        const double *row = w.dt + op * w.dstride + (long long)r * w.ld + w.max_ku - r;
+        double tx = 0, ty = 0;
+        const int c1 = min(n - 1, r + w.kl[op]);
+        for (int c = max(0, r - w.ku[op]); c <= c1; ++c) {
+            const double av = row[c];
+            tx = fma(av, xf[c].x, tx);
+            ty = fma(av, xf[c].y, ty);
+        }
+        Dx[e] = zd{tx, ty};
+    }
+    __syncthreads();
+    double ss = 0, xx = 0;
+    for (int e = tid; e < N; e += AC_NT) {
+        const int i = e / n, r = e - i * n;
+        const int wall = r == 0 ? 0 : (r == n - 1 ? 1 : -1);
+        zd v;
+        if (i < 4 && wall >= 0 && M.bc.enforce[wall]) {
+            const zd sc = M.bcs[(size_t)ls * 8 + wall * 4 + i];
+            const double fac = M.bc.fac[wall][i];
+            const zd xr = xs[4 * n + r];
+            const zd c = zd{xs[e].x - fac * xr.x, xs[e].y - fac * xr.y};
+            const zd p = sz::cmul(sc, c);
+            v = zd{-p.x, -p.y};
+        } else {
+            zd acc{0, 0};
+            for (int j = 0; j < 5; ++j) {
+                const int bb = i * 5 + j, g0 = sbg0[bb], ng = sbng[bb];
+                for (int g = g0; g < g0 + ng; ++g) {
+                    const zd W = weight_staged(sgt0, sgnt, stref, srp, srld, alpha, g, r);
+                    acc = sz::cfma(acc, W, Dx[(sgop[g] * 5 + j) * n + r]);
+                }
+            }
+            const zd m = Dx[(0 * 5 + i) * n + r];    // the mass operator M
+            const zd t = zd{acc.x + m.x, acc.y + m.y};
+            v = zd{b[e].x - t.x, b[e].y - t.y};
+        }
+        rr[e] = v;
+        ss += v.x * v.x + v.y * v.y;
+        xx += xs[e].x * xs[e].x + xs[e].y * xs[e].y;
+    }
+    const double res = sqrt(block_sum_ac(ss, red));
+    const double xn = sqrt(block_sum_ac(xx, red));
+    const int st = sz_prec_decide(S, res, xn, a.tolsc, a.aiter, a.smax, apprx);
+    if (st == SZ_P_ACTIVE) {
+        __syncthreads();
+        for (int i = tid; i < N; i += AC_NT) {
+            const zd v = rr[(i % 5) * n + i / 5];     // (P r)_i = r_{q(i)}
+            pb[i] = sz::zf{(float)v.x, (float)v.y};
+        }
+    }
+    if (tid == 0) {
+        a.S[s] = S;
+        a.skip[s] = st == SZ_P_ACTIVE ? 0 : 1;
+    }
 }
 
 // Grouped y = (M + phi L) x of every member's systems (one block each) on
@@ -924,7 +1062,8 @@ int sz_internal_imexop_member(const sz_imexop *op, const double *phi,
                               const sz_imexop_ref *r, const sz_imexop_refld *ld,
                               const sz_isothermal_bc *bc, int blk0, int sys0,
                               int nsys, int KL, int KU, int ldlu, void *lu,
-                              void *bcs, long long v0, long long frob0, void *out)
+                              void *bcs, int *apprx, long long v0,
+                              long long frob0, void *out)
 {
     if (!op || !phi || !r || !ld || !bc || !out) return -1;
     gmember m{};
@@ -950,6 +1089,7 @@ int sz_internal_imexop_member(const sz_imexop *op, const double *phi,
     m.ldlu = ldlu;
     m.lu = static_cast<sz::zf *>(lu);
     m.bcs = static_cast<zd *>(bcs);
+    m.apprx = apprx;
     m.v0 = v0;
     m.frob0 = frob0;
     std::memcpy(out, &m, sizeof(m));
@@ -983,6 +1123,20 @@ int sz_internal_imexop_apply_grouped(const void *dmem, int nm, int nsys,
     hipLaunchKernelGGL(k_imexop_apply_grouped, dim3(nsys), dim3(AC_NT), lds,
                        (hipStream_t)stream, (const gmember *)dmem, nm,
                        (const zd *)x, (zd *)y, skip);
+    return launch_ok() ? 0 : SZ_ERR_LAUNCH;
+}
+
+// One refinement sweep of the mixed precision solve over every member's
+// systems (nsys blocks; nmax the largest n).
+int sz_internal_imexop_sweep_grouped(const void *dmem, int nm, int nsys,
+                                     int nmax, const sz_prec_sweep *a,
+                                     void *stream)
+{
+    if (nsys <= 0) return 0;
+    const size_t lds = sizeof(zd) * (MAXTERMS + 25 * (size_t)nmax);
+    if (lds > 150 * 1024) return SZ_ERR_LDS;
+    hipLaunchKernelGGL(k_imexop_sweep_grouped, dim3(nsys), dim3(AC_NT), lds,
+                       (hipStream_t)stream, (const gmember *)dmem, nm, *a);
     return launch_ok() ? 0 : SZ_ERR_LAUNCH;
 }
 

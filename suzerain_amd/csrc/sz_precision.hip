@@ -63,14 +63,14 @@ extern "C" int sz_internal_imexop_member(
     const sz_imexop *op, const double *phi, const double *km, const double *kn,
     const sz_imexop_ref *r, const sz_imexop_refld *ld, const sz_isothermal_bc *bc,
     int blk0, int sys0, int nsys, int KL, int KU, int ldlu, void *lu, void *bcs,
-    long long v0, long long frob0, void *out);
+    int *apprx, long long v0, long long frob0, void *out);
 extern "C" int sz_internal_imexop_pack_grouped(const void *dmem, int nm,
                                                int nblocks, int ldmax,
                                                const int *skip, double *frob,
                                                void *stream);
-extern "C" int sz_internal_imexop_apply_grouped(const void *dmem, int nm,
-                                                int nsys, int nmax, const void *x,
-                                                void *y, const int *skip,
+extern "C" int sz_internal_imexop_sweep_grouped(const void *dmem, int nm,
+                                                int nsys, int nmax,
+                                                const sz_prec_sweep *a,
                                                 void *stream);
 
 namespace {
@@ -79,12 +79,9 @@ using sz::zd;
 using sz::zf;
 
 constexpr int PT = 256;
-enum : int { P_DONE = 0, P_ACTIVE = 1, P_RESTART = 2, P_FAIL = 3 };
-
-struct psys {
-    int st, siter, diter, info;
-    double lastres, res, tol, tolconst;
-};
+enum : int { P_DONE = SZ_P_DONE, P_ACTIVE = SZ_P_ACTIVE, P_RESTART = SZ_P_RESTART,
+             P_FAIL = SZ_P_FAIL };
+using psys = sz_psys;
 
 // per system: order n, member, offset of its vectors (and pivots)
 struct pdesc {
@@ -130,7 +127,6 @@ struct p_args {
     const pdesc *D;
     const pmem *PM;
     zd *B, *X;
-    const zd *T;
     zf *PB;
     psys *S;
     int *skip, *fskip;
@@ -247,102 +243,6 @@ __global__ void k_p_afterfactor(p_args A, int nsys)
     }
     const double eps = DBL_EPSILON * 0.5;
     S.tolconst = use_eps ? 1.0 : M.afrob[ls] * eps * sqrt(5.0 * d.n) * A.tolsc;
-}
-
-// x += P^T (float correction) for the active systems; one block per system.
-__global__ __launch_bounds__(PT) void k_p_update(p_args A)
-{
-    const int s = blockIdx.x;
-    if (A.skip[s]) return;
-    const pdesc d = A.D[s];
-    const int n = d.n, N = 5 * n;
-    const zf *pb = A.PB + d.v;
-    zd *x = A.X + d.v;
-    for (int a = threadIdx.x; a < N; a += PT) {
-        const int f = a / n, y = a - f * n;
-        const zf c = pb[y * 5 + f];                    // (P^T c)_a = c_{q^-1(a)}
-        x[a] = zd{x[a].x + (double)c.x, x[a].y + (double)c.y};
-    }
-}
-
-// r = b - A x (T holds the matrix-free (M + phi L) x; the wall rows are the
-// constraints s (x_f - fac x_rho) = 0 with the scalings s the factors hold),
-// ||r||_2, ||x||_2 and the decision of dsgbsvx.def:186-227 for each active
-// system; the next correction's right hand side (float, permuted) for those
-// that continue.
-__global__ __launch_bounds__(PT) void k_p_post(p_args A)
-{
-    extern __shared__ __align__(16) unsigned char psm[];
-    __shared__ double red[PT / 64];
-    const int s = blockIdx.x, tid = threadIdx.x;
-    if (A.skip[s]) return;
-    psys &S = A.S[s];
-    // read before the reductions' barriers: thread 0 rewrites S at the end
-    const int siter0 = S.siter, info0 = S.info;
-    const double lastres0 = S.lastres, tolconst = S.tolconst;
-    const pdesc d = A.D[s];
-    const pmem &M = A.PM[d.m];
-    const int ls = s - M.sys0;
-    const wall_consts &wc = M.wc;
-    const int apprx = M.apprx[ls];
-    const int n = d.n, N = 5 * n;
-    const zd *b = A.B + d.v;
-    const zd *x = A.X + d.v;
-    const zd *t = A.T + d.v;
-    zd *r = reinterpret_cast<zd *>(psm);
-    double ss = 0, xs = 0;
-    for (int a = tid; a < N; a += PT) {
-        const int f = a / n, y = a - f * n;
-        zd v;
-        const int wall = (y == 0) ? 0 : (y == n - 1 ? 1 : -1);
-        if (f < 4 && wall >= 0 && wc.enforce[wall]) {
-            const zd sc = M.bcs[(size_t)ls * 8 + wall * 4 + f];
-            const double fac = wc.fac[wall][f];
-            const zd xr = x[4 * n + y];
-            const zd c = zd{x[a].x - fac * xr.x, x[a].y - fac * xr.y};
-            const zd p = sz::cmul(sc, c);
-            v = zd{-p.x, -p.y};
-        } else {
-            v = zd{b[a].x - t[a].x, b[a].y - t[a].y};
-        }
-        r[a] = v;
-        ss += v.x * v.x + v.y * v.y;
-        xs += x[a].x * x[a].x + x[a].y * x[a].y;
-    }
-    const double res = sqrt(block_sum(ss, red));
-    const double xn = sqrt(block_sum(xs, red));
-    const double eps = DBL_EPSILON * 0.5;
-    const bool use_eps = A.tolsc == 0.0;
-    const double tol = (use_eps ? eps : xn) * tolconst;
-    const int iter = siter0 + 1;
-    int st = P_ACTIVE, info = info0;
-    double lastres = res;
-    if (res != res) {
-        st = P_DONE;
-        info = -100;                       // NaN residual from a finite start
-    } else if (iter >= A.aiter && lastres0 < res * 2.0) {
-        lastres = lastres0;                // residual failed to halve
-        st = apprx ? P_RESTART : (res <= tol ? P_DONE : P_FAIL);
-    } else if (!(iter < A.smax && res > tol)) {
-        st = res <= tol ? P_DONE : P_FAIL;
-    }
-    if (st == P_ACTIVE) {
-        __syncthreads();
-        zf *pb = A.PB + d.v;
-        for (int i = tid; i < N; i += PT) {
-            const zd v = r[(i % 5) * n + i / 5];
-            pb[i] = zf{(float)v.x, (float)v.y};
-        }
-    }
-    if (tid == 0) {
-        S.siter = iter;
-        S.lastres = lastres;
-        S.res = res;
-        S.tol = tol;
-        S.info = info;
-        S.st = st;
-        A.skip[s] = st == P_ACTIVE ? 0 : 1;
-    }
 }
 
 // counts[0] <- systems marked P_RESTART, counts[1] <- P_FAIL, counts[2] <-
@@ -535,7 +435,7 @@ struct sz_prec {
     long long cap_vec = 0, cap_frob = 0;
     int cap_sys = 0, cap_mem = 0;
     pdesc *D = nullptr;
-    zd *B = nullptr, *X = nullptr, *T = nullptr;
+    zd *B = nullptr, *X = nullptr;
     zf *PB = nullptr;
     psys *S = nullptr;
     int *skip = nullptr, *fskip = nullptr, *finfo = nullptr;
@@ -569,7 +469,7 @@ struct sz_prec {
     }
     void free_pools()
     {
-        void *ps[] = {D, B, X, T, PB, S, skip, fskip, finfo, frob, list, skm, skn,
+        void *ps[] = {D, B, X, PB, S, skip, fskip, finfo, frob, list, skm, skn,
                       o_siter, o_diter, o_info, o_first, o_apprx, o_fact, o_tolsc,
                       o_res, lu64, pb64, ipiv64, info64, counts};
         for (void *q : ps) if (q) (void)hipFree(q);
@@ -669,7 +569,6 @@ void rebuild(sz_prec *h)
         const long long c = std::max(std::max(h->nvec, 1LL), h->cap_vec * 5 / 4);
         h->realloc_pool(h->B, (size_t)c);
         h->realloc_pool(h->X, (size_t)c);
-        h->realloc_pool(h->T, (size_t)c);
         h->realloc_pool(h->PB, (size_t)c);
         h->cap_vec = c;
     }
@@ -690,7 +589,7 @@ void rebuild(sz_prec *h)
     }
     if (!h->counts) h->realloc_pool(h->counts, 3);
     h->pool_bytes = (long long)h->cap_sys * (sizeof(pdesc) + sizeof(psys) + 64) +
-                    h->cap_vec * (3 * sizeof(zd) + sizeof(zf)) +
+                    h->cap_vec * (2 * sizeof(zd) + sizeof(zf)) +
                     h->cap_frob * (long long)sizeof(double);
     if (!D.empty())
         (void)hipMemcpy(h->D, D.data(), D.size() * sizeof(pdesc),
@@ -803,6 +702,7 @@ sz_prec *sz_prec_create(const std::vector<sz_prec_member> &members,
 {
     if (p.nslots < 1) return nullptr;
     sz_prec *h = new sz_prec();
+    (void)hipGetLastError();                  // start from a clean error state
     try {
         h->p = p;
         h->stream = stream;
@@ -936,12 +836,12 @@ int sz_prec_prepare(sz_prec *h, int slot, const sz_prec_problem *pr)
         const int ldlu = 2 * m.g.KL + m.g.KU + 1;
         int rc = sz_internal_imexop_member(
             q.op, phi2, q.km, q.kn, q.ref, q.ld, q.bc, blk, h->sys0[(size_t)i],
-            m.g.nsys, m.g.KL, m.g.KU, ldlu, sd.lu, sd.bcs, h->v0[(size_t)i],
+            m.g.nsys, m.g.KL, m.g.KU, ldlu, sd.lu, sd.bcs, sd.apprx, h->v0[(size_t)i],
             h->frob0[(size_t)i], buf.data() + (size_t)gbytes * i);
         if (rc) return rc;
         rc = sz_internal_imexop_member(
             q.op, phi2, q.km, q.kn, q.ref, q.ld, q.bc, pblk, h->sys0[(size_t)i],
-            m.g.nsys, m.g.KL, m.g.KU, ldlu, sd.lu, sd.bcs, h->v0[(size_t)i],
+            m.g.nsys, m.g.KL, m.g.KU, ldlu, sd.lu, sd.bcs, sd.apprx, h->v0[(size_t)i],
             h->frob0[(size_t)i], buf.data() + gsz + (size_t)gbytes * i);
         if (rc) return rc;
         blk += m.g.nsys;                          // one accumulate block a system
@@ -1001,21 +901,26 @@ int single_pass(sz_prec *h, sz_prec::slot &sl, int slot, const p_args &A0, int p
     hipLaunchKernelGGL(k_p_afterfactor, dim3(nb(h->nsys)), dim3(PT), 0, st, A,
                        h->nsys);
     if (!ok()) return SZ_ERR_LAUNCH;
-    const size_t lds_post = (size_t)5 * h->nmax * sizeof(zd);
+    sz_prec_sweep sw{};
+    sw.B = h->B;
+    sw.X = h->X;
+    sw.PB = h->PB;
+    sw.S = h->S;
+    sw.skip = h->skip;
+    sw.tolsc = h->p.tolsc;
+    sw.aiter = h->p.aiter;
+    sw.smax = h->p.smax;
     for (int it = 0; it <= h->p.smax; ++it) {
+        // c = (P A^T P^T)^-T P r in float, then x += P^T c, r = b - A x,
+        // decision and the next float right hand side in one launch
         for (const auto &c : h->classes) {
             rc = sz_internal_gbtrs_grouped(1, 'T', c.KL, c.KU, nullptr, c.ldlu,
                                            nullptr, nullptr, h->skip + c.s0,
                                            c.gs[(size_t)slot], c.ns, c.nmax, st);
             if (rc) return rc;
         }
-        hipLaunchKernelGGL(k_p_update, dim3(h->nsys), dim3(PT), 0, st, A);
-        if (!ok()) return SZ_ERR_LAUNCH;
-        rc = sz_internal_imexop_apply_grouped(sl.gmem, nm, h->nsys, h->nmax, h->X, h->T,
-                                              h->skip, st);
+        rc = sz_internal_imexop_sweep_grouped(sl.gmem, nm, h->nsys, h->nmax, &sw, st);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_p_post, dim3(h->nsys), dim3(PT), lds_post, st, A);
-        if (!ok()) return SZ_ERR_LAUNCH;
     }
     return 0;
 }
@@ -1140,7 +1045,6 @@ int sz_prec_solve(sz_prec *h, int slot, const sz_prec_problem *pr)
     A.PM = sl.pm;
     A.B = h->B;
     A.X = h->X;
-    A.T = h->T;
     A.PB = h->PB;
     A.S = h->S;
     A.skip = h->skip;

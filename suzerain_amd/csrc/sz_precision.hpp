@@ -25,6 +25,60 @@
 
 struct sz_prec;
 
+// Per-system state of the mixed precision solve, shared by the solver's
+// kernels and the fused residual sweep of the operator kernels.
+enum : int { SZ_P_DONE = 0, SZ_P_ACTIVE = 1, SZ_P_RESTART = 2, SZ_P_FAIL = 3 };
+struct sz_psys {
+    int st, siter, diter, info;
+    double lastres, res, tol, tolconst;
+};
+
+// The decision after a sweep (dsgbsvx.def:186-227): residual norm res,
+// ||x||_2 xn; returns the next state and updates the sweep record (the
+// caller writes it back).  NaN residual: done with info -100; a residual
+// that fails to halve after aiter sweeps blames reused factors (restart) or
+// ends the single precision attempt; the sweep limit or the tolerance end
+// it otherwise.
+__host__ __device__ inline int sz_prec_decide(sz_psys &S, double res, double xn,
+                                              double tolsc, int aiter, int smax,
+                                              int apprx)
+{
+    const double eps = 1.1102230246251565e-16;      // 2^-53
+    const bool use_eps = tolsc == 0.0;
+    const double tol = (use_eps ? eps : xn) * S.tolconst;
+    const int iter = S.siter + 1;
+    int st = SZ_P_ACTIVE;
+    double lastres = res;
+    if (res != res) {
+        st = SZ_P_DONE;
+        S.info = -100;                  // NaN residual from a finite start
+    } else if (iter >= aiter && S.lastres < res * 2.0) {
+        lastres = S.lastres;            // residual failed to halve
+        st = apprx ? SZ_P_RESTART : (res <= tol ? SZ_P_DONE : SZ_P_FAIL);
+    } else if (!(iter < smax && res > tol)) {
+        st = res <= tol ? SZ_P_DONE : SZ_P_FAIL;
+    }
+    S.siter = iter;
+    S.lastres = lastres;
+    S.res = res;
+    S.tol = tol;
+    S.st = st;
+    return st;
+}
+
+// Arguments of the fused sweep (x += P^T c, r = b - A x with the wall
+// constraints, norms, decision, next c = P r in float) over a grouped
+// launch; the members' descriptors carry their slot's apprx and wall data.
+struct sz_prec_sweep {
+    const sz::zd *B;
+    sz::zd *X;
+    sz::zf *PB;
+    sz_psys *S;
+    int *skip;
+    double tolsc;
+    int aiter, smax;
+};
+
 // One member's systems: n B-spline coefficients per field (N = 5n), the
 // BSMBSM band (KL, KU) of P A^T P^T and the number of (kx, kz) systems.
 struct sz_prec_member {

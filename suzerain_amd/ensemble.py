@@ -118,7 +118,7 @@ def member_bytes(cfg, slots=3):
     F = dNz * Xw * Ny
     state = 2 * 5 * F * 16
     lu = slots * nsys * N * (3 * KL + 1) * 8 + slots * nsys * N * 4
-    vec = nsys * N * (3 * 16 + 8)
+    vec = nsys * N * (2 * 16 + 8)
     return state + lu + vec + nsys * 400
 
 

@@ -37,3 +37,16 @@ def test_out_of_order_and_spacing():
 def test_invalid(bad):
     with pytest.raises(ValueError, match="invalid"):
         spec_normalize(bad)
+
+
+def test_zgbsvx_is_refused_with_a_message():
+    """zgbsvx (the equilibrated expert driver, bsmbsm_solver.cpp:184-291)
+    is not provided: build refuses it with std::invalid_argument, which the
+    C handle reports as NULL plus the message (no device work happens)."""
+    import ctypes as C
+    from suzerain_amd._lib import load
+    lib = load()
+    err = C.create_string_buffer(256)
+    h = lib.sz_solver_create(b"zgbsvx,equil=true", 5, 12, 6, 6, 1, 2, None, err, 256)
+    assert not h
+    assert "zgbsvx is not provided" in err.value.decode()
