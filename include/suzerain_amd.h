@@ -27,6 +27,7 @@ extern "C" {
 #define SZ_ERR_LAUNCH (-1001)
 #define SZ_ERR_LDS    (-1002)
 #define SZ_ERR_ALLOC  (-1003)
+#define SZ_ERR_SCRATCH (-1004)  /* batch exceeds a no-alloc entry's scratch */
 
 /* ------------------------------------------------------------------ bsmbsm
  * Replaces suzerain_bsmbsm_construct / _q / _qinv (suzerain/bsmbsm.h:104-186).
@@ -213,6 +214,28 @@ int sz_zcgbsvx_batched(char *fact, int *apprx, int aiter, char trans, int n,
                        int *siter_out, int *diter_out, double *tolsc_out,
                        void *r, long long strider, double *res, int *info,
                        void *work, int nbatch, void *stream);
+
+/* No counterpart in the reference: sz_zcgbsvx_batched for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  The per-system refinement state (the
+ * `work` argument above) lives in a static device array of the library, one
+ * per GPU, sized for SZ_ZCGBSVX_NOALLOC_MAX systems (7 MiB); larger
+ * batches return SZ_ERR_SCRATCH.  Nothing is allocated and nothing from an
+ * earlier call is read.  Calls that overlap on one GPU share that array and
+ * race: serialize them (one stream, or events).  r stays a caller array, as
+ * in the reference's interface. */
+#define SZ_ZCGBSVX_NOALLOC_MAX 131072
+int sz_zcgbsvx_batched_noalloc(char *fact, int *apprx, int aiter, char trans,
+                               int n, int kl, int ku, const void *ab, int ldab,
+                               long long strideab, double *afrob, void *afb,
+                               int ldafb, long long strideafb, int *ipiv,
+                               long long strideipiv, const void *b,
+                               long long strideb, void *x, long long stridex,
+                               int siter, int diter, double tolsc,
+                               int *siter_out, int *diter_out,
+                               double *tolsc_out, void *r, long long strider,
+                               double *res, int *info, int nbatch,
+                               void *stream);
 
 /* ------------------------------------------- compressible implicit operator
  * M + phi L for the state (rho E, rho u, rho v, rho w, rho), L linearized

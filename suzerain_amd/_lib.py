@@ -108,6 +108,10 @@ SIGNATURES = {
                                  vp, vp, i32, i64, vp, i64, vp, i64, vp, i64,
                                  i32, i32, f64, vp, vp, vp, vp, i64, vp, vp,
                                  vp, i32, vp]),
+    "sz_zcgbsvx_batched_noalloc": (i32, [vp, vp, i32, ch, i32, i32, i32, vp,
+                                         i32, i64, vp, vp, i32, i64, vp, i64,
+                                         vp, i64, vp, i64, i32, i32, f64, vp,
+                                         vp, vp, vp, i64, vp, vp, i32, vp]),
     "sz_imexop_create": (vp, [vp, vp, pi]),
     "sz_imexop_free": (None, [vp]),
     "sz_imexop_nterms": (i32, [vp]),

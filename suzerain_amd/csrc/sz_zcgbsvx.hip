@@ -18,8 +18,10 @@
 // x {masked CGBTRS, sweep}, double check, masked factor, diter x {masked
 // ZGBTRS, sweep}, finish), so nothing waits on the host.
 #include "sz_common.hpp"
+#include "../../include/suzerain_amd.h"
 
 #include <cfloat>
+#include <mutex>
 
 extern "C" int sz_internal_gbtrf_masked(int single, int n, int kl, int ku,
                                         void *ab, int ldab, long long strideab,
@@ -359,25 +361,56 @@ __global__ __launch_bounds__(NT) void k_zc_finish(zc_args A)
 
 inline unsigned nb1(int n) { return (unsigned)((n + 255) / 256); }
 
-}  // namespace
-
-extern "C" {
-
-/* Workspace bytes sz_zcgbsvx_batched needs for `nbatch` systems. */
-long long sz_zcgbsvx_workspace_bytes(int nbatch)
+constexpr long long workspace_bytes(long long nbatch)
 {
-    return (long long)nbatch * ((long long)sizeof(zc_state) + 2 * sizeof(int)) + 256;
+    return nbatch * ((long long)sizeof(zc_state) + 2 * sizeof(int)) + 256;
 }
 
-int sz_zcgbsvx_batched(char *fact, int *apprx, int aiter, char trans, int n,
-                       int kl, int ku, const void *ab, int ldab,
-                       long long strideab, double *afrob, void *afb, int ldafb,
-                       long long strideafb, int *ipiv, long long strideipiv,
-                       const void *b, long long strideb, void *x,
-                       long long stridex, int siter, int diter, double tolsc,
-                       int *siter_out, int *diter_out, double *tolsc_out,
-                       void *r, long long strider, double *res, int *info,
-                       void *work, int nbatch, void *stream)
+// Scratch of sz_zcgbsvx_batched_noalloc: the refinement state, masks and
+// factorization statuses of up to SZ_ZCGBSVX_NOALLOC_MAX systems.  Each GPU
+// holds its own copy for the life of the process; host code reaches it only
+// through the address hipGetSymbolAddress returns for that GPU.
+constexpr long long NOALLOC_BYTES = workspace_bytes(SZ_ZCGBSVX_NOALLOC_MAX);
+__device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+void *scratch_ptr[MAX_DEVICES];
+
+// Device address of `scratch` on the GPU that runs `stream`, looked up once
+// per GPU with that GPU current.
+int scratch_address(hipStream_t stream, void **out)
+{
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return SZ_ERR_LAUNCH;
+    int dev = cur;
+    if (stream) {
+        hipDevice_t d = 0;
+        if (hipStreamGetDevice(stream, &d) != hipSuccess) return SZ_ERR_LAUNCH;
+        dev = (int)d;
+    }
+    if (dev < 0 || dev >= MAX_DEVICES) return SZ_ERR_LAUNCH;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        if (dev != cur && hipSetDevice(dev) != hipSuccess) return SZ_ERR_LAUNCH;
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (dev != cur) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p) return SZ_ERR_LAUNCH;
+        scratch_ptr[dev] = p;
+    }
+    *out = scratch_ptr[dev];
+    return 0;
+}
+
+int zcgbsvx_run(char *fact, int *apprx, int aiter, char trans, int n, int kl,
+                int ku, const void *ab, int ldab, long long strideab,
+                double *afrob, void *afb, int ldafb, long long strideafb,
+                int *ipiv, long long strideipiv, const void *b,
+                long long strideb, void *x, long long stridex, int siter,
+                int diter, double tolsc, int *siter_out, int *diter_out,
+                double *tolsc_out, void *r, long long strider, double *res,
+                int *info, void *work, int nbatch, void *stream)
 {
     int tr;
     switch (trans) {
@@ -457,6 +490,60 @@ int sz_zcgbsvx_batched(char *fact, int *apprx, int aiter, char trans, int n,
         if (!apprx) break;
     }
     return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+/* Workspace bytes sz_zcgbsvx_batched needs for `nbatch` systems. */
+long long sz_zcgbsvx_workspace_bytes(int nbatch)
+{
+    return workspace_bytes(nbatch);
+}
+
+int sz_zcgbsvx_batched(char *fact, int *apprx, int aiter, char trans, int n,
+                       int kl, int ku, const void *ab, int ldab,
+                       long long strideab, double *afrob, void *afb, int ldafb,
+                       long long strideafb, int *ipiv, long long strideipiv,
+                       const void *b, long long strideb, void *x,
+                       long long stridex, int siter, int diter, double tolsc,
+                       int *siter_out, int *diter_out, double *tolsc_out,
+                       void *r, long long strider, double *res, int *info,
+                       void *work, int nbatch, void *stream)
+{
+    return zcgbsvx_run(fact, apprx, aiter, trans, n, kl, ku, ab, ldab, strideab,
+                       afrob, afb, ldafb, strideafb, ipiv, strideipiv, b, strideb,
+                       x, stridex, siter, diter, tolsc, siter_out, diter_out,
+                       tolsc_out, r, strider, res, info, work, nbatch, stream);
+}
+
+// The same solve with the refinement state in the library's static scratch
+// instead of a caller workspace: no device allocation anywhere on the path.
+// Every kernel writes a system's state before it reads it, so nothing left
+// in the scratch by an earlier call is used; calls that overlap on one GPU
+// share the scratch and race.
+int sz_zcgbsvx_batched_noalloc(char *fact, int *apprx, int aiter, char trans,
+                               int n, int kl, int ku, const void *ab, int ldab,
+                               long long strideab, double *afrob, void *afb,
+                               int ldafb, long long strideafb, int *ipiv,
+                               long long strideipiv, const void *b,
+                               long long strideb, void *x, long long stridex,
+                               int siter, int diter, double tolsc,
+                               int *siter_out, int *diter_out,
+                               double *tolsc_out, void *r, long long strider,
+                               double *res, int *info, int nbatch, void *stream)
+{
+    if (nbatch > SZ_ZCGBSVX_NOALLOC_MAX) return SZ_ERR_SCRATCH;
+    void *work = nullptr;
+    if (nbatch > 0) {
+        const int rc = scratch_address((hipStream_t)stream, &work);
+        if (rc) return rc;
+    }
+    return zcgbsvx_run(fact, apprx, aiter, trans, n, kl, ku, ab, ldab, strideab,
+                       afrob, afb, ldafb, strideafb, ipiv, strideipiv, b, strideb,
+                       x, stridex, siter, diter, tolsc, siter_out, diter_out,
+                       tolsc_out, r, strider, res, info, work, nbatch, stream);
 }
 
 }  // extern "C"
