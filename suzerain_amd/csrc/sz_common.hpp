@@ -79,6 +79,32 @@ struct gsys {
     unsigned long long ab, ipiv, b;
 };
 
+// The same address seen as global memory: accesses through the result
+// compile to global_* instructions, counted by vmcnt alone.  Through a
+// generic pointer (one read from a descriptor, say) they are flat_*
+// instructions, which also count against lgkmcnt, so every wait for an LDS
+// access drains the loads still in flight.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gmem(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+
+template <typename R>
+__device__ __forceinline__ cpx<R> gload(const cpx<R> *p)
+{
+    const auto q = gmem(p);
+    return {q->x, q->y};
+}
+
+template <typename R>
+__device__ __forceinline__ void gstore(cpx<R> *p, cpx<R> v)
+{
+    const auto q = gmem(p);
+    q->x = v.x;
+    q->y = v.y;
+}
+
 // a - b*c with fused multiply-adds.
 template <typename R>
 __device__ inline cpx<R> cfnms(cpx<R> a, cpx<R> b, cpx<R> c)

@@ -27,6 +27,7 @@
 #include "sz_common.hpp"
 
 #include <cstdlib>
+#include <type_traits>
 #include <utility>
 
 namespace {
@@ -559,6 +560,279 @@ void k_gbtrs(int trans, int n, int kl, int ku, int nrhs,
         __builtin_amdgcn_wave_barrier();                                      \
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");                \
     } while (0)
+
+// ------------------------------------------------------------ t16 solve
+// A^T x = b (A^H x = b) from the factors, sixteen lanes per system and four
+// systems per wavefront, for the band shapes of the channel operators.
+//   * U^T y = b, ascending, in dot-product form: y_j = (b_j - sum_{t=1..kv}
+//     U(j-t, j) y_{j-t}) / U(j, j).  Column j of U is one contiguous run of
+//     the band storage, so every load reads whole cache lines.  Lane l of a
+//     system owns the rows i = l (mod 16): during group g (columns 16g ..
+//     16g+15) its register slot k holds y_{16(g-k)+l} (slot 0 fills during
+//     the group), and slot k of column j is the run of band rows
+//     kv - (j mod 16) - 16k + l, one load per slot.  No y moves between
+//     lanes: a step shares only the 16-lane sum (four DPP stages).  Terms
+//     outside the band are never loaded (zero), and the lane owning row j
+//     finds the diagonal U(j, j) in its own slot 0, so it forms
+//     y_j = (b_j - sum) / U(j, j) from its own registers.
+//   * L^T x = y and the interchanges, descending, through LDS: lane l forms
+//     the terms r = l (mod 16) of the dot product with the multipliers of
+//     column j (one contiguous run), the 16-lane sum completes it and lane 0
+//     of the system subtracts and swaps, as ZGBTRS does column by column.
+// Factor columns are requested T16_PF steps before their use.  UT = false
+// solves only the L^T half (U^T was solved during the factorization).  The
+// sums run in a different order than ZGBTRS's; the solve is used where the
+// result is refined or checked to a tolerance, not compared bit for bit.
+constexpr int T16_PF = 8;
+
+// Sum over the 16 lanes of a DPP row, delivered to all 16 (the first four
+// stages of wave_sum_asc).
+template <typename R>
+__device__ __forceinline__ R rowsum16(R v)
+{
+    v += dpp_r<0xB1>(v);                    // quad_perm [1,0,3,2]
+    v += dpp_r<0x4E>(v);                    // quad_perm [2,3,0,1]
+    v += dpp_r<0x141>(v);                   // row_half_mirror
+    v += dpp_r<0x140>(v);                   // row_mirror
+    return v;
+}
+
+// 1/z as conj(z)/|z|^2 after rescaling z by a power of two, so |z|^2
+// neither overflows nor underflows; float uses the hardware reciprocal.
+template <typename R>
+__device__ __forceinline__ cpx<R> rcp_scaled(cpx<R> z)
+{
+    int e;
+    (void)frexp(fmax(fabs(z.x), fabs(z.y)), &e);
+    const R zx = ldexp(z.x, -e), zy = ldexp(z.y, -e);
+    const R d = fma(zx, zx, zy * zy);
+    R r;
+    if constexpr (sizeof(R) == 4) r = __builtin_amdgcn_rcpf(d);
+    else r = R(1) / d;
+    return {ldexp(zx * r, -e), ldexp(-zy * r, -e)};
+}
+
+template <typename R, int KL, int KV, bool UT, bool CJ>
+__global__ __launch_bounds__(64)
+void k_gbtrs_t16(int n, const cpx<R> *__restrict__ ab, int ldab, long long sab,
+                 const int *__restrict__ ipiv, long long sipiv,
+                 cpx<R> *__restrict__ b, long long sb,
+                 const int *__restrict__ skip, int nsys, int ldx,
+                 const sz::gsys *__restrict__ gs)
+{
+    using T = cpx<R>;
+    constexpr int NS = (KV + 15) / 16 + 1;   // U^T register slots
+    constexpr int NM = (KL + 15) / 16;       // L^T terms per lane
+    constexpr int PF = T16_PF;
+    static_assert(16 % PF == 0, "queue entries must repeat within a group");
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x, l = lane & 15, grp = lane >> 4;
+    const int sys = blockIdx.x * 4 + grp;
+    // Every lane loads from the storage of its system (of system 0 past the
+    // end of the batch), unconditionally and at clamped addresses, so the
+    // loads in flight are the same on every path and the queue drains only
+    // as far as each use needs; what a lane must not use is masked where
+    // it is used.  nl: columns of the storage read (n >= 2); ns: columns
+    // solved and stored (0 past the end and for skipped systems).
+    const int s0 = sys < nsys ? sys : 0;
+    const T *a;
+    const int *pv;
+    T *bs;
+    int nl;
+    if (gs) {
+        const sz::gsys g = gs[s0];
+        nl = g.n;
+        a = reinterpret_cast<const T *>(g.ab);
+        pv = reinterpret_cast<const int *>(g.ipiv);
+        bs = reinterpret_cast<T *>(g.b);
+    } else {
+        nl = n;
+        a = ab + (size_t)s0 * (size_t)sab;
+        pv = ipiv + (size_t)s0 * (size_t)sipiv;
+        bs = b + (size_t)s0 * (size_t)sb;
+    }
+    const int ns = (sys < nsys && !(skip && skip[sys] != 0)) ? nl : 0;
+    const int nw = max(max(__builtin_amdgcn_readlane(ns, 0), __builtin_amdgcn_readlane(ns, 16)),
+                       max(__builtin_amdgcn_readlane(ns, 32),
+                           __builtin_amdgcn_readlane(ns, 48)));
+    if (nw == 0) return;
+    T *x = reinterpret_cast<T *>(smem) + (size_t)grp * ldx;
+
+    if constexpr (UT) {
+        R yx[NS], yy[NS], qx[PF][NS], qy[PF][NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) yx[k] = yy[k] = R(0);
+        // column c (wave-uniform, c mod 16 = u16) into one queue entry.
+        // Lane l of slot k reads row KV - u16 - 16k + l; lanes whose row
+        // falls outside the U rows 0..KV of the column (masked at use in
+        // any case) read the nearest U row instead, so the loads touch the
+        // column's U lines only and stay inside the storage for every c.
+        auto fetch = [&](int c, int u16, R (&fx)[NS], R (&fy)[NS]) {
+            const T *col = a + (size_t)min(c, nl - 1) * ldab;
+#pragma unroll
+            for (int k = 0; k < NS; ++k) {
+                const int r0 = KV - u16 - 16 * k;
+                int row = r0 + l;
+                if (r0 < 0) row = max(row, 0);
+                if (r0 + 15 > KV) row = min(row, KV);
+                const T v = sz::gload(col + row);
+                fx[k] = v.x;
+                fy[k] = v.y;
+            }
+        };
+        T bn = sz::gload(bs + min(l, nl - 1));
+        // the prologue issues the queue in order of use (no reordering
+        // across entries), as every later step refills it
+#pragma unroll
+        for (int c = 0; c < PF; ++c) {
+            fetch(c, c, qx[c], qy[c]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // one group of 16 columns; HEAD: the first groups, where slot k > g
+        // holds rows above row 0
+        auto group = [&](int g, auto head) {
+            constexpr bool HEAD = decltype(head)::value;
+            const T bc = bn;
+            bn = sz::gload(bs + min(16 * g + 16 + l, nl - 1));
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int q = u % PF;
+                // slots NS-1..1 in two independent chains, then slot 0 (the
+                // newest y, the only term that waits on the last step)
+                R ax2[2] = {R(0), R(0)}, ay2[2] = {R(0), R(0)};
+                R sx = R(0), sy = R(0);
+#pragma unroll
+                for (int k = NS - 1; k >= 0; --k) {
+                    // lanes with 1 <= u + 16k - l <= KV hold a term
+                    const int lo = u + 16 * k - KV, hi = u + 16 * k - 1;
+                    if (hi < 0) continue;
+                    R ax = qx[q][k], ay = CJ ? -qy[q][k] : qy[q][k];
+                    if (lo > 0 || hi < 15 || (HEAD && k > 0)) {
+                        const bool ok = l >= lo && l <= hi && (!HEAD || k == 0 || g >= k);
+                        ax = ok ? ax : R(0);
+                        ay = ok ? ay : R(0);
+                    }
+                    if (k > 0) {
+                        ax2[k & 1] = fma(ax, yx[k], ax2[k & 1]);
+                        ax2[k & 1] = fma(-ay, yy[k], ax2[k & 1]);
+                        ay2[k & 1] = fma(ax, yy[k], ay2[k & 1]);
+                        ay2[k & 1] = fma(ay, yx[k], ay2[k & 1]);
+                    } else {
+                        sx = ax2[0] + ax2[1];
+                        sy = ay2[0] + ay2[1];
+                        sx = fma(ax, yx[0], sx);
+                        sx = fma(-ay, yy[0], sx);
+                        sy = fma(ax, yy[0], sy);
+                        sy = fma(ay, yx[0], sy);
+                    }
+                }
+                if (u == 0) {                      // slot 0 holds no term yet
+                    sx = ax2[0] + ax2[1];
+                    sy = ay2[0] + ay2[1];
+                }
+                sx = rowsum16(sx);
+                sy = rowsum16(sy);
+                // lane u's slot 0 holds the diagonal U(j, j)
+                const T dg = {qx[q][0], CJ ? -qy[q][0] : qy[q][0]};
+                const T r = sz::cmul(T{bc.x - sx, bc.y - sy}, rcp_scaled(dg));
+                if (l == u) {
+                    yx[0] = r.x;
+                    yy[0] = r.y;
+                }
+                fetch(16 * g + u + PF, (u + PF) & 15, qx[q], qy[q]);
+            }
+            if (16 * g + l < ns) {
+                x[16 * g + l].x = yx[0];
+                x[16 * g + l].y = yy[0];
+            }
+#pragma unroll
+            for (int k = NS - 1; k > 0; --k) {
+                yx[k] = yx[k - 1];
+                yy[k] = yy[k - 1];
+            }
+            yx[0] = yy[0] = R(0);
+        };
+        // two loops rather than a branch per group, so the queue registers
+        // carry from group to group without copies
+        const int ng = (nw + 15) / 16, nh = min(ng, NS - 1);
+        int g = 0;
+        for (; g < nh; ++g) group(g, std::true_type{});
+        for (; g < ng; ++g) group(g, std::false_type{});
+        // the queue's last entries (columns past the end) are not used:
+        // retire them here, so the L^T queue starts from an empty counter
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
+    } else {
+        for (int i = l; i < ns; i += 16) x[i] = sz::gload(bs + i);
+    }
+    WAVE_SYNC();
+
+    if constexpr (KL > 0) {
+        R lx[PF][NM], ly[PF][NM];
+        int lp[PF];
+        // step t: column j = ns - 2 - t, terms r < min(KL, t + 1); a lane
+        // reads multiplier rows r = l + 16m, the last run clamped to the
+        // column, and the pivot of row j
+        auto lfetch = [&](int t, R (&fx)[NM], R (&fy)[NM], int &p) {
+            const int jl = max(nl - 2 - t, 0);
+            const T *col = a + (size_t)jl * ldab + (KV + 1);
+#pragma unroll
+            for (int m = 0; m < NM; ++m) {
+                const T v = sz::gload(col + min(l + 16 * m, KL - 1));
+                fx[m] = v.x;
+                fy[m] = v.y;
+            }
+            p = *sz::gmem(pv + jl);
+        };
+#pragma unroll
+        for (int t = 0; t < PF; ++t) {
+            lfetch(t, lx[t], ly[t], lp[t]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // whole groups of 16 steps; steps past a system's last column
+        // (j < 0) read and write nothing of it
+        for (int t0 = 0; t0 < nw - 1; t0 += 16) {
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int t = t0 + u;
+                const int q = u % PF;
+                const int j = ns - 2 - t;
+                const int lm = min(KL, t + 1);
+                R sx = R(0), sy = R(0);
+#pragma unroll
+                for (int m = 0; m < NM; ++m) {
+                    const int r = l + 16 * m;
+                    const bool ok = j >= 0 && r < lm;
+                    const T xv = x[ok ? j + 1 + r : 0];
+                    const R vx = ok ? xv.x : R(0), vy = ok ? xv.y : R(0);
+                    const R ax = ok ? lx[q][m] : R(0);
+                    const R ay = ok ? (CJ ? -ly[q][m] : ly[q][m]) : R(0);
+                    sx = fma(ax, vx, sx);
+                    sx = fma(-ay, vy, sx);
+                    sy = fma(ax, vy, sy);
+                    sy = fma(ay, vx, sy);
+                }
+                sx = rowsum16(sx);
+                sy = rowsum16(sy);
+                if (l == 0 && j >= 0) {
+                    int p = lp[q] - 1;
+                    if (p < j || p >= ns) p = j;
+                    const R vx = x[j].x - sx, vy = x[j].y - sy;
+                    const R px = x[p].x, py = x[p].y;
+                    x[p].x = vx;
+                    x[p].y = vy;
+                    if (p != j) {
+                        x[j].x = px;
+                        x[j].y = py;
+                    }
+                }
+                lfetch(t + PF, lx[q], ly[q], lp[q]);
+                WAVE_SYNC();
+            }
+        }
+    }
+    WAVE_SYNC();
+    for (int i = l; i < ns; i += 16) sz::gstore(bs + i, x[i]);
+}
 
 // ---------------------------------------------------------------- wave
 // One wavefront per system, no barriers.  Lane l holds the window column
@@ -1125,6 +1399,44 @@ int trans_code(char t)
     }
 }
 
+// The sixteen-lane transposed solve for the channel band shapes in single
+// precision (the refinement solves of the mixed precision solver); false
+// when the shape or precision has no such kernel.  gs != nullptr: grouped,
+// every system of order 2 or more (the kernel clamps its loads to the
+// system's last column).
+template <typename R>
+bool launch_gbtrs_t16(int tc, int n, int kl, int ku, const void *ab, int ldab,
+                      long long sab, const int *ipiv, long long sipiv, void *b,
+                      long long sb, const int *skip, int nsys, int ldx,
+                      const sz::gsys *gs, hipStream_t stream)
+{
+    if constexpr (sizeof(R) != 4) {
+        return false;
+    } else {
+        if (tc == 0 || ldx <= 0 || (!gs && n < 2)) return false;
+        const size_t lds = 4 * (size_t)ldx * sizeof(cpx<R>);
+        if (lds > LDS_MAX) return false;
+        const dim3 grid((nsys + 3) / 4);
+        const cpx<R> *a = reinterpret_cast<const cpx<R> *>(ab);
+        cpx<R> *x = reinterpret_cast<cpx<R> *>(b);
+#define SZ_T16(KL_, KV_, CJ_)                                                   \
+        hipLaunchKernelGGL((k_gbtrs_t16<R, KL_, KV_, true, CJ_>), grid, dim3(64),  \
+                           lds, stream, n, a, ldab, sab, ipiv, sipiv, x, sb, skip, \
+                           nsys, ldx, gs)
+        if (kl == 34 && ku == 34) {
+            if (tc == 2) SZ_T16(34, 68, true);
+            else SZ_T16(34, 68, false);
+        } else if (kl == 24 && ku == 24) {
+            if (tc == 2) SZ_T16(24, 48, true);
+            else SZ_T16(24, 48, false);
+        } else {
+            return false;
+        }
+#undef SZ_T16
+        return true;
+    }
+}
+
 template <typename R>
 int launch_gbtrs(char trans, int n, int kl, int ku, int nrhs, const void *ab,
                  int ldab, long long sab, const int *ipiv, long long sipiv,
@@ -1151,6 +1463,10 @@ int launch_gbtrs(char trans, int n, int kl, int ku, int nrhs, const void *ab,
                                      + 64 * sizeof(int)
                                : (size_t)n * sizeof(cpx<R>);
     if (lds > LDS_MAX) return SZ_ERR_LDS;
+    if (nrhs == 1 && !skip_ut &&
+        launch_gbtrs_t16<R>(tc, n, kl, ku, ab, ldab, sab, ipiv, sipiv, b, sb, skip,
+                            nsys, n, nullptr, stream))
+        return hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH;
     if (tc == 0)
         hipLaunchKernelGGL((k_gbtrs<R, false>), dim3(nsys), dim3(TRS_NT), lds, stream, tc, n,
                        kl, ku, nrhs, reinterpret_cast<const cpx<R> *>(ab), ldab,
@@ -1211,6 +1527,9 @@ int launch_gbtrs_grouped(char trans, int kl, int ku, const void *ab, int ldab,
     if (kl + ku + 1 > TRS_PRE * TRS_NT) return SZ_ERR_LDS;
     const size_t lds = (size_t)nmax * sizeof(cpx<R>);
     if (lds > LDS_MAX) return SZ_ERR_LDS;
+    if (launch_gbtrs_t16<R>(tc, 0, kl, ku, ab, ldab, 0LL, ipiv, 0LL, b, 0LL, skip, nsys,
+                            nmax, gs, stream))
+        return hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH;
     hipLaunchKernelGGL((k_gbtrs<R, true>), dim3(nsys), dim3(TRS_NT), lds, stream, tc,
                        nmax, kl, ku, 1, reinterpret_cast<const cpx<R> *>(ab), ldab,
                        0LL, ipiv, 0LL, reinterpret_cast<cpx<R> *>(b), nmax, 0LL, skip,
