@@ -37,14 +37,28 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--workload", choices=("rk3", "solve"), default="rk3")
+    p.add_argument("--workload", choices=("rk3", "solve", "precision", "ensemble",
+                                          "serving"), default="rk3")
     p.add_argument("--steps", type=int, default=40)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-traffic", action="store_true",
                    help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes")
-    p.add_argument("--pmc-child", choices=("rk3",), default=None,
-                   help=argparse.SUPPRESS)
+    p.add_argument("--pmc-child", default=None, help=argparse.SUPPRESS)
+    p.add_argument("--no-extra", action="store_true",
+                   help="rk3 only: skip the precision, ensemble and serving keys")
+    p.add_argument("--prec-sizes", default="96,192",
+                   help="cube sizes of the precision-form comparison")
+    p.add_argument("--prec-steps", type=int, default=4)
+    p.add_argument("--members", type=int, default=48,
+                   help="ensemble members (the uncertainty sweep's first ones)")
+    p.add_argument("--ens-steps", type=int, default=3)
+    p.add_argument("--requests", type=int, default=16,
+                   help="serving trace length")
+    p.add_argument("--rate", type=float, default=2.0,
+                   help="serving arrivals per second")
+    p.add_argument("--req-steps", default="10,60",
+                   help="serving: steps per request ~ U{lo..hi}")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only run a plain device-to-device copy of MIB MiB "
                         "three times (reference reading for HBM traffic "
@@ -392,14 +406,245 @@ def pmc_traffic(kernel_regex, child_args, copy_mib=256, timeout=150):
 
 
 def pmc_child(kind):
-    """The short run a counter pass profiles: one step of the workload."""
+    """The short run a counter pass profiles: one step of the workload, or
+    (t16:NSYS:N) two launches of the single precision transposed solve."""
     import torch
-    if kind == "rk3":
+    if kind.startswith("t16:"):
+        _, nsys, N = kind.split(":")
+        t16_roofline(int(nsys), int(N), 34, reps=2)
+    elif kind == "rk3":
         from suzerain_amd.channel import Channel, laminar_state
         ch = Channel(**RK3_CONFIG)
         ch.set_state(torch.from_numpy(laminar_state(ch)).to("cuda:0"))
         ch.step()
     torch.cuda.synchronize()
+
+
+PREC_SPEC = "zcgbsvx,reuse=true,siter=3,diter=5,tolsc=0.1"
+
+
+def field_errors(got, want):
+    """Per-field max |got - want| / max |want| over the five conserved
+    fields (rho E, rho u, rho v, rho w, rho)."""
+    return [float(np.abs(got[f] - want[f]).max() / np.abs(want[f]).max())
+            for f in range(5)]
+
+
+def t16_roofline(nsys, N, KL, reps=5):
+    """The mixed precision path's dominant kernel, the single precision
+    transposed band solve (sixteen lanes per system), on nsys systems of
+    order N with KL = KU: one launch reads the U and L rows of every column
+    (N (2 KL + KL + 1) complex floats) and the pivots, and reads and writes
+    the right hand side.  Timed with HIP events on the stream it runs on
+    (torch's current stream) over `reps` launches; the factors are random
+    with a dominant diagonal and identity pivots (the kernel's work does not
+    depend on the values)."""
+    import torch
+    import suzerain_amd as sa
+    dev = torch.device("cuda:0")
+    ldab = 3 * KL + 1
+    lu = torch.randn((nsys, N, ldab), dtype=torch.complex64, device=dev)
+    lu[:, :, 2 * KL] += 4.0 * (2 * KL + 1)
+    ipiv = torch.arange(1, N + 1, dtype=torch.int32, device=dev).repeat(nsys, 1)
+    b = torch.randn((nsys, 1, N), dtype=torch.complex64, device=dev)
+    sa.gbtrs("T", lu, KL, KL, ipiv, b, precision="c")
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        sa.gbtrs("T", lu, KL, KL, ipiv, b, precision="c")
+    e1.record(st)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    bytes_sys = N * (3 * KL + 1) * 8 + N * 4 + 2 * N * 8
+    achieved = bytes_sys * nsys / (ms * 1e-3) / 1e9
+    del lu, ipiv, b
+    torch.cuda.empty_cache()
+    return {"bound": "hbm", "kernel": f"k_gbtrs_t16<float, {KL}, {2 * KL}, true, false>",
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "avg_launch_ms": round(ms, 4), "bytes_per_launch": bytes_sys * nsys,
+            "systems": nsys}
+
+
+def precision_workload(size, steps, dt):
+    """The precision form on the channel substep at size^3 (dealiased
+    3/2 size, k = 8, the rk3 workload's physics): the plain FP64 path
+    (zgbsv every substep) and the FP32-refined path (float factors reused
+    across steps per substep slot, FP64 residual sweeps) advance the same
+    initial state with the same fixed time step; one untimed step first
+    (it holds the refined path's factorizations), then `steps` timed."""
+    import torch
+    from suzerain_amd.channel import Channel, laminar_state
+    cfg = dict(Nx=size, Ny=size, Nz=size, dNx=3 * size // 2, dNz=3 * size // 2,
+               k=8, htdelta=3.0)
+    out = {"size": [size, size, size], "dealiased": [3 * size // 2] * 2,
+           "spec": PREC_SPEC, "dt": dt, "steps_timed": steps}
+    states, rates = {}, {}
+    for name, spec in (("fp64", "zgbsv"), ("refined", PREC_SPEC)):
+        ch = Channel(**cfg, solver=spec)
+        if name == "fp64":
+            W0 = torch.from_numpy(laminar_state(ch)).to("cuda:0")
+            out["systems"], out["N"] = ch.nsys, 5 * size
+        ch.set_state(W0)
+        got = ch.step(max_dt=dt)
+        assert got == dt, (got, dt)
+        first = ch.get_state(0).cpu().numpy()
+        torch.cuda.synchronize()
+        ch.substep_timing(enable=1)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            assert ch.step(max_dt=dt) == dt
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        sub = ch.substep_timing(enable=0)
+        rates[name] = el / steps
+        states[name] = (first, ch.get_state(0).cpu().numpy())
+        if name == "refined":
+            si = ch.solver_stats("siter")
+            out["siter_last_substep"] = {"min": int(si.min()), "mean": round(float(si.mean()), 3),
+                                         "max": int(si.max())}
+            out["restarted_or_double_last_substep"] = list(ch.solver_counts())[1:]
+            out["latency_ms_per_substep"] = percentiles(sub)
+        else:
+            out["fp64_latency_ms_per_substep"] = percentiles(sub)
+        del ch
+        torch.cuda.empty_cache()
+    out["fp64_ms_per_step"] = round(rates["fp64"] * 1e3, 3)
+    out["ms_per_step"] = round(rates["refined"] * 1e3, 3)
+    out["substeps_per_s"] = round(3 / rates["refined"], 3)
+    out["fp64_substeps_per_s"] = round(3 / rates["fp64"], 3)
+    out["implicit_solves_per_s"] = round(3 * out["systems"] / rates["refined"], 1)
+    out["speedup_vs_fp64"] = round(rates["fp64"] / rates["refined"], 3)
+    out["err_per_field_first_step"] = ["%.2e" % e for e in
+                                       field_errors(states["refined"][0], states["fp64"][0])]
+    out["err_per_field_after_all_steps"] = ["%.2e" % e for e in
+                                            field_errors(states["refined"][1], states["fp64"][1])]
+    out["roofline"] = t16_roofline(out["systems"], out["N"], 34)
+    return out
+
+
+def ensemble_workload(count, steps):
+    """The batched uneven ensemble: `count` members of the seeded uncertainty
+    sweep advance together (explicit stages on the autotuned streams, all
+    members' implicit solves in the grouped FP32-refined solver, the step
+    replayed from a captured hipGraph); against it, the plain FP64 drop-in
+    steps the same members one by one.  Both start from the same states and
+    take the same number of steps (2 untimed, then `steps` timed)."""
+    import torch
+    from suzerain_amd.channel import Channel, laminar_state
+    from suzerain_amd.ensemble import Ensemble, uncertainty_sweep, default_dt
+    members = uncertainty_sweep(count)
+    W = []
+    for c in members:
+        ch = Channel(**c, solver="none")
+        W.append(laminar_state(ch, seed=7 + c["Ny"]))
+        del ch
+    ens = Ensemble(members, graph=True)
+    for m in range(count):
+        ens.set_state(m, torch.from_numpy(W[m]).to("cuda:0"))
+    ens.step(1)                          # factorizations and the first capture
+    tuning = ens.autotune(1)
+    ens.step(1)                          # captured again with the tuned choice
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lat = []
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        ens.step(1)
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - t1) * 1e3)
+    el = time.perf_counter() - t0
+    ens.check()
+    nsys = sum(((c["Nx"] - 1) // 2 + 1) * (2 * ((c["Nz"] - 1) // 2) + 1) for c in members)
+    out = {"members": count, "systems": nsys, "resident_gb": round(ens.bytes() / 1e9, 1),
+           "spec": PREC_SPEC, "graph": True, "tuning": tuning, "steps_timed": steps,
+           "ms_per_step": round(el / steps * 1e3, 2),
+           "member_substeps_per_s": round(3 * count * steps / el, 2),
+           "step_latency_ms": percentiles(lat)}
+    finals = [ens.get_state(m).cpu().numpy() for m in range(count)]
+    del ens
+    torch.cuda.empty_cache()
+    fp64 = 0.0
+    errs = np.zeros(5)
+    for m, c in enumerate(members):
+        dt = default_dt(c)
+        ch = Channel(**c, solver="zgbsv")
+        ch.set_state(torch.from_numpy(W[m]).to("cuda:0"))
+        for _ in range(2):
+            ch.step(max_dt=dt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ch.step(max_dt=dt)
+        torch.cuda.synchronize()
+        fp64 += time.perf_counter() - t0
+        errs = np.maximum(errs, field_errors(finals[m], ch.get_state(0).cpu().numpy()))
+        del ch
+        torch.cuda.empty_cache()
+    out["fp64_ms_per_step"] = round(fp64 / steps * 1e3, 2)
+    out["fp64_member_substeps_per_s"] = round(3 * count * steps / fp64, 2)
+    out["speedup_vs_fp64"] = round(fp64 / el, 3)
+    out["err_per_field_max_over_members"] = ["%.2e" % e for e in errs]
+    return out
+
+
+def serving_requests(nreq, rate, steps_range, seed=20261204, large_frac=0.05,
+                     large=(192, 192, 192)):
+    """Open-loop Poisson arrivals at `rate` per second: each request is a
+    channel drawn from the uncertainty sweep's distribution (or, with
+    probability large_frac, the large case) with a step count ~ U(steps_range)
+    and the sweep's initial-condition recipe."""
+    from suzerain_amd.serving import poisson_trace
+    from suzerain_amd.ensemble import uncertainty_sweep
+    from suzerain_amd.channel import DEFAULT_SCENARIO
+
+    def make(i, rng):
+        if rng.random() < large_frac:
+            cfg = dict(Nx=large[0], Ny=large[1], Nz=large[2], k=8, htdelta=3.0,
+                       scenario=dict(DEFAULT_SCENARIO))
+        else:
+            cfg = uncertainty_sweep(1, seed=int(rng.integers(1 << 30)))[0]
+        steps = int(rng.integers(steps_range[0], steps_range[1] + 1))
+        return cfg, steps, _initial_state(cfg)
+    return poisson_trace(rate, nreq, make, seed=seed)
+
+
+def _initial_state(cfg):
+    def init():
+        import torch
+        from suzerain_amd.channel import Channel, laminar_state
+        ch = Channel(**cfg, solver="none")
+        W = laminar_state(ch, seed=7 + cfg["Ny"])
+        del ch
+        return torch.from_numpy(W).to("cuda:0")
+    return init
+
+
+def serving_workload(nreq, rate, steps_range, budget=240e9):
+    """Continuous batching of the trace within the HBM budget (FP32-refined,
+    captured steps), then the plain FP64 drop-in serving the same trace
+    (each request its own channel, live ones stepped round robin); final
+    states compared request by request."""
+    import torch
+    from suzerain_amd.serving import Server, Fp64Server
+    out = {"requests": nreq, "rate_per_s": rate, "steps_range": list(steps_range),
+           "budget_gb": budget / 1e9, "spec": PREC_SPEC}
+    reqs = serving_requests(nreq, rate, steps_range)
+    out["large_requests"] = sum(r.cfg["Nx"] == 192 for r in reqs)
+    out["member_steps"] = sum(r.steps for r in reqs)
+    rep = Server(budget_bytes=budget, solver=PREC_SPEC, graph=True).run(reqs)
+    torch.cuda.empty_cache()
+    ref = serving_requests(nreq, rate, steps_range)
+    rep64 = Fp64Server(budget_bytes=budget).run(ref)
+    torch.cuda.empty_cache()
+    errs = np.zeros(5)
+    for a, b in zip(sorted(reqs, key=lambda r: r.rid), sorted(ref, key=lambda r: r.rid)):
+        errs = np.maximum(errs, field_errors(a.result, b.result))
+    out.update({"refined": rep, "fp64": rep64,
+                "speedup_substeps_per_s": round(rep["substeps_per_s"] / rep64["substeps_per_s"], 3),
+                "err_per_field_max_over_requests": ["%.2e" % e for e in errs]})
+    return out
 
 
 def main_rk3(args):
@@ -469,6 +714,8 @@ def main_rk3(args):
             out["roofline"]["traffic_passes"] = t["passes"]
         else:
             out["roofline"]["traffic_note"] = why
+    if not args.no_extra:
+        out.update(extra_keys(args))
     if not args.no_cpu:
         threads, nproc = host_threads()
         ch = Channel(**RK3_CONFIG)
@@ -484,6 +731,32 @@ def main_rk3(args):
                       f"ZGBTRF/ZGBTRS (scipy's OpenBLAS), per-wavenumber "
                       f"work on {threads} threads"}
     print(json.dumps(out), flush=True)
+
+
+PREC_DT = {96: 0.02, 192: 0.01}
+
+
+def extra_keys(args):
+    """The precision form (96^3 and 192^3), the 48-member ensemble and the
+    serving trace, each against the plain FP64 drop-in on the same work."""
+    out = {}
+    for size in [int(x) for x in args.prec_sizes.split(",") if x]:
+        r = precision_workload(size, args.prec_steps, PREC_DT.get(size, 0.01))
+        if not args.no_traffic and size == min(int(x) for x in args.prec_sizes.split(",") if x):
+            t, why = pmc_traffic("k_gbtrs_t16", ["--pmc-child",
+                                                 f"t16:{r['systems']}:{r['N']}"])
+            r["roofline"]["traffic"] = t["bytes"] if t else None
+            if t:
+                r["roofline"]["traffic_passes"] = t["passes"]
+            else:
+                r["roofline"]["traffic_note"] = why
+        out[f"precision_{size}"] = r
+    if args.members:
+        out["ensemble"] = ensemble_workload(args.members, args.ens_steps)
+    if args.requests:
+        lo, hi = (int(x) for x in args.req_steps.split(","))
+        out["serving"] = serving_workload(args.requests, args.rate, (lo, hi))
+    return out
 
 
 def main():
@@ -502,6 +775,14 @@ def main():
         return
     if args.workload == "rk3":
         main_rk3(args)
+        return
+    if args.workload in ("precision", "ensemble", "serving"):
+        keep = {"precision": ("members", "requests"),
+                "ensemble": ("prec_sizes", "requests"),
+                "serving": ("prec_sizes", "members")}[args.workload]
+        for k in keep:
+            setattr(args, k, "" if k == "prec_sizes" else 0)
+        print(json.dumps(extra_keys(args)), flush=True)
         return
     out = solve_workload(args.steps, args.warmup, cpu=not args.no_cpu)
     out.update({"n_gpus": 1, "higher_is_better": True, "scaling": "weak",

@@ -499,7 +499,8 @@ int sz_ensemble_check(sz_ensemble *e);
 void *sz_ensemble_solver_stats(sz_ensemble *e, int which);
 /* Autotune the explicit-stage stream count and the factorization unroll by
  * timing `steps` eager steps per candidate from the same saved states; the
- * states are restored afterwards.  Call before the first graph step. */
+ * states are restored afterwards; a captured step is dropped and captured
+ * again at the next step. */
 int sz_ensemble_autotune(sz_ensemble *e, int steps);
 int sz_ensemble_tuning(const sz_ensemble *e, int *nstreams, int *lu_bs,
                        double *trial_ms, int max, int *ntrials);

@@ -495,8 +495,13 @@ int sz_ensemble_autotune(sz_ensemble *e, int steps)
 {
     if (!e) return -1;
     if (steps < 1) return -2;
-    if (e->exec) return -3;            // tune before the first captured step
     if (order_after(e->sync_ev, e->user, e->main)) return SZ_ERR_LAUNCH;
+    // a captured step holds the old stream assignment: retire it (the
+    // next step captures again with the tuned choice)
+    if (e->exec) {
+        if (hipStreamSynchronize(e->main) != hipSuccess) return SZ_ERR_LAUNCH;
+        e->drop_graph();
+    }
     const size_t nm = e->ch.size();
     std::vector<void *> saved(nm, nullptr);
     int rc = 0;

@@ -225,6 +225,28 @@ __device__ inline zd weight_staged(const int *gt0, const int *gnt, const int *tr
     return zd{wx, wy};
 }
 
+// weight_staged with the reference values staged in LDS, refl[q * n + r] =
+// ref_q[r]: the same arithmetic without a global gather per term.
+__device__ inline zd weight_lds(const int *gt0, const int *gnt, const int *tref,
+                                const double *refl, int n, const zd *alpha, int g,
+                                int r)
+{
+    const int t0 = gt0[g], nt = gnt[g];
+    double wx = 0, wy = 0;
+    for (int q = 0; q < nt; ++q) {
+        const int t = t0 + q, rt = tref[t];
+        const double v = rt == ONE ? 1.0 : refl[rt * n + r];
+        if (q == 0) {
+            wx = alpha[t].x * v;
+            wy = alpha[t].y * v;
+        } else {
+            wx = fma(alpha[t].x, v, wx);
+            wy = fma(alpha[t].y, v, wy);
+        }
+    }
+    return zd{wx, wy};
+}
+
 struct bops {
     const double *dt;     // derivative d's transposed band at dt + d * dstride
     long long dstride;
@@ -236,6 +258,20 @@ struct wallbc {
     int enforce[2];       // lower, upper
     double fac[2][4];     // E_factor, u, v, w per wall
 };
+
+// refl[q * n + y] = ref_q[y] for every point y (0 for an absent profile),
+// from the profile columns rp/rld already staged in LDS; the block's
+// threads share the 26n loads (a few each).
+template <int NTH>
+__device__ __forceinline__ void stage_refs(const double *const *rp, const int *rld,
+                                           int n, double *refl)
+{
+    for (int e = threadIdx.x; e < NREF * n; e += NTH) {
+        const int q = e / n, y = e - q * n;
+        const double *p = rp[q];
+        refl[e] = p ? p[(long long)y * rld[q]] : 0.0;
+    }
+}
 
 constexpr int PK_NT = 256;
 constexpr int PK_RB = 4;  // wall-normal rows (5 columns each) per workgroup
@@ -486,11 +522,16 @@ void k_imexop_pack_grouped(const gmember *__restrict__ mem, int nm,
 // the x, D x buffers take most of a CU's LDS (one workgroup per CU), so the
 // workgroup is large enough to hide the latency of its own loads
 constexpr int AC_NT = 1024;
+// LDS of the reference profiles staged at every point (order n), which the
+// accumulate and apply kernels use when they have room for it (the sweep
+// keeps the gather: the extra LDS would cost it a resident workgroup).
+#define REFL_BYTES(n) (sizeof(double) * NREF * (size_t)(n))
 
 // y_i <- (M + phi L) x + beta y_i for one system per workgroup.  Field f of
 // system s starts at in[f] + inoff(s) (complex elements, stride 1 in y).
 // The body of one accumulate workgroup: field j of x at in[j] (null: absent),
 // field i of y at out[i], wavenumbers kms, kns.
+template <bool STAGED>
 __device__ __forceinline__ void
 acc_body(const table *__restrict__ tab, const bops &w, const refs &rf, zd phi,
          double kms, double kns, const zd *const (&in)[5], zd *const (&out)[5],
@@ -501,6 +542,7 @@ acc_body(const table *__restrict__ tab, const bops &w, const refs &rf, zd phi,
     zd *alpha = reinterpret_cast<zd *>(smem);       // MAXTERMS
     zd *xs = alpha + MAXTERMS;                      // 5 x n
     zd *Dx = xs + 5 * n;                            // 3 x 5 x n
+    double *refl = reinterpret_cast<double *>(Dx + 15 * n);   // NREF x n if STAGED
     const int tid = threadIdx.x;
     __shared__ int sbg0[25], sbng[25], sgop[MAXGROUPS], sgt0[MAXGROUPS], sgnt[MAXGROUPS];
     __shared__ int stref[MAXTERMS], srld[NREF];
@@ -532,6 +574,7 @@ acc_body(const table *__restrict__ tab, const bops &w, const refs &rf, zd phi,
         xs[e] = in[f] ? in[f][io + y] : zd{0, 0};
     }
     __syncthreads();
+    if constexpr (STAGED) stage_refs<AC_NT>(srp, srld, n, refl);
     for (int e = tid; e < 15 * n; e += AC_NT) {
         const int op = e / (5 * n), rem = e - op * 5 * n;
         const int f = rem / n, r = rem - f * n;
@@ -555,7 +598,8 @@ acc_body(const table *__restrict__ tab, const bops &w, const refs &rf, zd phi,
             if (!in[j]) continue;
             const int b = i * 5 + j, g0 = sbg0[b], ng = sbng[b];
             for (int g = g0; g < g0 + ng; ++g) {
-                const zd W = weight_staged(sgt0, sgnt, stref, srp, srld, alpha, g, r);
+                const zd W = STAGED ? weight_lds(sgt0, sgnt, stref, refl, n, alpha, g, r)
+                                    : weight_staged(sgt0, sgnt, stref, srp, srld, alpha, g, r);
                 acc = sz::cfma(acc, W, Dx[(sgop[g] * 5 + j) * n + r]);
             }
         }
@@ -567,6 +611,7 @@ acc_body(const table *__restrict__ tab, const bops &w, const refs &rf, zd phi,
     }
 }
 
+template <bool STAGED>
 __global__ __launch_bounds__(AC_NT)
 void k_imexop_accumulate(const table *__restrict__ tab, bops w, refs rf, zd phi,
                          const double *__restrict__ km,
@@ -588,7 +633,7 @@ void k_imexop_accumulate(const table *__restrict__ tab, bops w, refs rf, zd phi,
     zd *const out[5] = {out0 ? out0 + oo : nullptr, out1 ? out1 + oo : nullptr,
                         out2 ? out2 + oo : nullptr, out3 ? out3 + oo : nullptr,
                         out4 ? out4 + oo : nullptr};
-    acc_body(tab, w, rf, phi, km[s], kn[s], in, out, beta, beta_zero);
+    acc_body<STAGED>(tab, w, rf, phi, km[s], kn[s], in, out, beta, beta_zero);
 }
 
 __device__ __forceinline__ double block_sum_ac(double v, double *red)
@@ -730,6 +775,7 @@ void k_imexop_sweep_grouped(const gmember *__restrict__ mem, int nm,
 // Grouped y = (M + phi L) x of every member's systems (one block each) on
 // the compact (5, n) vectors at x + v0 + ls*5n, y likewise; systems with
 // skip[s] != 0 left alone.
+template <bool STAGED>
 __global__ __launch_bounds__(AC_NT)
 void k_imexop_apply_grouped(const gmember *__restrict__ mem, int nm,
                             const zd *__restrict__ x, zd *__restrict__ y,
@@ -745,7 +791,7 @@ void k_imexop_apply_grouped(const gmember *__restrict__ mem, int nm,
                              x + o + 4 * n};
     zd *const out[5] = {y + o, y + o + n, y + o + 2 * n, y + o + 3 * n,
                         y + o + 4 * n};
-    acc_body(M.tab, M.w, M.rf, M.phi, M.km[ls], M.kn[ls], in, out, zd{0, 0}, 1);
+    acc_body<STAGED>(M.tab, M.w, M.rf, M.phi, M.km[ls], M.kn[ls], in, out, zd{0, 0}, 1);
 }
 
 // Zero the permuted right hand side entries of the replaced wall equations.
@@ -988,10 +1034,13 @@ int sz_internal_imexop_accumulate_masked(
     if (nsys < 0) return -14;
     if (nsys == 0) return 0;
     const bops w = make_bops(op->w);
-    const size_t lds = sizeof(zd) * (MAXTERMS + 20 * (size_t)w.n);
+    size_t lds = sizeof(zd) * (MAXTERMS + 20 * (size_t)w.n);
     if (lds > 156 * 1024) return SZ_ERR_LDS;   // 160 KB less the staged tables
+    const bool staged = lds + REFL_BYTES(w.n) <= 156 * 1024;
+    if (staged) lds += REFL_BYTES(w.n);
     const int bz = beta[0] == 0 && beta[1] == 0;
-    hipLaunchKernelGGL(k_imexop_accumulate, dim3(nsys), dim3(AC_NT), lds,
+    hipLaunchKernelGGL((staged ? k_imexop_accumulate<true> : k_imexop_accumulate<false>),
+                       dim3(nsys), dim3(AC_NT), lds,
                        (hipStream_t)stream, op->dev, w, make_refs(r, ld),
                        zd{phi[0], phi[1]}, km, kn, (const zd *)in[0],
                        (const zd *)in[1], (const zd *)in[2], (const zd *)in[3],
@@ -1118,9 +1167,12 @@ int sz_internal_imexop_apply_grouped(const void *dmem, int nm, int nsys,
                                      const int *skip, void *stream)
 {
     if (nsys <= 0) return 0;
-    const size_t lds = sizeof(zd) * (MAXTERMS + 20 * (size_t)nmax);
+    size_t lds = sizeof(zd) * (MAXTERMS + 20 * (size_t)nmax);
     if (lds > 156 * 1024) return SZ_ERR_LDS;
-    hipLaunchKernelGGL(k_imexop_apply_grouped, dim3(nsys), dim3(AC_NT), lds,
+    const bool staged = lds + REFL_BYTES(nmax) <= 156 * 1024;
+    if (staged) lds += REFL_BYTES(nmax);
+    hipLaunchKernelGGL((staged ? k_imexop_apply_grouped<true> : k_imexop_apply_grouped<false>),
+                       dim3(nsys), dim3(AC_NT), lds,
                        (hipStream_t)stream, (const gmember *)dmem, nm,
                        (const zd *)x, (zd *)y, skip);
     return launch_ok() ? 0 : SZ_ERR_LAUNCH;
@@ -1133,6 +1185,8 @@ int sz_internal_imexop_sweep_grouped(const void *dmem, int nm, int nsys,
                                      void *stream)
 {
     if (nsys <= 0) return 0;
+    // the sweep keeps the gather: staging the profiles would take the LDS
+    // of a third resident block per CU, which costs more than it saves
     const size_t lds = sizeof(zd) * (MAXTERMS + 25 * (size_t)nmax);
     if (lds > 150 * 1024) return SZ_ERR_LDS;
     hipLaunchKernelGGL(k_imexop_sweep_grouped, dim3(nsys), dim3(AC_NT), lds,
