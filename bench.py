@@ -544,6 +544,7 @@ def ensemble_workload(count, steps):
     for m in range(count):
         ens.set_state(m, torch.from_numpy(W[m]).to("cuda:0"))
     ens.step(1)                          # factorizations and the first capture
+    first = [ens.get_state(m).cpu().numpy() for m in range(count)]
     tuning = ens.autotune(1)
     ens.step(1)                          # captured again with the tuned choice
     torch.cuda.synchronize()
@@ -566,13 +567,14 @@ def ensemble_workload(count, steps):
     del ens
     torch.cuda.empty_cache()
     fp64 = 0.0
-    errs = np.zeros(5)
+    errs, errs1 = np.zeros(5), np.zeros(5)
     for m, c in enumerate(members):
         dt = default_dt(c)
         ch = Channel(**c, solver="zgbsv")
         ch.set_state(torch.from_numpy(W[m]).to("cuda:0"))
-        for _ in range(2):
-            ch.step(max_dt=dt)
+        ch.step(max_dt=dt)
+        errs1 = np.maximum(errs1, field_errors(first[m], ch.get_state(0).cpu().numpy()))
+        ch.step(max_dt=dt)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
@@ -585,7 +587,12 @@ def ensemble_workload(count, steps):
     out["fp64_ms_per_step"] = round(fp64 / steps * 1e3, 2)
     out["fp64_member_substeps_per_s"] = round(3 * count * steps / fp64, 2)
     out["speedup_vs_fp64"] = round(fp64 / el, 3)
-    out["err_per_field_max_over_members"] = ["%.2e" % e for e in errs]
+    out["err_per_field_first_step"] = ["%.2e" % e for e in errs1]
+    out["err_per_field_after_all_steps"] = ["%.2e" % e for e in errs]
+    out["err_note"] = ("max over members of max|refined - fp64| / max|fp64| per field: "
+                       "after one step from the same state, and after all "
+                       f"{steps + 2} steps (the two trajectories drift apart "
+                       "by the refined solve's residual floor each step)")
     return out
 
 
@@ -638,12 +645,18 @@ def serving_workload(nreq, rate, steps_range, budget=240e9):
     ref = serving_requests(nreq, rate, steps_range)
     rep64 = Fp64Server(budget_bytes=budget).run(ref)
     torch.cuda.empty_cache()
-    errs = np.zeros(5)
+    errs, per_step = np.zeros(5), np.zeros(5)
     for a, b in zip(sorted(reqs, key=lambda r: r.rid), sorted(ref, key=lambda r: r.rid)):
-        errs = np.maximum(errs, field_errors(a.result, b.result))
+        e = np.asarray(field_errors(a.result, b.result))
+        errs = np.maximum(errs, e)
+        per_step = np.maximum(per_step, e / a.steps)
     out.update({"refined": rep, "fp64": rep64,
                 "speedup_substeps_per_s": round(rep["substeps_per_s"] / rep64["substeps_per_s"], 3),
-                "err_per_field_max_over_requests": ["%.2e" % e for e in errs]})
+                "err_per_field_final_max_over_requests": ["%.2e" % e for e in errs],
+                "err_per_field_per_step_max_over_requests": ["%.2e" % e for e in per_step],
+                "err_note": f"final states after each request's {steps_range[0]}.."
+                            f"{steps_range[1]} steps; per step: that error divided "
+                            "by the request's step count"})
     return out
 
 

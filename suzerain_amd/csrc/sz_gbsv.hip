@@ -932,14 +932,14 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
     };
     auto band = [&](int i, int c) __attribute__((always_inline)) -> T {
         const bool ok = i < n && c < n && c - i <= ku && i - c <= kl;
-        const T *p = ab + (ok ? (size_t)c * ldab + KV + i - c : 0);
+        const auto p = sz::gmem(ab + (ok ? (size_t)c * ldab + KV + i - c : 0));
         const R x = p->x, y = p->y;
         return T{ok ? x : R(0), ok ? y : R(0)};
     };
     // raw element (clamped address; validity is applied when consumed)
     auto raw = [&](int i, int c) __attribute__((always_inline)) -> T {
         const bool ok = i < n && c < n;
-        return ab[ok ? (size_t)c * ldab + KV + i - c : 0];
+        return sz::gload(ab + (ok ? (size_t)c * ldab + KV + i - c : 0));
     };
 
     T a[WR];
@@ -966,12 +966,12 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
     constexpr int BNEW = NTAIL > 0 ? KV + 1 : 64;   // column j+BNEW enters at step j
     T breg{R(0), R(0)}, qb[WV_PF];
     if constexpr (UT) {
-        breg = sel0(lane < n, bv[lane < n ? lane : 0]);
+        breg = sel0(lane < n, sz::gload(bv + (lane < n ? lane : 0)));
         if (NTAIL > 0 && lane < NTAIL) {
-            tailb[lane] = sel0(64 + lane < n, bv[64 + lane < n ? 64 + lane : 0]);
+            tailb[lane] = sel0(64 + lane < n, sz::gload(bv + (64 + lane < n ? 64 + lane : 0)));
         }
 #pragma unroll
-        for (int u = 0; u < WV_PF; ++u) qb[u] = bv[min(u + BNEW, n - 1)];
+        for (int u = 0; u < WV_PF; ++u) qb[u] = sz::gload(bv + min(u + BNEW, n - 1));
     }
     WAVE_SYNC();
 
@@ -1040,7 +1040,7 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
             const bool nz = sz::cnonzero(piv);
             if (nz) ju = max(ju, min(j + ku + jp, n - 1));
             else if (first_zero == 0) first_zero = j + 1;
-            ipiv[j] = j + jp + 1;
+            *sz::gmem(ipiv + j) = j + jp + 1;
 
             // ---- multipliers (row jp now holds old row j)
             T l = v;
@@ -1061,7 +1061,7 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
                 T st;
                 st.x = isl ? l.x : piv.x;
                 st.y = isl ? l.y : piv.y;
-                ab[isl ? ujj + lane : ujj] = st;
+                sz::gstore(ab + (isl ? ujj + lane : ujj), st);
             }
 
             // ---- interchange rows j and j+jp in columns j .. ju
@@ -1136,7 +1136,7 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
                 T st;
                 st.x = ok ? uc.x : piv.x;
                 st.y = ok ? uc.y : piv.y;
-                ab[ok ? (size_t)cg * ldab + KV - d : ujj] = st;
+                sz::gstore(ab + (ok ? (size_t)cg * ldab + KV - d : ujj), st);
             }
             if constexpr (NTAIL > 0) {
                 int ts = th + lane;
@@ -1146,7 +1146,7 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
                 T st;
                 st.x = ok ? tv.x : piv.x;
                 st.y = ok ? tv.y : piv.y;
-                ab[ok ? (size_t)(j + 64 + lane) * ldab + KV - 64 - lane : ujj] = st;
+                sz::gstore(ab + (ok ? (size_t)(j + 64 + lane) * ldab + KV - 64 - lane : ujj), st);
             }
 
             // ---- U^T solve: y_j, then U(j, j+d) y_j out of every pending b
@@ -1155,7 +1155,7 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
                 bj.x = readlane_r(breg.x, c0);
                 bj.y = readlane_r(breg.y, c0);
                 const T y = sz::cdiv(bj, piv);       // U(j, j) = piv
-                bv[j] = y;
+                sz::gstore(bv + j, y);
                 if (d >= 1 && d <= KV) breg = sz::cfnms(breg, uc, y);
                 if constexpr (NTAIL > 0) {
                     if (lane < NTAIL && j + 64 + lane < n) {
@@ -1257,7 +1257,7 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
                 th = th1;
             }
             if constexpr (UT)
-                qb[u % WV_PF] = bv[min(j + WV_PF + BNEW, n - 1)];
+                qb[u % WV_PF] = sz::gload(bv + min(j + WV_PF + BNEW, n - 1));
             WAVE_SYNC();
             c0 = (c0 + 1) & 63;
         }
@@ -1269,7 +1269,7 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
         for (int s = 0; s < WR; ++s) a[s] = t[s];
         ob = ob + WV_BS >= WR ? ob + WV_BS - WR : ob + WV_BS;
     }
-    if (lane == 0 && info) info[sys] = first_zero;
+    if (lane == 0 && info) *sz::gmem(info + sys) = first_zero;
 }
 
 // 1 when a batch of one-wave factorizations leaves some SIMDs with two waves

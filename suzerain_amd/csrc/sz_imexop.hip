@@ -205,7 +205,7 @@ __device__ inline zd weight_staged(const int *gt0, const int *gnt, const int *tr
             v[q] = 0;
             if (tb + q < nt) {
                 const int rt = tref[t0 + tb + q];
-                v[q] = rt == ONE ? 1.0 : rp[rt][(long long)r * rld[rt]];
+                v[q] = rt == ONE ? 1.0 : *sz::gmem(rp[rt] + (long long)r * rld[rt]);
             }
         }
 #pragma unroll
@@ -269,7 +269,7 @@ __device__ __forceinline__ void stage_refs(const double *const *rp, const int *r
     for (int e = threadIdx.x; e < NREF * n; e += NTH) {
         const int q = e / n, y = e - q * n;
         const double *p = rp[q];
-        refl[e] = p ? p[(long long)y * rld[q]] : 0.0;
+        refl[e] = p ? *sz::gmem(p + (long long)y * rld[q]) : 0.0;
     }
 }
 
@@ -426,10 +426,10 @@ pack_body(const table *__restrict__ tab, const bops &w, const refs &rf, zd phi,
                 v = R == C ? sc : (R == rhoR ? zd{-sc.x * f, -sc.y * f} : zd{0, 0});
                 if (bcs && R == C) {
                     const OUT o = to_out<OUT>(sc);
-                    bcs[(lo ? 0 : 4) + fi] = zd{(double)o.x, (double)o.y};
+                    sz::gstore(bcs + (lo ? 0 : 4) + fi, zd{(double)o.x, (double)o.y});
                 }
             }
-            P[(size_t)C * ldpapt + KU + R - C] = to_out<OUT>(v);
+            sz::gstore(P + (size_t)C * ldpapt + KU + R - C, to_out<OUT>(v));
             if (frob) fsum = fma(v.x, v.x, fma(v.y, v.y, fsum));
         }
         rr += drr;
@@ -571,7 +571,7 @@ acc_body(const table *__restrict__ tab, const bops &w, const refs &rf, zd phi,
     }
     for (int e = tid; e < 5 * n; e += AC_NT) {
         const int f = e / n, y = e - f * n;
-        xs[e] = in[f] ? in[f][io + y] : zd{0, 0};
+        xs[e] = in[f] ? sz::gload(in[f] + io + y) : zd{0, 0};
     }
     __syncthreads();
     if constexpr (STAGED) stage_refs<AC_NT>(srp, srld, n, refl);
@@ -583,7 +583,7 @@ acc_body(const table *__restrict__ tab, const bops &w, const refs &rf, zd phi,
         double tx = 0, ty = 0;
         const int c1 = min(n - 1, r + w.kl[op]);
         for (int c = max(0, r - w.ku[op]); c <= c1; ++c) {
-            const double a = row[c];
+            const double a = *sz::gmem(row + c);
             tx = fma(a, x[c].x, tx);
             ty = fma(a, x[c].y, ty);
         }
@@ -604,10 +604,10 @@ acc_body(const table *__restrict__ tab, const bops &w, const refs &rf, zd phi,
             }
         }
         zd *yp = out[i] + oo + r;
-        zd by = beta_zero ? zd{0, 0} : sz::cmul(beta, *yp);
+        zd by = beta_zero ? zd{0, 0} : sz::cmul(beta, sz::gload(yp));
         const zd sm{by.x + acc.x, by.y + acc.y};
         const zd m = Dx[(M * 5 + i) * n + r];
-        *yp = zd{sm.x + m.x, sm.y + m.y};
+        sz::gstore(yp, zd{sm.x + m.x, sm.y + m.y});
     }
 }
 
@@ -706,10 +706,11 @@ void k_imexop_sweep_grouped(const gmember *__restrict__ mem, int nm,
     }
     for (int e = tid; e < N; e += AC_NT) {
         const int f = e / n, y = e - f * n;
-        const sz::zf c = pb[y * 5 + f];          // (P^T c)_e = c_{q^-1(e)}
-        const zd v = zd{x[e].x + (double)c.x, x[e].y + (double)c.y};
+        const sz::zf c = sz::gload(pb + y * 5 + f);   // (P^T c)_e = c_{q^-1(e)}
+        const zd xo = sz::gload(x + e);
+        const zd v = zd{xo.x + (double)c.x, xo.y + (double)c.y};
         xs[e] = v;
-        x[e] = v;
+        sz::gstore(x + e, v);
     }
     __syncthreads();
     for (int e = tid; e < 15 * n; e += AC_NT) {
@@ -720,7 +721,7 @@ void k_imexop_sweep_grouped(const gmember *__restrict__ mem, int nm,
         double tx = 0, ty = 0;
         const int c1 = min(n - 1, r + w.kl[op]);
         for (int c = max(0, r - w.ku[op]); c <= c1; ++c) {
-            const double av = row[c];
+            const double av = *sz::gmem(row + c);
             tx = fma(av, xf[c].x, tx);
             ty = fma(av, xf[c].y, ty);
         }
@@ -733,7 +734,7 @@ void k_imexop_sweep_grouped(const gmember *__restrict__ mem, int nm,
         const int wall = r == 0 ? 0 : (r == n - 1 ? 1 : -1);
         zd v;
         if (i < 4 && wall >= 0 && M.bc.enforce[wall]) {
-            const zd sc = M.bcs[(size_t)ls * 8 + wall * 4 + i];
+            const zd sc = sz::gload(M.bcs + (size_t)ls * 8 + wall * 4 + i);
             const double fac = M.bc.fac[wall][i];
             const zd xr = xs[4 * n + r];
             const zd c = zd{xs[e].x - fac * xr.x, xs[e].y - fac * xr.y};
@@ -750,7 +751,8 @@ void k_imexop_sweep_grouped(const gmember *__restrict__ mem, int nm,
             }
             const zd m = Dx[(0 * 5 + i) * n + r];    // the mass operator M
             const zd t = zd{acc.x + m.x, acc.y + m.y};
-            v = zd{b[e].x - t.x, b[e].y - t.y};
+            const zd be = sz::gload(b + e);
+            v = zd{be.x - t.x, be.y - t.y};
         }
         rr[e] = v;
         ss += v.x * v.x + v.y * v.y;
@@ -763,7 +765,7 @@ void k_imexop_sweep_grouped(const gmember *__restrict__ mem, int nm,
         __syncthreads();
         for (int i = tid; i < N; i += AC_NT) {
             const zd v = rr[(i % 5) * n + i / 5];     // (P r)_i = r_{q(i)}
-            pb[i] = sz::zf{(float)v.x, (float)v.y};
+            sz::gstore(pb + i, sz::zf{(float)v.x, (float)v.y});
         }
     }
     if (tid == 0) {
