@@ -140,6 +140,10 @@ int sz_bsplineop_max_ku(const sz_bsplineop *w);
 int sz_bsplineop_kl(const sz_bsplineop *w, int d);
 int sz_bsplineop_ku(const sz_bsplineop *w, int d);
 double sz_bsplineop_collocation_point(const sz_bsplineop *w, int i);
+/* Integral of each basis function over the breakpoint range: replaces
+ * suzerain_bspline_integration_coefficients (suzerain/bspline.c:223-282) with
+ * nderiv = 0 over the whole domain; n values. */
+int sz_bsplineop_integration_coefficients(const sz_bsplineop *w, double *coeff);
 const double *sz_bsplineop_D_T(const sz_bsplineop *w, int d);
 const double *sz_bsplineop_D_T_device(sz_bsplineop *w);
 int sz_bsplineop_accumulate_complex_batched(sz_bsplineop *w, int nderiv,
@@ -379,6 +383,14 @@ typedef struct sz_channel_spec {
     sz_isothermal_bc bc;     /* isothermal wall data                        */
     const char *solver;      /* specification_zgbsv string; NULL = "zgbsv" */
     int linearize;           /* SZ_LINEARIZE_RHOME_XYZ                     */
+    /* Bulk (integral) constraints on the (0,0) mode, apps/perfect/
+     * main_advance.cpp:259-272 with suzerain/treatment_constraint.cpp: when
+     * constrained is nonzero, bulk[f] is the target of (1/Ly) * integral of
+     * field f's mean over y for f = rho E, rho u, rho v, rho w, rho (the
+     * reference's bulk_rho_E, bulk_rho_u, -, -, bulk_rho); NaN disables one.
+     * Zero (the default of a zeroed spec): no constraint treatment. */
+    int constrained;
+    double bulk[5];
 } sz_channel_spec;
 typedef struct sz_channel sz_channel;
 
@@ -393,7 +405,13 @@ void *sz_channel_state(sz_channel *h, int which);
  * (46), 2 wave derivative fields (41), 3 wave values (5), 4 km, 5 kn,
  * 6 per-system pencil offsets, 7 lambda1_y then lambda2_y, 8 the
  * sz_bsplineop, 9 the suzerain_amd::bsmbsm_solver of slot 0, 10 the 12
- * step candidates of the last zeroth substep. */
+ * step candidates of the last zeroth substep; with bulk constraints, 11 the
+ * running means of the implicit forcing (17 x Ny: fx fy fz f_dot_u qb CrhoE
+ * C2rhoE Crhou C2rhou Crhov C2rhov Crhow C2rhow Crho C2rho Crhou_dot_u
+ * C2rhou_dot_u, treatment_constraint.cpp:226-316), 12 the 5 multipliers of
+ * the last constraint solve (per field, 0 when disabled), 13 the bulk
+ * weights (Ny), 14 the plane means (42 x Ny) of the last non-zeroth
+ * nonlinear evaluation (collect_instantaneous, perfect.cpp:1405-1480). */
 void *sz_channel_buffer(sz_channel *h, int which);
 /* operator_nonlinear::apply_operator (navier_stokes.hpp:107-986): state
  * `which` <- N(state) in place; on zeroth also the references and the 12
@@ -408,6 +426,18 @@ int sz_channel_apply_operator(sz_channel *h, double phi, int which);
 /* invert_mass_plus_scaled_operator (operator_hybrid_isothermal.cpp:528-689):
  * state <- (M + phi L)^{-1} state with isothermal walls; asynchronous. */
 int sz_channel_invert_operator(sz_channel *h, double phi, int which);
+/* invert_mass_plus_scaled_operator with the substep context lowstorage
+ * passes (operator_hybrid_isothermal.cpp:528-689 wrapped by
+ * constraint::treatment, suzerain/treatment_constraint.cpp:87-316): with
+ * bulk constraints, after the solve the (0,0) mode gets the combination of
+ * the operator's responses to the uniform forcing profiles (solved with the
+ * same factorization) whose bulk values meet the targets (least squares of
+ * the constraint matrix by singular value decomposition); delta_t > 0 also
+ * folds the scaled multipliers into the running means of the implicit
+ * forcing (sz_channel_buffer 11) as substep `substep_index` of 3. */
+int sz_channel_invert_operator_substep(sz_channel *h, double phi, int which,
+                                       int slot, double delta_t,
+                                       int substep_index);
 /* As sz_channel_invert_operator with factorization slot `slot`.  A channel
  * whose solver spec is zcgbsvx with reuse=true keeps three slots, one per
  * SMR91 substep: the factors made at substep i serve substep i of later steps

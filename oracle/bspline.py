@@ -121,6 +121,26 @@ def collocation(k, bp, nderiv):
     return {"D": D, "kl": kl, "ku": ku, "points": xi, "n": n, "knots": t}
 
 
+def integration_coefficients(k, bp):
+    """Integral over [bp[0], bp[-1]] of each basis function
+    (suzerain_bspline_integration_coefficients, suzerain/bspline.c:223-282,
+    nderiv = 0): a (k+1)//2-point Gauss-Legendre rule on every breakpoint
+    interval, the basis evaluated with scipy.interpolate.BSpline."""
+    t = knots(k, bp)
+    n = len(t) - k
+    xg, wg = np.polynomial.legendre.leggauss((k + 1) // 2)
+    out = np.zeros(n)
+    for a, b in zip(bp[:-1], bp[1:]):
+        x = (b - a) / 2 * xg + (a + b) / 2
+        wts = (b - a) / 2 * wg
+        for j in range(n):
+            c = np.zeros(n)
+            c[j] = 1.0
+            v = np.nan_to_num(BSpline(t, c, k - 1, extrapolate=False)(x))
+            out[j] += np.dot(wts, v)
+    return out
+
+
 def band_of(M, kl, ku):
     """Column-major general band storage (ld = kl+1+ku) of dense M, shaped
     (n, ld): ab[j, ku + i - j] = M[i, j] inside the band, zero elsewhere."""

@@ -151,14 +151,18 @@ def lambdas(g, y_points, k, htdelta, C1low, C2low):
 
 def apply_ns(g, swave, scenario, linearize="rhome_xyz", refs=None,
              zeroth=True, lam=None, evmax=(SMR91_EVMAXMAG_REAL,
-                                           SMR91_EVMAXMAG_IMAG)):
+                                           SMR91_EVMAXMAG_IMAG), means=None):
     """Returns (N wave (5, dNz, Xw, Ny), refs dict, delta_t candidates or
-    None).  refs: plane means to use (collected here when zeroth)."""
+    None).  refs: plane means to use (collected here when zeroth).  means:
+    a dict receiving this evaluation's own plane means (the instantaneous
+    ones of a non-zeroth substep, navier_stokes.hpp:352-356)."""
     s, aux = aux_fields(g, swave)
     P = g.to_physical(s)                                  # (5, Ny, dNz, dNx)
     A = {k: g.to_physical(v) for k, v in aux.items()}
     if zeroth or refs is None:
         refs = references(g, scenario, P)
+    if means is not None:
+        means.update(refs if zeroth else references(g, scenario, P))
     out, cand = rhs_physical(scenario, P, A, refs, linearize,
                              lam if zeroth else None, evmax)
     W = g.to_wave(out)
@@ -367,13 +371,12 @@ def apply_operator(g, phi, scen5, R, x, beta=0.0, y=None, threads=1):
     return out
 
 
-def invert_operator(g, phi, scen5, R, x, bc, threads=1, systems=None,
-                    lapack=False):
-    """(M + phi L)^{-1} x with isothermal walls, per retained wavenumber
-    (operator_hybrid_isothermal.cpp:612-689); bc = (lower, upper) (T,u,v,w).
-    systems: only these (iz, ix) pencils are solved (the others stay 0).
-    lapack: factor and solve with LAPACK ZGBTRF/ZGBTRS (oracle.lapack, the
-    CPU baseline) instead of the restatement."""
+def system_solver(g, phi, scen5, R, bc, iz, ix, lapack=False):
+    """Factor (M + phi L) with isothermal walls for the (iz, ix) wavenumber
+    (operator_hybrid_isothermal.cpp:612-661: packf, BC rows, zgbtrf) and
+    return solve(rhs) for right hand sides (5, Ny) of collocation values:
+    permute, zero the wall rows (IsothermalPATPTEnforcer::rhs), ZGBTRS 'T',
+    permute back (:662-673) -> B-spline coefficients (5, Ny)."""
     from . import imexop_pack, isothermal_op, isothermal_rhs, zaPxpby
     if lapack:
         from .lapack import zgbtrf, zgbtrs
@@ -385,25 +388,118 @@ def invert_operator(g, phi, scen5, R, x, bc, threads=1, systems=None,
     LD = KL + 1 + KU
     N = S * n
     ld = LD + KL
-    out = np.zeros_like(x)
+    ab = np.zeros(N * ld, dtype=complex)
+    imexop_pack(phi, g.kx[ix], g.kz[iz], scen5, R, g.ops, ab, ld, offset=KL)
+    isothermal_op(S, n, KL, KU, ab, ld, scen5, bc[0], bc[1], offset=KL)
+    info, ipiv = zgbtrf(N, KL, KU, ab, ld)
+    if info:
+        raise ArithmeticError(f"zero pivot {info} at ({iz}, {ix})")
 
-    def one(iz, ix):
-        ab = np.zeros(N * ld, dtype=complex)
-        imexop_pack(phi, g.kx[ix], g.kz[iz], scen5, R, g.ops, ab, ld, offset=KL)
-        isothermal_op(S, n, KL, KU, ab, ld, scen5, bc[0], bc[1], offset=KL)
+    def solve(rhs):
         p = np.zeros(N, dtype=complex)
-        zaPxpby("N", S, n, 1, np.ascontiguousarray(x[:, iz, ix, :]).reshape(-1),
+        zaPxpby("N", S, n, 1, np.ascontiguousarray(rhs, dtype=complex).reshape(-1),
                 0, p)
         isothermal_rhs(S, n, p)
-        info, ipiv = zgbtrf(N, KL, KU, ab, ld)
-        if info:
-            raise ArithmeticError(f"zero pivot {info} at ({iz}, {ix})")
         zgbtrs("T", N, KL, KU, ab, ld, ipiv, p)
         v = np.zeros(N, dtype=complex)
         zaPxpby("T", S, n, 1, p, 0, v)
-        out[:, iz, ix, :] = v.reshape(S, n)
+        return v.reshape(S, n)
+    return solve
+
+
+def invert_operator(g, phi, scen5, R, x, bc, threads=1, systems=None,
+                    lapack=False):
+    """(M + phi L)^{-1} x with isothermal walls, per retained wavenumber
+    (operator_hybrid_isothermal.cpp:612-689); bc = (lower, upper) (T,u,v,w).
+    systems: only these (iz, ix) pencils are solved (the others stay 0).
+    lapack: factor and solve with LAPACK ZGBTRF/ZGBTRS (oracle.lapack, the
+    CPU baseline) instead of the restatement."""
+    out = np.zeros_like(x)
+
+    def one(iz, ix):
+        solve = system_solver(g, phi, scen5, R, bc, iz, ix, lapack)
+        out[:, iz, ix, :] = solve(x[:, iz, ix, :])
     _for_systems(g, one, threads, systems)
     return out
+
+
+# ------------------------------------------------------ bulk constraints
+IMPLICIT_MEANS = ("fx fy fz f_dot_u qb CrhoE C2rhoE Crhou C2rhou Crhov C2rhov "
+                  "Crhow C2rhow Crho C2rho Crhou_dot_u C2rhou_dot_u").split()
+
+
+def constraint_treatment(g, phi, scen5, R, bc, out, targets, weights, uvw, Ma,
+                         imp=None, delta_t=0.0, substep=0, substeps=3,
+                         lapack=False):
+    """constraint::treatment::invert_mass_plus_scaled_operator after the
+    operator's solve (suzerain/treatment_constraint.cpp:87-316) with the
+    bulk constraints of constraint.cpp:122-128,230-235 on the physical
+    slots (e, mx, my, mz, rho) and no numerical (one-sided) constraints.
+    out: the solved state (5, dNz, Xw, Ny), its (0,0) mode corrected in
+    place; targets: 5 bulk targets (NaN: disabled); weights: integral of each
+    basis function / L; uvw: plane means of u, v, w (3, Ny) of the substep.
+    imp: dict of the running means (IMPLICIT_MEANS) updated when
+    delta_t > 0.  Returns the 10 multipliers (physical then numerical)."""
+    n = g.Ny
+    assert g.keep[0, 0]
+    enabled = [t == t for t in targets] + [False] * 5
+    # right hand sides (treatment_constraint.cpp:123-144), uniform profiles
+    shape = np.ones(n)
+    cdata = np.zeros((10, 5, n), dtype=complex)
+    cdata[E, E] = shape
+    cdata[MX, E] = shape * uvw[0] * (Ma * Ma)
+    cdata[MX, MX] = shape
+    cdata[MY, E] = shape * uvw[1] * (Ma * Ma)
+    cdata[MY, MY] = shape
+    cdata[MZ, E] = shape * uvw[2] * (Ma * Ma)
+    cdata[MZ, MZ] = shape
+    cdata[RHO, RHO] = shape
+    for f in range(5):
+        cdata[5 + f, f] = shape
+    # the same (0,0) operator solves every column (hybrid ic0 path, :662-686)
+    solve = system_solver(g, phi, scen5, R, bc, 0, 0, lapack)
+    for j in range(10):
+        cdata[j] = solve(cdata[j])
+    mean = out[:, 0, 0, :]
+    # the 10 x 10 matrix, nonzero where both constraints are enabled
+    cmat = np.zeros((10, 10))
+    for j in range(10):
+        if enabled[j]:
+            for i in range(5):
+                if enabled[i]:
+                    cmat[i, j] = np.dot(weights, cdata[j, i].real)
+    crhs = np.zeros(10)
+    for i in range(5):
+        if enabled[i]:
+            crhs[i] = targets[i] - np.dot(weights, mean[i].real)
+    # least squares by SVD; Eigen's default rank threshold for a 10 x 10
+    # matrix drops singular values below 10 eps sigma_max
+    cphi = np.linalg.pinv(cmat, rcond=10 * np.finfo(float).eps) @ crhs
+    mean += np.einsum("jfy,j->fy", cdata, cphi)
+    out[:, 0, 0, :] = mean
+    if imp is not None and delta_t > 0:
+        c = cphi * (substeps / delta_t)
+        iota = 1.0 / (substep + 1)
+        u, v, w = uvw
+        num = c[5:] * 0                     # numerical constraints disabled
+        smp = {
+            "fx": c[MX] * shape, "fy": c[MY] * shape, "fz": c[MZ] * shape,
+            "f_dot_u": c[MX] * shape * u + c[MY] * shape * v + c[MZ] * shape * w,
+            "qb": c[E] * shape,
+            "CrhoE": num[E] * shape, "C2rhoE": (num[E] * shape) ** 2,
+            "Crhou": num[MX] * shape, "C2rhou": (num[MX] * shape) ** 2,
+            "Crhov": num[MY] * shape, "C2rhov": (num[MY] * shape) ** 2,
+            "Crhow": num[MZ] * shape, "C2rhow": (num[MZ] * shape) ** 2,
+            "Crho": c[RHO] * shape + num[RHO] * shape,
+            "C2rho": ((c[RHO] * shape) ** 2 + 2 * (c[RHO] * shape) * (num[RHO] * shape)
+                      + (num[RHO] * shape) ** 2),
+            "Crhou_dot_u": num[MX] * shape * u + num[MY] * shape * v + num[MZ] * shape * w,
+            "C2rhou_dot_u": np.zeros(n),
+        }
+        for k in IMPLICIT_MEANS:
+            imp.setdefault(k, np.zeros(n))
+            imp[k] = imp[k] + iota * (smp[k] - imp[k])
+    return cphi
 
 
 class HybridOperator:
@@ -411,9 +507,18 @@ class HybridOperator:
     walls on every retained wavenumber (operator_hybrid_isothermal.cpp
     :103-374 apply/accumulate, :528-689 invert), as lowstorage.step uses it."""
 
-    def __init__(self, g, scen5, R, bc, threads=1, lapack=False):
+    def __init__(self, g, scen5, R, bc, threads=1, lapack=False, bulk=None,
+                 weights=None, Ma=None):
         self.g, self.scen5, self.R, self.bc = g, scen5, R, bc
         self.threads, self.lapack = threads, lapack
+        # bulk constraints: 5 targets (NaN disabled) wrap invert in the
+        # constraint treatment; uvw: the plane mean velocities of the last
+        # nonlinear evaluation; imp: running means of the implicit forcing
+        self.bulk, self.weights, self.Ma = bulk, weights, Ma
+        self.uvw = None
+        self.imp = {}
+        self.cphi = None
+        self.history = []        # (cphi, uvw) of every constrained solve
 
     def apply(self, phi, a):
         return apply_operator(self.g, phi, self.scen5, self.R, a,
@@ -423,30 +528,46 @@ class HybridOperator:
         return apply_operator(self.g, phi, self.scen5, self.R, a, beta=beta,
                               y=b, threads=self.threads)
 
-    def invert(self, phi, a):
-        return invert_operator(self.g, phi, self.scen5, self.R, a, self.bc,
-                               threads=self.threads, lapack=self.lapack)
+    def invert(self, phi, a, delta_t=0.0, substep=0):
+        out = invert_operator(self.g, phi, self.scen5, self.R, a, self.bc,
+                              threads=self.threads, lapack=self.lapack)
+        if self.bulk is not None:
+            self.cphi = constraint_treatment(
+                self.g, phi, self.scen5, self.R, self.bc, out, self.bulk,
+                self.weights, self.uvw, self.Ma, self.imp, delta_t, substep,
+                lapack=self.lapack)
+            self.history.append((self.cphi.copy(), self.uvw.copy()))
+        return out
 
 
-def smr91_step(g, a, scenario, bc, lam, max_dt=0.0, threads=1, lapack=False):
+def smr91_step(g, a, scenario, bc, lam, max_dt=0.0, threads=1, lapack=False,
+               bulk=None, weights=None, L=None):
     """lowstorage::step (lowstorage.hpp:1461-1520) for the hybrid operator:
     returns (new a, dt, refs).  The references are collected on substep 0
-    and the implicit operator built from them for the whole step."""
+    and the implicit operator built from them for the whole step.  bulk: 5
+    bulk targets (NaN: disabled) for the constraint treatment around the
+    implicit solve, weights the bulk weights; L: a HybridOperator to reuse
+    (its running means of the implicit forcing carry over)."""
     scen5 = [scenario[k] for k in ("Re", "Pr", "Ma", "alpha", "gamma")]
     chi = 1.0 / (g.dNx * g.dNz)
-    L = HybridOperator(g, scen5, None, bc, threads, lapack)
+    if L is None:
+        L = HybridOperator(g, scen5, None, bc, threads, lapack, bulk=bulk,
+                           weights=weights, Ma=scenario["Ma"])
     held = {}
 
     def N(b, i):
+        means = {}
         if i == 0:
             b, refs, cand = apply_ns(g, b, scenario, "rhome_xyz", zeroth=True,
-                                     lam=lam)
+                                     lam=lam, means=means)
             held["refs"] = refs
             L.R = ns.imexop_refs(refs)
-            return b, cand
-        b, _, _ = apply_ns(g, b, scenario, "rhome_xyz", refs=held["refs"],
-                           zeroth=False)
-        return b, None
+        else:
+            b, _, cand = apply_ns(g, b, scenario, "rhome_xyz", refs=held["refs"],
+                                  zeroth=False, means=means)
+            cand = None
+        L.uvw = np.array([means["u"], means["v"], means["w"]])
+        return b, cand
 
     a, dt = lowstorage.step(L, chi, N, a, max_dt=max_dt)
     return a, dt, held["refs"]

@@ -87,8 +87,10 @@ def reduce_candidates(cand):
 def step(L, chi, N, a, max_dt=0.0, m=SMR91):
     """lowstorage::step with scheme m.  L provides apply(phi, a),
     accumulate(phi, a, beta, b) (returns beta b + (M + phi L) a) and
-    invert(phi, a) (returns (M + phi L)^{-1} a); N(b, i) returns the new b
-    and, for i == 0, the stable step candidates.  Returns (a, dt)."""
+    invert(phi, a, delta_t, substep) (returns (M + phi L)^{-1} a; the step
+    size and substep index as lowstorage.hpp:1503,1515 pass them); N(b, i)
+    returns the new b and, for i == 0, the stable step candidates.  Returns
+    (a, dt)."""
     b = a.copy()
     b, cand = N(b, 0)
     dt = reduce_candidates(list(cand))
@@ -96,11 +98,11 @@ def step(L, chi, N, a, max_dt=0.0, m=SMR91):
         dt = minnan(dt, max_dt)
     a = L.apply(dt * m.alpha[0], a)
     a = a + chi * dt * m.gamma[0] * b
-    a = L.invert(-dt * m.beta[0], a)
+    a = L.invert(-dt * m.beta[0], a, dt, 0)
     for i in range(1, m.substeps):
         b = L.accumulate(dt * m.alpha[i], a, chi * dt * m.zeta[i], b)
         a, b = b, a                      # b.exchange(a)
         b, _ = N(b, i)
         a = a + chi * dt * m.gamma[i] * b
-        a = L.invert(-dt * m.beta[i], a)
+        a = L.invert(-dt * m.beta[i], a, dt, i)
     return a, dt

@@ -86,6 +86,7 @@ SIGNATURES = {
     "sz_bsplineop_kl": (i32, [vp, i32]),
     "sz_bsplineop_ku": (i32, [vp, i32]),
     "sz_bsplineop_collocation_point": (f64, [vp, i32]),
+    "sz_bsplineop_integration_coefficients": (i32, [vp, pd]),
     "sz_bsplineop_D_T": (pd, [vp, i32]),
     "sz_bsplineop_D_T_device": (vp, [vp]),
     "sz_bsplineop_accumulate_complex_batched": (i32, [vp, i32, i32, pd, vp, i32,

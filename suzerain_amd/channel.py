@@ -39,7 +39,27 @@ class sz_channel_spec(C.Structure):
                 ("htdelta", C.c_double), ("Lx", C.c_double),
                 ("Ly", C.c_double), ("Lz", C.c_double),
                 ("scenario", sz_ns_scenario), ("bc", sz_isothermal_bc),
-                ("solver", C.c_char_p), ("linearize", C.c_int)]
+                ("solver", C.c_char_p), ("linearize", C.c_int),
+                ("constrained", C.c_int), ("bulk", C.c_double * 5)]
+
+
+# state fields a bulk target may name (the reference's bulk_rho_E,
+# bulk_rho_u, bulk_rho; rho_v and rho_w complete the treatment's set)
+BULK_FIELDS = ("rho_E", "rho_u", "rho_v", "rho_w", "rho")
+IMPLICIT_MEANS = ("fx fy fz f_dot_u qb CrhoE C2rhoE Crhou C2rhou Crhov C2rhov "
+                  "Crhow C2rhow Crho C2rho Crhou_dot_u C2rhou_dot_u").split()
+
+
+def bulk_spec(bulk):
+    """(constrained, 5 targets) from a dict naming BULK_FIELDS (others
+    NaN: disabled), or (0, zeros) for None."""
+    if not bulk:
+        return 0, (C.c_double * 5)()
+    unknown = set(bulk) - set(BULK_FIELDS)
+    if unknown:
+        raise ValueError(f"unknown bulk fields {sorted(unknown)}")
+    return 1, (C.c_double * 5)(*[float(bulk.get(f, math.nan))
+                                 for f in BULK_FIELDS])
 
 
 _SIGS = {
@@ -60,6 +80,9 @@ _SIGS = {
                                              C.c_int]),
     "sz_channel_invert_operator_slot": (C.c_int, [C.c_void_p, C.c_double,
                                                   C.c_int, C.c_int]),
+    "sz_channel_invert_operator_substep": (C.c_int, [C.c_void_p, C.c_double,
+                                                     C.c_int, C.c_int,
+                                                     C.c_double, C.c_int]),
     "sz_channel_solver_slots": (C.c_int, [C.c_void_p]),
     "sz_channel_solver": (C.c_void_p, [C.c_void_p, C.c_int]),
     "sz_channel_solver_stats": (C.c_void_p, [C.c_void_p, C.c_int]),
@@ -94,7 +117,11 @@ class Channel:
     def __init__(self, Nx=96, Ny=96, Nz=96, dNx=None, dNz=None, k=8,
                  htdelta=3.0, Lx=4 * math.pi, Ly=2.0, Lz=4 * math.pi / 3,
                  scenario=None, lower=(1.0, 0.0, 0.0, 0.0),
-                 upper=(1.0, 0.0, 0.0, 0.0), solver="zgbsv", stream=None):
+                 upper=(1.0, 0.0, 0.0, 0.0), solver="zgbsv", stream=None,
+                 bulk=None):
+        """bulk: bulk (y-averaged) targets of the (0,0) mode by field name
+        (BULK_FIELDS), e.g. {"rho": 1, "rho_u": 1, "rho_E": e_b}; the
+        implicit solve then applies the integral constraint treatment."""
         import torch
         sc = dict(DEFAULT_SCENARIO, **(scenario or {}))
         self.scenario = sc
@@ -108,7 +135,9 @@ class Channel:
             Nx, Ny, Nz, dNx, dNz, k, htdelta, Lx, Ly, Lz,
             sz_ns_scenario(*[sc[q] for q in ("alpha", "beta", "gamma", "Ma",
                                              "Pr", "Re")]),
-            sz_isothermal_bc(1, 1, *lower, *upper), solver.encode(), 1)
+            sz_isothermal_bc(1, 1, *lower, *upper), solver.encode(), 1,
+            *bulk_spec(bulk))
+        self.bulk = dict(bulk or {})
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream
         self._stream = stream
@@ -192,6 +221,33 @@ class Channel:
         self._copy(out.ctypes.data, p, out.nbytes, 2)
         return out
 
+    def _host(self, which, shape):
+        p = _lib().sz_channel_buffer(self._h, which)
+        if not p:
+            return None
+        out = np.empty(shape)
+        self._copy(out.ctypes.data, p, out.nbytes, 2)
+        return out
+
+    def implicit_means(self):
+        """Running means of the implicit forcing (17, Ny) by
+        IMPLICIT_MEANS name order; None without bulk constraints."""
+        return self._host(11, (17, self.Ny))
+
+    def constraint_multipliers(self):
+        """The 5 multipliers (per field) of the last constraint solve."""
+        return self._host(12, (5,))
+
+    def bulk_weights(self):
+        """w with w . c = (1/Ly) integral over y of the B-spline
+        expansion c."""
+        return self._host(13, (self.Ny,))
+
+    def instantaneous(self):
+        """Plane means (42, Ny) of the last non-zeroth nonlinear
+        evaluation (kept with bulk constraints)."""
+        return self._host(14, (42, self.Ny))
+
     def collocation_points(self):
         w = _lib().sz_channel_buffer(self._h, 8)
         f = load().sz_bsplineop_collocation_point
@@ -232,9 +288,12 @@ class Channel:
         check(_lib().sz_channel_apply_operator(self._h, phi, which),
               "sz_channel_apply_operator")
 
-    def invert_operator(self, phi, which=0, slot=0):
-        check(_lib().sz_channel_invert_operator_slot(self._h, phi, which, slot),
-              "sz_channel_invert_operator_slot")
+    def invert_operator(self, phi, which=0, slot=0, delta_t=0.0, substep=0):
+        """delta_t > 0: a substep of a step of that size (the constraint
+        treatment's running means)."""
+        check(_lib().sz_channel_invert_operator_substep(self._h, phi, which, slot,
+                                                        delta_t, substep),
+              "sz_channel_invert_operator_substep")
         rc = _lib().sz_channel_check(self._h)
         if rc:
             raise ArithmeticError(f"implicit solve: zero pivot, info {rc}")

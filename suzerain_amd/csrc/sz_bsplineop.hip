@@ -353,6 +353,19 @@ int sz_bsplineop_kl(const sz_bsplineop *w, int d) { return w->kl[d]; }
 int sz_bsplineop_ku(const sz_bsplineop *w, int d) { return w->ku[d]; }
 double sz_bsplineop_collocation_point(const sz_bsplineop *w, int i) { return w->points[i]; }
 
+// Integral of every basis function over the whole breakpoint range
+// (suzerain_bspline_integration_coefficients, suzerain/bspline.c:223-282,
+// with nderiv = 0 over all of [breakpoint 0, last breakpoint]): the order-k
+// B-spline on knots t_i .. t_{i+k} integrates to (t_{i+k} - t_i) / k.
+int sz_bsplineop_integration_coefficients(const sz_bsplineop *w, double *coeff)
+{
+    if (!w) return -1;
+    if (!coeff) return -2;
+    for (int i = 0; i < w->n; ++i)
+        coeff[i] = (w->knots[(size_t)i + w->k] - w->knots[i]) / w->k;
+    return 0;
+}
+
 // Host copy of derivative d's band, (n x ld), element D_T(j,i) at
 // i*ld + max_ku + j - i.
 const double *sz_bsplineop_D_T(const sz_bsplineop *w, int d)

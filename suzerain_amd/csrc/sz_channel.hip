@@ -19,6 +19,7 @@
 #include "sz_common.hpp"
 #include "../../include/suzerain_amd.h"
 #include "../../include/suzerain_amd_solver.hpp"
+#include "sz_constraint.hpp"
 #include "sz_precision.hpp"
 
 #include <algorithm>
@@ -309,6 +310,16 @@ struct sz_channel {
     double *phys = nullptr, *refs = nullptr, *work = nullptr, *dbuf = nullptr;
     double *km = nullptr, *kn = nullptr;
     long long *sysoff = nullptr;
+    // bulk constraints: nsolve = nsys + cons.ncon systems are solved, the
+    // extra ones (kx = kz = 0) carry the constraint columns in dealiasing
+    // pencils; weights (Ny), the multipliers of the last solve (5), the
+    // running means of the implicit forcing, and the plane means the
+    // constraint columns use (sub: the references after a zeroth nonlinear
+    // evaluation, inst after the others)
+    sz_cons cons{};
+    int nsolve = 0;
+    double *cweights = nullptr, *cphi = nullptr, *imp = nullptr, *inst = nullptr;
+    const double *sub = nullptr;
     sz_imexop_ref iref{};
     sz_imexop_refld ild{};
     std::vector<double> lambda1_y, lambda2_y;
@@ -415,6 +426,10 @@ sz_channel *sz_internal_channel_create(const sz_channel_spec *spec, void *stream
         set_err(err, errlen, "only the rhome_xyz linearization is provided");
         return nullptr;
     }
+    if (sp.constrained && sp.solver && std::string(sp.solver) == "none") {
+        set_err(err, errlen, "bulk constraints need the channel's own implicit solver");
+        return nullptr;
+    }
     sz_channel *h = new sz_channel();
     try {
         h->spec = sp;
@@ -458,6 +473,32 @@ sz_channel *sz_internal_channel_create(const sz_channel_spec *spec, void *stream
             }
         }
         h->nsys = (int)km.size();
+        // bulk constraints (main_advance.cpp:259-272): one extra system per
+        // enabled target, its pencils the first dealiasing pencils
+        if (sp.constrained) {
+            for (int f = 0; f < 5; ++f) {
+                if (sp.bulk[f] != sp.bulk[f]) continue;
+                h->cons.field[h->cons.ncon] = f;
+                h->cons.target[h->cons.ncon] = sp.bulk[f];
+                ++h->cons.ncon;
+            }
+            int found = 0;
+            for (int z = 0; z < sp.dNz && found < h->cons.ncon; ++z) {
+                const int wz = z < (sp.dNz + 1) / 2 ? z : z - sp.dNz;
+                for (int x = 0; x < h->Xw && found < h->cons.ncon; ++x) {
+                    if (x < h->xkeep && (wz < 0 ? -wz : wz) <= h->zabs) continue;
+                    h->cons.off[found++] = ((long long)z * h->Xw + x) * sp.Ny;
+                }
+            }
+            if (found < h->cons.ncon)
+                throw std::runtime_error("bulk constraints need dealiasing pencils");
+            for (int j = 0; j < h->cons.ncon; ++j) {
+                km.push_back(0.0);
+                kn.push_back(0.0);
+                off.push_back(h->cons.off[j]);
+            }
+        }
+        h->nsolve = (int)km.size();
         h->km = h->dalloc<double>(km.size());
         h->kn = h->dalloc<double>(kn.size());
         h->sysoff = h->dalloc<long long>(off.size());
@@ -482,13 +523,13 @@ sz_channel *sz_internal_channel_create(const sz_channel_spec *spec, void *stream
             pp.aiter = zspec.aiter();
             pp.tolsc = zspec.tolsc();
             pp.reuse = zspec.reuse() ? 1 : 0;
-            const std::vector<sz_prec_member> mem{{sp.Ny, g.KL, g.KU, h->nsys}};
+            const std::vector<sz_prec_member> mem{{sp.Ny, g.KL, g.KU, h->nsolve}};
             h->prec = sz_prec_create(mem, pp, h->stream);
             if (!h->prec) throw std::runtime_error("mixed precision solver setup");
             h->prec_slots = pp.nslots;
         } else {
             h->solvers.push_back(
-                suzerain_amd::bsmbsm_solver::build(g, zspec, 1, h->nsys, stream));
+                suzerain_amd::bsmbsm_solver::build(g, zspec, 1, h->nsolve, stream));
         }
 
         // device storage
@@ -507,6 +548,23 @@ sz_channel *sz_internal_channel_create(const sz_channel_spec *spec, void *stream
         hipMemsetAsync(h->buf[0], 0, (size_t)h->F * 5 * sizeof(zd), h->stream);
         hipMemsetAsync(h->buf[1], 0, (size_t)h->F * 5 * sizeof(zd), h->stream);
         hipMemsetAsync(h->refs, 0, (size_t)SZ_NS_NREF * sp.Ny * 8, h->stream);
+        if (h->cons.ncon) {
+            // bulk weights: integral of each basis function / Ly
+            // (constraint.cpp:122-128)
+            std::vector<double> wts((size_t)sp.Ny);
+            if (sz_bsplineop_integration_coefficients(h->bop, wts.data()))
+                throw std::runtime_error("integration coefficients");
+            for (double &v : wts) v /= sp.Ly;
+            h->cweights = h->dalloc<double>((size_t)sp.Ny);
+            hipMemcpy(h->cweights, wts.data(), wts.size() * 8, hipMemcpyHostToDevice);
+            h->cphi = h->dalloc<double>(5);
+            h->imp = h->dalloc<double>((size_t)SZ_CONS_NIMP * sp.Ny);
+            h->inst = h->dalloc<double>((size_t)SZ_NS_NREF * sp.Ny);
+            hipMemsetAsync(h->cphi, 0, 5 * 8, h->stream);
+            hipMemsetAsync(h->imp, 0, (size_t)SZ_CONS_NIMP * sp.Ny * 8, h->stream);
+            hipMemsetAsync(h->inst, 0, (size_t)SZ_NS_NREF * sp.Ny * 8, h->stream);
+            h->sub = h->inst;           // no evaluation yet: zero velocities
+        }
 
         // imexop views of the reference rows (references.cpp:51-108)
         const int qmap[26] = {5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17,
@@ -587,6 +645,10 @@ void *sz_channel_buffer(sz_channel *h, int which)
     case 8: return h->bop;
     case 9: return h->solvers.empty() ? nullptr : h->solvers[0];
     case 10: return h->dbuf;                  // the 12 step candidates
+    case 11: return h->imp;
+    case 12: return h->cphi;
+    case 13: return h->cweights;
+    case 14: return h->inst;
     default: return nullptr;
     }
 }
@@ -704,6 +766,15 @@ int sz_channel_nonlinear(sz_channel *h, int which, int zeroth,
         rc = sz_ns_references(&sp.scenario, sp.Ny, sp.dNz, sp.dNx, h->phys, h->Fp,
                               h->refs, h->work, st);
         if (rc) return rc;
+        h->sub = h->refs;                 // common.sub = common.ref
+    } else if (h->cons.ncon) {
+        // the constraint columns need this substep's mean velocities
+        // (navier_stokes.hpp:352-356); the references' plane means hold them
+        sz::scope_range r2("collect_instantaneous");
+        rc = sz_ns_references(&sp.scenario, sp.Ny, sp.dNz, sp.dNx, h->phys, h->Fp,
+                              h->inst, h->work, st);
+        if (rc) return rc;
+        h->sub = h->inst;
     }
     double *cand = h->dbuf;
     sz::scope_range r3("nonlinear right hand sides");
@@ -763,14 +834,36 @@ int sz_channel_apply_operator(sz_channel *h, double phi, int which)
 // Solve (M + phi L) x = state with isothermal walls for every retained
 // wavenumber, in place; other wavenumbers are zeroed
 // (operator_hybrid_isothermal.cpp:528-689), with the factorization slot
-// `slot`.  Asynchronous: per-system info stays in the solver's info array
-// (checked by sz_channel_check).
-int sz_channel_invert_operator_slot(sz_channel *h, double phi, int which, int slot)
+// `slot`; with bulk constraints the (0,0) mode is then corrected to meet the
+// targets (treatment_constraint.cpp:87-316), delta_t > 0 and substep_index
+// feeding the running means of the implicit forcing.  Asynchronous:
+// per-system info stays in the solver's info array (checked by
+// sz_channel_check).
+int sz_channel_invert_operator_substep(sz_channel *h, double phi, int which,
+                                       int slot, double delta_t, int substep_index)
 {
     if (!h) return -1;
     if (which != 0 && which != 1) return -3;
     if (slot < 0 || slot >= sz_channel_solver_slots(h)) return -4;   // or "none"
+    if (substep_index < 0) return -6;
     sz::scope_range range("invert_mass_plus_scaled_operator");
+    const int Ny = h->Ny;
+    if (h->cons.ncon) {
+        sz::scope_range r2("constraint::treatment");
+        const int rc = sz_constraint_rhs(h->cons, h->state(which), h->F, Ny,
+                                         h->sub + (size_t)5 * Ny,
+                                         h->spec.scenario.Ma, h->stream);
+        if (rc) return rc;
+    }
+    // constraint::treatment after the solve: SMR91 has 3 substeps
+    auto constrain = [&]() -> int {
+        if (!h->cons.ncon) return 0;
+        sz::scope_range r2("implicit constraint solution");
+        const double scale = delta_t > 0 ? 3.0 / delta_t : 0.0;
+        return sz_constraint_apply(h->cons, h->state(which), h->F, Ny, h->cweights,
+                                   h->sub + (size_t)5 * Ny, h->cphi, h->imp, scale,
+                                   1.0 / (substep_index + 1), h->stream);
+    };
     if (h->prec) {
         sz::scope_range r2("bsmbsm_solver_zcgbsvx::suzerain_lapackext_zcgbsvx");
         sz_prec_problem pr{};
@@ -785,8 +878,9 @@ int sz_channel_invert_operator_slot(sz_channel *h, double phi, int which, int sl
         pr.F = h->F;
         pr.sysoff = h->sysoff;
         if (h->time_factor) h->factor_mark();
-        const int rc = sz_prec_solve(h->prec, slot, &pr);
+        int rc = sz_prec_solve(h->prec, slot, &pr);
         if (h->time_factor) h->factor_mark();
+        if (!rc) rc = constrain();
         if (rc) return rc;
         return h->zero_dealias(h->state(which), 5);
     }
@@ -799,16 +893,16 @@ int sz_channel_invert_operator_slot(sz_channel *h, double phi, int which, int sl
         sz::scope_range r2("implicit operator assembly (packc)");
         rc = sz_rholut_imexop_packc_batched(
             h->imx, p, h->km, h->kn, &h->iref, &h->ild, 0, 1, 2, 3, 4, nullptr,
-            sv.PAPT, sv.ld_PAPT, sv.stride_PAPT, h->nsys, &h->spec.bc, h->stream);
+            sv.PAPT, sv.ld_PAPT, sv.stride_PAPT, h->nsolve, &h->spec.bc, h->stream);
     }
     if (rc) return rc;
     sv.supplied_PAPT();
     const int N = sv.N;
     {
         sz::scope_range r2("suzerain_bsmbsm_zaPxpby");
-        hipLaunchKernelGGL(k_gather, dim3(nblocks((long long)N * h->nsys)), dim3(NT),
+        hipLaunchKernelGGL(k_gather, dim3(nblocks((long long)N * h->nsolve)), dim3(NT),
                            0, h->stream, s, h->F, h->sysoff, 5, h->Ny,
-                           reinterpret_cast<zd *>(sv.PB), sv.stride_PB, h->nsys);
+                           reinterpret_cast<zd *>(sv.PB), sv.stride_PB, h->nsolve);
     }
     if (!ok()) return SZ_ERR_LAUNCH;
     {
@@ -816,22 +910,28 @@ int sz_channel_invert_operator_slot(sz_channel *h, double phi, int which, int sl
         rc = sz_isothermal_rhs_batched(5, h->Ny, 0, 1, 2, 3,
                                        h->spec.bc.enforce_lower,
                                        h->spec.bc.enforce_upper, sv.PB,
-                                       sv.stride_PB, h->nsys, h->stream);
+                                       sv.stride_PB, h->nsolve, h->stream);
     }
     if (rc) return rc;
     rc = sv.solve_async('T', 1);
     if (rc) return rc;
     rc = h->zero_dealias(s, 5);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_scatter, dim3(nblocks((long long)N * h->nsys)), dim3(NT), 0,
+    hipLaunchKernelGGL(k_scatter, dim3(nblocks((long long)N * h->nsolve)), dim3(NT), 0,
                        h->stream, s, h->F, h->sysoff, 5, h->Ny,
-                       reinterpret_cast<const zd *>(sv.PX), sv.stride_PB, h->nsys);
-    return ok() ? 0 : SZ_ERR_LAUNCH;
+                       reinterpret_cast<const zd *>(sv.PX), sv.stride_PB, h->nsolve);
+    if (!ok()) return SZ_ERR_LAUNCH;
+    return constrain();
+}
+
+int sz_channel_invert_operator_slot(sz_channel *h, double phi, int which, int slot)
+{
+    return sz_channel_invert_operator_substep(h, phi, which, slot, 0.0, 0);
 }
 
 int sz_channel_invert_operator(sz_channel *h, double phi, int which)
 {
-    return sz_channel_invert_operator_slot(h, phi, which, 0);
+    return sz_channel_invert_operator_substep(h, phi, which, 0, 0.0, 0);
 }
 
 int sz_channel_solver_slots(const sz_channel *h)
@@ -1044,7 +1144,7 @@ int sz_channel_step(sz_channel *h, double max_dt, double *dt_out)
     // substep i uses factorization slot i when the solver keeps one per
     // substep (factor reuse across steps), else the single slot
     const int per_substep = sz_channel_solver_slots(h) == 3 ? 1 : 0;
-    if (!rc) rc = sz_channel_invert_operator_slot(h, -dt * smr91_beta[0], 0, 0);
+    if (!rc) rc = sz_channel_invert_operator_substep(h, -dt * smr91_beta[0], 0, 0, dt, 0);
     if (!rc) h->substep_mark();
     for (int i = 1; i < 3 && !rc; ++i) {
         rc = sz_channel_accumulate_operator(h, dt * smr91_alpha[i], 0,
@@ -1053,8 +1153,8 @@ int sz_channel_step(sz_channel *h, double max_dt, double *dt_out)
         sz_channel_exchange(h);
         rc = sz_channel_nonlinear(h, 1, 0, nullptr);
         if (!rc) rc = sz_channel_add_scaled(h, chi * dt * smr91_gamma[i], 1, 0);
-        if (!rc) rc = sz_channel_invert_operator_slot(h, -dt * smr91_beta[i], 0,
-                                                      i * per_substep);
+        if (!rc) rc = sz_channel_invert_operator_substep(h, -dt * smr91_beta[i], 0,
+                                                         i * per_substep, dt, i);
         if (!rc) h->substep_mark();
     }
     if (rc) {

@@ -8,6 +8,9 @@ run) and writes the constant tables they contain as data:
                        breakpoints {0,1/4,3/4,1,5/4,2}, RHS 1..30, 30-digit XR)
   bsplineop_golden.json tests/test_bsplineop.cpp:610-741 (cubic D_T0 band and
                        apply goldens, almost-dense cubic, septic apply goldens)
+  bspline_integration_golden.json
+                       tests/test_bspline.c:108-330 (exact integrals of the
+                       order 2, 3, 4 basis functions on breakpoints 0..3)
 
 Usage: python tests/golden/make_golden.py [/root/reference]
 """
@@ -120,9 +123,30 @@ def bsplineop(ref):
     return out
 
 
+def bspline_integration(ref):
+    """Exact integrals of each basis function (nderiv = 0) over the whole
+    breakpoint range {0, 1, 2, 3}: test_bspline.c:108-330 (orders 2, 3, 4;
+    the order-2 table is strided by 2, -555 filling the gaps)."""
+    text = open(os.path.join(ref, "tests", "test_bspline.c")).read()
+    start = text.index("static void test_integration_coefficients()\n{")
+    body = text[start:text.index("static void test_linear_combination()\n{")]
+    b = [str(exact(t)) for t in tokens(re.search(
+        r"const double b\[\]\s*=\s*\{(.*?)\};", body, re.S).group(1))]
+    cases = []
+    for order, inc in ((2, 2), (3, 1), (4, 1)):
+        sect = body[body.index(f"alloc_workspaces({order}, "):]
+        e0 = re.search(r"const double e0\[\]\s*=\s*\{(.*?)\};", sect, re.S)
+        vals = [exact(t) for t in tokens(e0.group(1))][::inc]
+        cases.append({"k": order, "e0": [str(v) for v in vals]})
+    return {"source": "tests/test_bspline.c", "breakpoints": b,
+            "tolerance_eps_multiple": 1000, "cases": cases}
+
+
 def main(ref):
     dump(os.path.join(HERE, "bsmbsm_golden.json"), bsmbsm(ref))
     dump(os.path.join(HERE, "bsplineop_golden.json"), bsplineop(ref))
+    dump(os.path.join(HERE, "bspline_integration_golden.json"),
+         bspline_integration(ref))
 
 
 if __name__ == "__main__":

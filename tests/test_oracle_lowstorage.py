@@ -73,7 +73,7 @@ class _Multiplicative:
     def accumulate(self, phi, a, beta, b):
         return b * beta + a * (phi * self.f + 1)
 
-    def invert(self, phi, a):
+    def invert(self, phi, a, delta_t=0.0, substep=0):
         return a * (1 / (phi * self.f + 1))
 
 
@@ -137,7 +137,7 @@ def _step_flipped_sign(L, chi, N, a, max_dt=0.0, m=ls.SMR91):
         apply, accumulate = L.apply, L.accumulate
 
         @staticmethod
-        def invert(phi, x):
+        def invert(phi, x, delta_t=0.0, substep=0):
             return L.invert(-phi, x)
     return ls.step(Flip, chi, N, a, max_dt, m)
 
