@@ -144,6 +144,10 @@ double sz_bsplineop_collocation_point(const sz_bsplineop *w, int i);
  * suzerain_bspline_integration_coefficients (suzerain/bspline.c:223-282) with
  * nderiv = 0 over the whole domain; n values. */
 int sz_bsplineop_integration_coefficients(const sz_bsplineop *w, double *coeff);
+/* Galerkin L2 (Gram) matrix of the basis, G(i, j) = integral of B_i B_j
+ * (the SUZERAIN_BSPLINEOP_GALERKIN_L2 operator of bsplineop.c, nderiv = 0):
+ * band rows of width 2k-1, G(i, j) at band[i (2k-1) + k-1 + j - i]. */
+int sz_bsplineop_gram(const sz_bsplineop *w, double *band);
 const double *sz_bsplineop_D_T(const sz_bsplineop *w, int d);
 const double *sz_bsplineop_D_T_device(sz_bsplineop *w);
 int sz_bsplineop_accumulate_complex_batched(sz_bsplineop *w, int nderiv,
@@ -481,6 +485,56 @@ int sz_channel_substep_timing(sz_channel *h, int enable, double *ms, int max,
 /* lowstorage::step (lowstorage.hpp:1461-1520) with SMR91 and the
  * delta_t_reducer; dt_out receives the step taken. */
 int sz_channel_step(sz_channel *h, double max_dt, double *dt_out);
+
+/* ------------------------------------------------- manufactured solutions
+ * The nsctpl_rholut manufactured solution (apps/perfect/nsctpl_rholut_fwd.hpp,
+ * nsctpl_rholut.hpp; apps/perfect/manufactured_solution.{hpp,cpp}): each of
+ * rho, u, v, w and T is a sum of seven trigonometric terms
+ *   0:  a cos(g + f t)
+ *   x:  a cos(c + b 2pi x/Lx) cos(g + f t)      (y, z alike)
+ *   xy: a cos(c + b 2pi x/Lx) cos(e + d 2pi y/Ly) cos(g + f t)  (xz, yz alike)
+ * and the forcing Q makes it an exact solution of the nondimensional
+ * compressible Navier-Stokes equations (p = rho T / gamma, mu = T^beta,
+ * lambda = (alpha - 2/3) mu, Ma, Re, Pr). */
+typedef struct sz_mms_term { double a, b, c, d, e, f, g; } sz_mms_term;
+typedef struct sz_mms_primitive {
+    sz_mms_term t0, x, y, z, xy, xz, yz;
+} sz_mms_primitive;
+typedef struct sz_mms {
+    double alpha, beta, gamma, Ma, Pr, Re;
+    double Lx, Ly, Lz;
+    sz_mms_primitive rho, u, v, w, T;
+} sz_mms;
+/* The recommended isothermal channel parameters
+ * (nsctpl_rholut::isothermal_channel, nsctpl_rholut.hpp:991-1104). */
+void sz_mms_isothermal_channel(sz_mms *m);
+/* Conserved state (rho E, rho u, rho v, rho w, rho) and forcing Q
+ * (manufactured_solution::Q_conservative, nsctpl_rholut.hpp:795-965) at npts
+ * device points (x, y, z) and time t; cons and Q (either may be NULL) are
+ * 5 x npts device arrays. */
+int sz_mms_evaluate(const sz_mms *m, int npts, const double *x, const double *y,
+                    const double *z, double t, double *cons, double *Q,
+                    void *stream);
+/* Channels: with a manufactured solution set (NULL clears it) the
+ * nonlinear operator adds its forcing at the evaluation time
+ * (navier_stokes.hpp:905-950) and the density equation is transformed too;
+ * the channel's time advances with each step (lowstorage.hpp:1482-1520
+ * evaluates substep i at t + eta_i dt). */
+int sz_channel_set_manufactured(sz_channel *h, const sz_mms *m);
+int sz_channel_set_time(sz_channel *h, double t);
+double sz_channel_time(const sz_channel *h);
+/* N(state) as sz_channel_nonlinear with the evaluation time given. */
+int sz_channel_nonlinear_time(sz_channel *h, int which, int zeroth, double time,
+                              double *candidates);
+/* state `which` <- the manufactured solution at time t as the channel's
+ * wave-space coefficients (accumulate_manufactured_solution with alpha = 1,
+ * beta = 0, manufactured_solution.cpp:333-457: collocation values, dealiased
+ * forward transform, mass matrix solve). */
+int sz_channel_manufactured_state(sz_channel *h, int which, double t);
+/* L2 norms over the domain of each field of state `which`
+ * (compute_field_L2xyz, suzerain/l2.cpp:43-150): the mean (0,0) mode's and
+ * the rest's (total^2 = mean^2 + fluctuating^2); waits. */
+int sz_channel_field_L2(sz_channel *h, int which, double *mean, double *fluct);
 
 /* ------------------------------------------------------------ ensembles
  * No single counterpart in the reference, which advances one case per process

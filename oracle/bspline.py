@@ -141,6 +141,27 @@ def integration_coefficients(k, bp):
     return out
 
 
+def gram(k, bp):
+    """Dense Gram matrix G(i, j) = integral of B_i B_j over [bp[0], bp[-1]]
+    (the Galerkin L2 operator of bsplineop.c, nderiv = 0): a k-point
+    Gauss-Legendre rule per breakpoint interval (exact for the degree 2k-2
+    products), the basis evaluated with scipy.interpolate.BSpline."""
+    t = knots(k, bp)
+    n = len(t) - k
+    xg, wg = np.polynomial.legendre.leggauss(k)
+    G = np.zeros((n, n))
+    for a, b in zip(bp[:-1], bp[1:]):
+        x = (b - a) / 2 * xg + (a + b) / 2
+        wts = (b - a) / 2 * wg
+        B = np.zeros((n, len(x)))
+        for j in range(n):
+            c = np.zeros(n)
+            c[j] = 1.0
+            B[j] = np.nan_to_num(BSpline(t, c, k - 1, extrapolate=False)(x))
+        G += (B * wts) @ B.T
+    return G
+
+
 def band_of(M, kl, ku):
     """Column-major general band storage (ld = kl+1+ku) of dense M, shaped
     (n, ld): ab[j, ku + i - j] = M[i, j] inside the band, zero elsewhere."""

@@ -151,11 +151,15 @@ def lambdas(g, y_points, k, htdelta, C1low, C2low):
 
 def apply_ns(g, swave, scenario, linearize="rhome_xyz", refs=None,
              zeroth=True, lam=None, evmax=(SMR91_EVMAXMAG_REAL,
-                                           SMR91_EVMAXMAG_IMAG), means=None):
+                                           SMR91_EVMAXMAG_IMAG), means=None,
+             forcing=None):
     """Returns (N wave (5, dNz, Xw, Ny), refs dict, delta_t candidates or
     None).  refs: plane means to use (collected here when zeroth).  means:
     a dict receiving this evaluation's own plane means (the instantaneous
-    ones of a non-zeroth substep, navier_stokes.hpp:352-356)."""
+    ones of a non-zeroth substep, navier_stokes.hpp:352-356).  forcing:
+    (5, Ny, dNz, dNx) added to the pointwise right hand side before the
+    transform back (the manufactured forcing pass, navier_stokes.hpp:
+    905-950); the density equation then keeps its right hand side."""
     s, aux = aux_fields(g, swave)
     P = g.to_physical(s)                                  # (5, Ny, dNz, dNx)
     A = {k: g.to_physical(v) for k, v in aux.items()}
@@ -165,10 +169,12 @@ def apply_ns(g, swave, scenario, linearize="rhome_xyz", refs=None,
         means.update(refs if zeroth else references(g, scenario, P))
     out, cand = rhs_physical(scenario, P, A, refs, linearize,
                              lam if zeroth else None, evmax)
+    if forcing is not None:
+        out = out + forcing
     W = g.to_wave(out)
     for f in range(5):
         g.zero_dealiasing(W[f])
-    if linearize == "rhome_xyz":
+    if linearize == "rhome_xyz" and forcing is None:
         W[RHO] = 0
     return W, refs, cand
 
@@ -541,13 +547,15 @@ class HybridOperator:
 
 
 def smr91_step(g, a, scenario, bc, lam, max_dt=0.0, threads=1, lapack=False,
-               bulk=None, weights=None, L=None):
+               bulk=None, weights=None, L=None, mms=None, t0=0.0):
     """lowstorage::step (lowstorage.hpp:1461-1520) for the hybrid operator:
     returns (new a, dt, refs).  The references are collected on substep 0
     and the implicit operator built from them for the whole step.  bulk: 5
     bulk targets (NaN: disabled) for the constraint treatment around the
     implicit solve, weights the bulk weights; L: a HybridOperator to reuse
-    (its running means of the implicit forcing carry over)."""
+    (its running means of the implicit forcing carry over).  mms: (solution
+    with forcing(x, y, z, t), y points) for the manufactured forcing pass,
+    the step starting at time t0."""
     scen5 = [scenario[k] for k in ("Re", "Pr", "Ma", "alpha", "gamma")]
     chi = 1.0 / (g.dNx * g.dNz)
     if L is None:
@@ -555,19 +563,26 @@ def smr91_step(g, a, scenario, bc, lam, max_dt=0.0, threads=1, lapack=False,
                            weights=weights, Ma=scenario["Ma"])
     held = {}
 
-    def N(b, i):
+    def N(b, i, time=None):
         means = {}
+        Q = None
+        if mms is not None:
+            from .mms import grid_points
+            sol, ypts = mms
+            X, Y, Z = grid_points(g, ypts)
+            Q = sol.forcing(X, Y, Z, time)
         if i == 0:
             b, refs, cand = apply_ns(g, b, scenario, "rhome_xyz", zeroth=True,
-                                     lam=lam, means=means)
+                                     lam=lam, means=means, forcing=Q)
             held["refs"] = refs
             L.R = ns.imexop_refs(refs)
         else:
             b, _, cand = apply_ns(g, b, scenario, "rhome_xyz", refs=held["refs"],
-                                  zeroth=False, means=means)
+                                  zeroth=False, means=means, forcing=Q)
             cand = None
         L.uvw = np.array([means["u"], means["v"], means["w"]])
         return b, cand
 
-    a, dt = lowstorage.step(L, chi, N, a, max_dt=max_dt)
+    a, dt = lowstorage.step(L, chi, N, a, max_dt=max_dt,
+                            t0=t0 if mms is not None else None)
     return a, dt, held["refs"]

@@ -84,15 +84,18 @@ def reduce_candidates(cand):
     return dt
 
 
-def step(L, chi, N, a, max_dt=0.0, m=SMR91):
+def step(L, chi, N, a, max_dt=0.0, m=SMR91, t0=None):
     """lowstorage::step with scheme m.  L provides apply(phi, a),
     accumulate(phi, a, beta, b) (returns beta b + (M + phi L) a) and
     invert(phi, a, delta_t, substep) (returns (M + phi L)^{-1} a; the step
     size and substep index as lowstorage.hpp:1503,1515 pass them); N(b, i)
-    returns the new b and, for i == 0, the stable step candidates.  Returns
-    (a, dt)."""
+    returns the new b and, for i == 0, the stable step candidates; with a
+    start time t0, N is called as N(b, i, time=t0 + eta_i dt), the time
+    lowstorage.hpp:1495,1512 evaluates substep i at.  Returns (a, dt)."""
+    def Nt(b, i, dt=0.0):
+        return N(b, i) if t0 is None else N(b, i, time=t0 + m.eta(i) * dt)
     b = a.copy()
-    b, cand = N(b, 0)
+    b, cand = Nt(b, 0)
     dt = reduce_candidates(list(cand))
     if max_dt > 0:
         dt = minnan(dt, max_dt)
@@ -102,7 +105,7 @@ def step(L, chi, N, a, max_dt=0.0, m=SMR91):
     for i in range(1, m.substeps):
         b = L.accumulate(dt * m.alpha[i], a, chi * dt * m.zeta[i], b)
         a, b = b, a                      # b.exchange(a)
-        b, _ = N(b, i)
+        b, _ = Nt(b, i, dt)
         a = a + chi * dt * m.gamma[i] * b
         a = L.invert(-dt * m.beta[i], a, dt, i)
     return a, dt

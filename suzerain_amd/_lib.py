@@ -87,6 +87,9 @@ SIGNATURES = {
     "sz_bsplineop_ku": (i32, [vp, i32]),
     "sz_bsplineop_collocation_point": (f64, [vp, i32]),
     "sz_bsplineop_integration_coefficients": (i32, [vp, pd]),
+    "sz_bsplineop_gram": (i32, [vp, pd]),
+    "sz_mms_isothermal_channel": (None, [vp]),
+    "sz_mms_evaluate": (i32, [vp, i32, vp, vp, vp, f64, vp, vp, vp]),
     "sz_bsplineop_D_T": (pd, [vp, i32]),
     "sz_bsplineop_D_T_device": (vp, [vp]),
     "sz_bsplineop_accumulate_complex_batched": (i32, [vp, i32, i32, pd, vp, i32,

@@ -94,6 +94,15 @@ _SIGS = {
     "sz_channel_copy": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "sz_channel_exchange": (None, [C.c_void_p]),
     "sz_channel_step": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p]),
+    "sz_channel_set_manufactured": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "sz_channel_set_time": (C.c_int, [C.c_void_p, C.c_double]),
+    "sz_channel_time": (C.c_double, [C.c_void_p]),
+    "sz_channel_nonlinear_time": (C.c_int, [C.c_void_p, C.c_int, C.c_int,
+                                            C.c_double, C.c_void_p]),
+    "sz_channel_manufactured_state": (C.c_int, [C.c_void_p, C.c_int,
+                                                C.c_double]),
+    "sz_channel_field_L2": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p,
+                                      C.c_void_p]),
     "sz_channel_factor_timing": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p,
                                            C.c_void_p]),
     "sz_channel_substep_timing": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p,
@@ -343,6 +352,33 @@ class Channel:
             raise ArithmeticError(f"implicit solve: zero pivot, info {rc}")
         check(rc, "sz_channel_step")
         return dt.value
+
+    # ------------------------------------------------ manufactured solutions
+    def set_manufactured(self, m):
+        """m: a suzerain_amd.mms.Manufactured (its forcing joins N), or None."""
+        self._mms = m
+        check(_lib().sz_channel_set_manufactured(
+            self._h, C.byref(m.s) if m is not None else None),
+            "sz_channel_set_manufactured")
+
+    @property
+    def time(self):
+        return _lib().sz_channel_time(self._h)
+
+    def set_time(self, t):
+        check(_lib().sz_channel_set_time(self._h, t), "sz_channel_set_time")
+
+    def manufactured_state(self, t, which=0):
+        """State `which` <- the manufactured solution at time t."""
+        check(_lib().sz_channel_manufactured_state(self._h, which, t),
+              "sz_channel_manufactured_state")
+
+    def field_L2(self, which=0):
+        """(mean, fluctuating) L2 norms over the domain per field."""
+        mean, fluct = np.zeros(5), np.zeros(5)
+        check(_lib().sz_channel_field_L2(self._h, which, mean.ctypes.data,
+                                         fluct.ctypes.data), "sz_channel_field_L2")
+        return mean, fluct
 
     def factor_timing(self, enable=-1):
         """(summed ms, count) of the factorization launches bracketed by HIP

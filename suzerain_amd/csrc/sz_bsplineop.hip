@@ -353,6 +353,70 @@ int sz_bsplineop_kl(const sz_bsplineop *w, int d) { return w->kl[d]; }
 int sz_bsplineop_ku(const sz_bsplineop *w, int d) { return w->ku[d]; }
 double sz_bsplineop_collocation_point(const sz_bsplineop *w, int i) { return w->points[i]; }
 
+namespace {
+
+// n-point Gauss-Legendre rule on [-1, 1]: Newton iteration on the roots of
+// P_n (three-term recurrence), weights 2 / ((1 - x^2) P_n'(x)^2).
+void gauss_legendre(int n, std::vector<double> &x, std::vector<double> &w)
+{
+    const double pi = 3.14159265358979323846264338327950288;
+    x.assign((size_t)n, 0.0);
+    w.assign((size_t)n, 0.0);
+    for (int i = 0; i < (n + 1) / 2; ++i) {
+        double z = std::cos(pi * (i + 0.75) / (n + 0.5)), dp = 0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1, p1 = z;
+            for (int k = 2; k <= n; ++k) {
+                const double p2 = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            if (n == 1) p0 = 1;
+            dp = n * (z * p1 - p0) / (z * z - 1);
+            const double dz = p1 / dp;
+            z -= dz;
+            if (std::fabs(dz) < 1e-16) break;
+        }
+        x[(size_t)i] = -z;
+        x[(size_t)(n - 1 - i)] = z;
+        const double wt = 2 / ((1 - z * z) * dp * dp);
+        w[(size_t)i] = wt;
+        w[(size_t)(n - 1 - i)] = wt;
+    }
+}
+
+}  // namespace
+
+// The Galerkin L2 (Gram) matrix G(i, j) = integral of B_i B_j over the
+// breakpoint range (the operator of a SUZERAIN_BSPLINEOP_GALERKIN_L2
+// bsplineop for nderiv = 0, suzerain/bsplineop.c): a k-point Gauss-Legendre
+// rule per breakpoint interval integrates the degree 2k-2 products exactly.
+// Band rows of width 2k-1: G(i, j) at band[i (2k-1) + k-1 + j - i].
+int sz_bsplineop_gram(const sz_bsplineop *w, double *band)
+{
+    if (!w) return -1;
+    if (!band) return -2;
+    const int k = w->k, p = k - 1, n = w->n, wd = 2 * k - 1;
+    std::vector<double> gx, gw, ders;
+    gauss_legendre(k, gx, gw);
+    for (size_t i = 0; i < (size_t)n * wd; ++i) band[i] = 0;
+    for (int s = p; s < n; ++s) {                 // knot spans t[s] < t[s+1]
+        const double a = w->knots[s], b = w->knots[(size_t)s + 1];
+        if (!(b > a)) continue;
+        for (int q = 0; q < k; ++q) {
+            const double xq = (b - a) / 2 * gx[(size_t)q] + (a + b) / 2;
+            const double wq = (b - a) / 2 * gw[(size_t)q];
+            basis_derivs(w->knots, s, p, xq, 0, ders);
+            for (int r = 0; r <= p; ++r)
+                for (int c = 0; c <= p; ++c) {
+                    const int i = s - p + r, j = s - p + c;
+                    band[(size_t)i * wd + p + j - i] += wq * ders[(size_t)r] * ders[(size_t)c];
+                }
+        }
+    }
+    return 0;
+}
+
 // Integral of every basis function over the whole breakpoint range
 // (suzerain_bspline_integration_coefficients, suzerain/bspline.c:223-282,
 // with nderiv = 0 over all of [breakpoint 0, last breakpoint]): the order-k

@@ -20,6 +20,7 @@
 #include "../../include/suzerain_amd.h"
 #include "../../include/suzerain_amd_solver.hpp"
 #include "sz_constraint.hpp"
+#include "sz_mms.hpp"
 #include "sz_precision.hpp"
 
 #include <algorithm>
@@ -66,6 +67,14 @@ void k_copy(zd *dst, const zd *src, long long n)
 {
     const long long e = (long long)blockIdx.x * NT + threadIdx.x;
     if (e < n) dst[e] = src[e];
+}
+
+__global__ __launch_bounds__(NT)
+void k_scale(zd *y, double a, long long n)
+{
+    const long long i = (long long)blockIdx.x * NT + threadIdx.x;
+    if (i >= n) return;
+    y[i] = zd{a * y[i].x, a * y[i].y};
 }
 
 __global__ __launch_bounds__(NT)
@@ -284,6 +293,8 @@ constexpr double smr91_beta[3] = {111. / 480, 100. / 480, 80. / 480};
 constexpr double smr91_gamma[3] = {256. / 480, 200. / 480, 360. / 480};
 constexpr double smr91_zeta[3] = {0. / 480, -136. / 480, -200. / 480};
 constexpr double smr91_evmaxmag_real = 2.51274532661832862402373;
+// eta_i = sum_{j < i} (alpha_j + beta_j) (lowstorage.hpp:976-997)
+constexpr double smr91_eta[3] = {0.0, 256. / 480, 320. / 480};
 
 }  // namespace
 
@@ -320,6 +331,15 @@ struct sz_channel {
     int nsolve = 0;
     double *cweights = nullptr, *cphi = nullptr, *imp = nullptr, *inst = nullptr;
     const double *sub = nullptr;
+    // manufactured solution: its forcing joins the nonlinear operator at the
+    // evaluation time; the channel's time advances with each step; device
+    // collocation points, the collocation matrix's band LU (projection) and
+    // the Gram band (L2 norms) are made on first use
+    bool mms_on = false;
+    sz_mms mms{};
+    double time = 0;
+    double *ypts = nullptr, *mass_lu = nullptr, *gram = nullptr, *l2sums = nullptr;
+    int mass_kl = 0, mass_ku = 0;
     sz_imexop_ref iref{};
     sz_imexop_refld ild{};
     std::vector<double> lambda1_y, lambda2_y;
@@ -657,6 +677,13 @@ void *sz_channel_buffer(sz_channel *h, int which)
 int sz_channel_nonlinear(sz_channel *h, int which, int zeroth,
                          double *candidates)
 {
+    return sz_channel_nonlinear_time(h, which, zeroth, h ? h->time : 0.0, candidates);
+}
+
+// N(u) at simulation time `time` (the manufactured forcing's time).
+int sz_channel_nonlinear_time(sz_channel *h, int which, int zeroth, double time,
+                              double *candidates)
+{
     if (!h) return -1;
     sz::scope_range range("apply_navier_stokes_spatial_operator");
     if (which != 0 && which != 1) return -2;
@@ -783,16 +810,26 @@ int sz_channel_nonlinear(sz_channel *h, int which, int zeroth,
                    h->Fp, h->refs, h->dbuf + 16, h->dbuf + 16 + sp.Ny,
                    h->dbuf + 12, cand, h->work, st);
     if (rc) return rc;
-    // back to wave space; rhome_xyz has no explicit density equation
+    if (h->mms_on) {
+        sz::scope_range r2("manufactured forcing");
+        rc = sz_internal_mms_force(h->mms, sp.Ny, sp.dNz, sp.dNx, sp.Lx, sp.Lz, h->ypts,
+                                   time, h->phys, h->Fp, st);
+        if (rc) return rc;
+    }
+    // back to wave space; rhome_xyz has no explicit density equation unless
+    // a manufactured forcing feeds it (navier_stokes.hpp:953-976)
+    const int nback = h->mms_on ? 5 : 4;
     {
         sz::scope_range r2("transform_physical_to_wave");
-        rc = sz_fft_physical_to_wave(h->fft, 4, h->phys, h->Fp, h->mixed, smix,
+        rc = sz_fft_physical_to_wave(h->fft, nback, h->phys, h->Fp, h->mixed, smix,
                                      s, F, st);
     }
     if (rc) return rc;
-    hipLaunchKernelGGL(k_fill0, dim3(nblocks(F)), dim3(NT), 0, st,
-                       s + 4 * (size_t)F, F);
-    if (!ok()) return SZ_ERR_LAUNCH;
+    if (nback == 4) {
+        hipLaunchKernelGGL(k_fill0, dim3(nblocks(F)), dim3(NT), 0, st,
+                           s + 4 * (size_t)F, F);
+        if (!ok()) return SZ_ERR_LAUNCH;
+    }
     if (zeroth && candidates) {
         if (hipMemcpyAsync(candidates, cand, 12 * sizeof(double),
                            hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -1129,7 +1166,7 @@ int sz_channel_step(sz_channel *h, double max_dt, double *dt_out)
     const size_t mark0 = h->sub_events.size();
     h->substep_mark();
     int rc = sz_channel_copy(h, 0, 1);
-    if (!rc) rc = sz_channel_nonlinear(h, 1, 1, cand);
+    if (!rc) rc = sz_channel_nonlinear_time(h, 1, 1, h->time, cand);
     if (rc) return rc;
     double dt = cand[0];
     for (int k = 1; k < 12; ++k) {
@@ -1151,7 +1188,8 @@ int sz_channel_step(sz_channel *h, double max_dt, double *dt_out)
                                             chi * dt * smr91_zeta[i], 1);
         if (rc) break;
         sz_channel_exchange(h);
-        rc = sz_channel_nonlinear(h, 1, 0, nullptr);
+        // substep i is evaluated at t + eta_i dt (lowstorage.hpp:1512)
+        rc = sz_channel_nonlinear_time(h, 1, 0, h->time + dt * smr91_eta[i], nullptr);
         if (!rc) rc = sz_channel_add_scaled(h, chi * dt * smr91_gamma[i], 1, 0);
         if (!rc) rc = sz_channel_invert_operator_substep(h, -dt * smr91_beta[i], 0,
                                                          i * per_substep, dt, i);
@@ -1164,7 +1202,141 @@ int sz_channel_step(sz_channel *h, double max_dt, double *dt_out)
         h->sub_events.resize(mark0);
         return rc;
     }
+    h->time += dt;
     return sz_channel_check(h);
+}
+
+
+// ------------------------------------------------ manufactured solutions
+int sz_channel_set_manufactured(sz_channel *h, const sz_mms *m)
+{
+    if (!h) return -1;
+    if (!m) {
+        h->mms_on = false;
+        return 0;
+    }
+    try {
+        if (!h->ypts) {
+            std::vector<double> y((size_t)h->Ny);
+            for (int j = 0; j < h->Ny; ++j) y[(size_t)j] = sz_bsplineop_collocation_point(h->bop, j);
+            h->ypts = h->dalloc<double>(y.size());
+            if (hipMemcpy(h->ypts, y.data(), y.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+                return SZ_ERR_LAUNCH;
+        }
+    } catch (const std::bad_alloc &) {
+        return SZ_ERR_ALLOC;
+    }
+    h->mms = *m;
+    h->mms_on = true;
+    return 0;
+}
+
+int sz_channel_set_time(sz_channel *h, double t)
+{
+    if (!h) return -1;
+    h->time = t;
+    return 0;
+}
+
+double sz_channel_time(const sz_channel *h) { return h ? h->time : NAN; }
+
+// state `which` <- manufactured solution at t: collocation values on the
+// physical grid, forward transform scaled by 1 / (dNx dNz), dealiasing
+// modes zeroed, collocation matrix solve per pencil
+// (manufactured_solution.cpp:333-457).
+int sz_channel_manufactured_state(sz_channel *h, int which, double t)
+{
+    if (!h) return -1;
+    if (which != 0 && which != 1) return -2;
+    if (!h->mms_on) return -3;
+    if (!h->phys) return SZ_ERR_ALLOC;
+    const sz_channel_spec &sp = h->spec;
+    const int n = h->Ny;
+    try {
+        if (!h->mass_lu) {
+            // the collocation matrix M(r, c) = B_c(y_r) from the transposed
+            // band storage; B-spline collocation matrices are totally
+            // positive, so elimination without pivoting is stable (de Boor)
+            const double *dt0 = sz_bsplineop_D_T(h->bop, 0);
+            const int ld = sz_bsplineop_ld(h->bop), mku = sz_bsplineop_max_ku(h->bop);
+            const int lo = sz_bsplineop_ku(h->bop, 0), up = sz_bsplineop_kl(h->bop, 0);
+            const int wd = lo + up + 1;
+            std::vector<double> a((size_t)n * wd, 0.0);
+            auto A = [&](int r, int c) -> double & { return a[(size_t)r * wd + lo + c - r]; };
+            for (int r = 0; r < n; ++r)
+                for (int c = std::max(0, r - lo); c <= std::min(n - 1, r + up); ++c)
+                    A(r, c) = dt0[(size_t)r * ld + mku + c - r];
+            for (int k = 0; k < n; ++k) {
+                const double piv = A(k, k);
+                if (piv == 0) return -4;
+                for (int i = k + 1; i <= std::min(n - 1, k + lo); ++i) {
+                    const double l = A(i, k) / piv;
+                    A(i, k) = l;
+                    for (int j = k + 1; j <= std::min(n - 1, k + up); ++j) A(i, j) -= l * A(k, j);
+                }
+            }
+            h->mass_lu = h->dalloc<double>(a.size());
+            if (hipMemcpy(h->mass_lu, a.data(), a.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+                return SZ_ERR_LAUNCH;
+            h->mass_kl = lo;
+            h->mass_ku = up;
+        }
+    } catch (const std::bad_alloc &) {
+        return SZ_ERR_ALLOC;
+    }
+    hipStream_t st = h->stream;
+    zd *w = h->state(which);
+    int rc = sz_internal_mms_values(h->mms, n, sp.dNz, sp.dNx, sp.Lx, sp.Lz, h->ypts, t,
+                                    h->phys, h->Fp, st);
+    if (!rc)
+        rc = sz_fft_physical_to_wave(h->fft, 5, h->phys, h->Fp, h->mixed,
+                                     sz_fft_mixed_elements(h->fft), w, h->F, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_scale, dim3(nblocks(5 * h->F)), dim3(NT), 0, st, w,
+                       1.0 / ((double)sp.dNx * sp.dNz), 5 * h->F);
+    if (!ok()) return SZ_ERR_LAUNCH;
+    rc = h->zero_dealias(w, 5);
+    if (!rc) rc = sz_internal_mass_solve(n, h->mass_kl, h->mass_ku, h->mass_lu, w, h->F, 5,
+                                         sp.dNz * h->Xw, st);
+    return rc;
+}
+
+// L2 over the domain of each field (compute_field_L2xyz, suzerain/l2.cpp:
+// 43-150): Lx Lz sum over retained wavenumbers (twice for kx > 0) of
+// u^H G u with the Gram matrix G; mean: the (0,0) mode's part.
+int sz_channel_field_L2(sz_channel *h, int which, double *mean, double *fluct)
+{
+    if (!h) return -1;
+    if (which != 0 && which != 1) return -2;
+    const int n = h->Ny, k = h->spec.k;
+    try {
+        if (!h->gram) {
+            std::vector<double> g((size_t)n * (2 * k - 1));
+            if (sz_bsplineop_gram(h->bop, g.data())) return -3;
+            h->gram = h->dalloc<double>(g.size());
+            h->l2sums = h->dalloc<double>(20);
+            if (hipMemcpy(h->gram, g.data(), g.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+                return SZ_ERR_LAUNCH;
+        }
+    } catch (const std::bad_alloc &) {
+        return SZ_ERR_ALLOC;
+    }
+    int rc = sz_internal_field_l2(n, k - 1, h->gram, h->state(which), h->F, 5, h->spec.dNz,
+                                  h->Xw, h->xkeep, h->zabs, h->l2sums, h->stream);
+    if (rc) return rc;
+    double sums[20];
+    if (hipMemcpyAsync(sums, h->l2sums, sizeof sums, hipMemcpyDeviceToHost, h->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+        return SZ_ERR_LAUNCH;
+    const double LxLz = h->spec.Lx * h->spec.Lz;
+    for (int f = 0; f < 5; ++f) {
+        const double tr = sums[2 * f] * LxLz, ti = sums[2 * f + 1] * LxLz;
+        const double mr = sums[10 + 2 * f] * LxLz, mi = sums[10 + 2 * f + 1] * LxLz;
+        if (mean) mean[f] = std::sqrt(std::hypot(mr, mi));
+        if (fluct) fluct[f] = std::sqrt(std::hypot(tr - mr, ti - mi));
+    }
+    return 0;
 }
 
 }  // extern "C"

@@ -4,8 +4,8 @@ Reads apps/perfect/manual.gold/channel.rhome_xyz/{initial,restart00000}.h5
 (HDF5 data files the reference's regression check compares against,
 apps/perfect/manual.gold/channel.rhome_xyz/check.sh) with the minimal reader
 in hdf5_read.py and writes channel_rhome_xyz_gold.json: the grid and
-scenario, the B-spline breakpoints, collocation points and integration
-weights, the initial state and the state after advancing to t = 0.000919
+scenario, the B-spline breakpoints, collocation points, integration
+weights and Gram (Galerkin L2) band, the initial state and the state after advancing to t = 0.000919
 with the rhome_xyz hybrid implicit operator, and the running means of the
 implicit (bulk constraint) forcing saved with the restart.
 
@@ -51,6 +51,9 @@ def read_case(path):
         out[s] = int(a) if s in ("Nx", "Ny", "Nz", "k") else _num(a)
     for s in ("breakpoints_y", "collocation_points_y", "integration_weights"):
         out[s] = [float(v) for v in get(s).ravel()]
+    # the Galerkin L2 (Gram) operator, transposed band storage rows
+    g = get("Gy0T")
+    out["Gy0T"] = {"shape": list(g.shape), "values": [float(v) for v in g.ravel()]}
     # state fields: (Nz, Nx/2+1, Ny) complex stored as trailing (re, im)
     state = {}
     for name in FIELDS:

@@ -11,6 +11,9 @@ run) and writes the constant tables they contain as data:
   bspline_integration_golden.json
                        tests/test_bspline.c:108-330 (exact integrals of the
                        order 2, 3, 4 basis functions on breakpoints 0..3)
+  mms_isothermal_channel.json
+                       apps/perfect/nsctpl_rholut.hpp:991-1104 (parameters of
+                       the isothermal channel manufactured solution)
 
 Usage: python tests/golden/make_golden.py [/root/reference]
 """
@@ -142,11 +145,61 @@ def bspline_integration(ref):
             "tolerance_eps_multiple": 1000, "cases": cases}
 
 
+def mms_isothermal_channel(ref):
+    """nsctpl_rholut::isothermal_channel's parameter assignments
+    (apps/perfect/nsctpl_rholut.hpp:991-1104) as numbers: each right hand
+    side is an arithmetic expression of integers and pi, parsed here (no
+    evaluation of anything else)."""
+    import ast
+    import math
+    text = open(os.path.join(ref, "apps", "perfect", "nsctpl_rholut.hpp")).read()
+    start = text.index("template <class T> void isothermal_channel(T& t)")
+    body = text[start:text.index("}", text.index("t.T.g_yz", start)) + 1]
+    allowed = (ast.Expression, ast.BinOp, ast.UnaryOp, ast.Constant, ast.Name,
+               ast.Add, ast.Sub, ast.Mult, ast.Div, ast.USub, ast.Load)
+
+    def value(expr):
+        expr = re.sub(r"scalar\((\d+)\)", r"\1", expr)
+        tree = ast.parse(expr, mode="eval")
+        for node in ast.walk(tree):
+            if not isinstance(node, allowed) or (
+                    isinstance(node, ast.Name) and node.id != "pi"):
+                raise ValueError(f"unexpected expression {expr!r}")
+
+        def ev(n):
+            if isinstance(n, ast.Expression):
+                return ev(n.body)
+            if isinstance(n, ast.Constant):
+                return float(n.value)
+            if isinstance(n, ast.Name):
+                return math.pi
+            if isinstance(n, ast.UnaryOp):
+                return -ev(n.operand)
+            a, b = ev(n.left), ev(n.right)
+            return {ast.Add: a + b, ast.Sub: a - b, ast.Mult: a * b,
+                    ast.Div: a / b}[type(n.op)]
+        return ev(tree)
+
+    out = {"source": "apps/perfect/nsctpl_rholut.hpp:991-1104",
+           "note": "parameters not listed are zero", "scenario": {},
+           "fields": {}}
+    for lhs, rhs in re.findall(r"t\.([A-Za-z_.0-9]+)\s*=\s*([^;]+);", body):
+        v = value(rhs)
+        if "." in lhs:
+            field, param = lhs.split(".")
+            out["fields"].setdefault(field, {})[param] = v
+        else:
+            out["scenario"][lhs] = v
+    return out
+
+
 def main(ref):
     dump(os.path.join(HERE, "bsmbsm_golden.json"), bsmbsm(ref))
     dump(os.path.join(HERE, "bsplineop_golden.json"), bsplineop(ref))
     dump(os.path.join(HERE, "bspline_integration_golden.json"),
          bspline_integration(ref))
+    dump(os.path.join(HERE, "mms_isothermal_channel.json"),
+         mms_isothermal_channel(ref))
 
 
 if __name__ == "__main__":
