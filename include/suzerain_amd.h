@@ -536,6 +536,25 @@ int sz_channel_manufactured_state(sz_channel *h, int which, double t);
  * the rest's (total^2 = mean^2 + fluctuating^2); waits. */
 int sz_channel_field_L2(sz_channel *h, int which, double *mean, double *fluct);
 
+/* ---------------------------------------------------------- restart files
+ * The reference's restart layout (ESIO over HDF5; suzerain/support/
+ * field.cpp:51-200 save_coefficients/load_coefficients): datasets /rho_E,
+ * /rho_u, /rho_v, /rho_w, /rho of shape (Nz, Nx/2+1, Ny) complex (z
+ * wavenumbers in order, non-negative x wavenumbers, B-spline coefficients),
+ * /t, the basis (/k, /breakpoints_y, /knots, /collocation_points_y), the grid
+ * (/Nx /Ny /Nz /DAFx /DAFz /htdelta /Lx /Ly /Lz /kx /kz), the scenario
+ * (/alpha /beta /gamma /Ma /Pr /Re /bulk_rho /bulk_rho_u /bulk_rho_E) and
+ * the walls (/lower_T ... /upper_w).  Host I/O: save copies the state back,
+ * load copies a file's retained modes into the state (others zero) and sets
+ * the channel's time; a file on a different B-spline basis is refused (the
+ * reference would reproject it).  Ensemble members checkpoint through
+ * sz_ensemble_member.  Errors: -1 with a message. */
+int sz_channel_save(sz_channel *h, int which, const char *path, char *err, int errlen);
+int sz_channel_load(sz_channel *h, int which, const char *path, char *err, int errlen);
+/* The channel specification a restart file describes (solver NULL). */
+int sz_restart_read_spec(const char *path, sz_channel_spec *out, char *err,
+                         int errlen);
+
 /* ------------------------------------------------------------ ensembles
  * No single counterpart in the reference, which advances one case per process
  * (apps/perfect/main_advance.cpp, lowstorage::step per member): an ensemble

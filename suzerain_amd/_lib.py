@@ -90,6 +90,9 @@ SIGNATURES = {
     "sz_bsplineop_gram": (i32, [vp, pd]),
     "sz_mms_isothermal_channel": (None, [vp]),
     "sz_mms_evaluate": (i32, [vp, i32, vp, vp, vp, f64, vp, vp, vp]),
+    "sz_internal_h5_write": (i32, [C.c_char_p, i32, vp, vp, vp, vp, vp, C.c_char_p, i32]),
+    "sz_internal_h5_read": (i32, [C.c_char_p, C.c_char_p, vp, vp, vp, i32, vp, i64,
+                                  C.c_char_p, i32]),
     "sz_bsplineop_D_T": (pd, [vp, i32]),
     "sz_bsplineop_D_T_device": (vp, [vp]),
     "sz_bsplineop_accumulate_complex_batched": (i32, [vp, i32, i32, pd, vp, i32,

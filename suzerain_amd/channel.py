@@ -17,7 +17,7 @@ import numpy as np
 
 from ._lib import load, check, sz_isothermal_bc
 
-__all__ = ["Channel", "laminar_state", "DEFAULT_SCENARIO", "SMR91"]
+__all__ = ["Channel", "laminar_state", "restart_spec", "DEFAULT_SCENARIO", "SMR91"]
 
 # SMR91 alpha, beta, gamma, zeta (suzerain/lowstorage.hpp:1241-1305)
 SMR91 = ((29 / 96, -3 / 40, 1 / 6), (37 / 160, 5 / 24, 1 / 6),
@@ -103,6 +103,12 @@ _SIGS = {
                                                 C.c_double]),
     "sz_channel_field_L2": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p,
                                       C.c_void_p]),
+    "sz_channel_save": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_char_p,
+                                  C.c_int]),
+    "sz_channel_load": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_char_p,
+                                  C.c_int]),
+    "sz_restart_read_spec": (C.c_int, [C.c_char_p, C.c_void_p, C.c_char_p,
+                                       C.c_int]),
     "sz_channel_factor_timing": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p,
                                            C.c_void_p]),
     "sz_channel_substep_timing": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p,
@@ -380,6 +386,20 @@ class Channel:
                                          fluct.ctypes.data), "sz_channel_field_L2")
         return mean, fluct
 
+    # ------------------------------------------------------- restart files
+    def save(self, path, which=0):
+        """Write state `which` and the run's metadata in the reference's
+        restart layout (HDF5)."""
+        err = C.create_string_buffer(512)
+        if _lib().sz_channel_save(self._h, which, str(path).encode(), err, 512):
+            raise IOError(err.value.decode())
+
+    def load(self, path, which=0):
+        """Read a restart file's state into `which` (and the time)."""
+        err = C.create_string_buffer(512)
+        if _lib().sz_channel_load(self._h, which, str(path).encode(), err, 512):
+            raise IOError(err.value.decode())
+
     def factor_timing(self, enable=-1):
         """(summed ms, count) of the factorization launches bracketed by HIP
         events since the last call; enable 1/0 switches bracketing on/off."""
@@ -405,6 +425,25 @@ class Channel:
                 self._h = None
         except Exception:
             pass
+
+
+def restart_spec(path):
+    """Channel(...) keyword arguments describing a restart file's run."""
+    sp = sz_channel_spec()
+    err = C.create_string_buffer(512)
+    if _lib().sz_restart_read_spec(str(path).encode(), C.byref(sp), err, 512):
+        raise IOError(err.value.decode())
+    sc = {k: getattr(sp.scenario, k) for k in ("alpha", "beta", "gamma", "Ma",
+                                               "Pr", "Re")}
+    bc = sp.bc
+    out = dict(Nx=sp.Nx, Ny=sp.Ny, Nz=sp.Nz, dNx=sp.dNx, dNz=sp.dNz, k=sp.k,
+               htdelta=sp.htdelta, Lx=sp.Lx, Ly=sp.Ly, Lz=sp.Lz, scenario=sc,
+               lower=(bc.lower_T, bc.lower_u, bc.lower_v, bc.lower_w),
+               upper=(bc.upper_T, bc.upper_u, bc.upper_v, bc.upper_w))
+    if sp.constrained:
+        out["bulk"] = {f: sp.bulk[i] for i, f in enumerate(BULK_FIELDS)
+                       if sp.bulk[i] == sp.bulk[i]}
+    return out
 
 
 def laminar_state(ch, seed=20261202, amp=1e-2, kmax=8):

@@ -1143,6 +1143,11 @@ int sz_internal_channel_problem(sz_channel *h, double phi, int which, void *out,
 
 hipStream_t sz_internal_channel_stream(sz_channel *h) { return h ? h->stream : nullptr; }
 
+void sz_internal_channel_spec(const sz_channel *h, sz_channel_spec *out)
+{
+    if (h && out) *out = h->spec;
+}
+
 void sz_internal_channel_set_stream(sz_channel *h, hipStream_t s)
 {
     if (h) h->stream = s;

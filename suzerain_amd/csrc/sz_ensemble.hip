@@ -465,13 +465,16 @@ int sz_ensemble_step(sz_ensemble *e, int nsteps)
                 if (!rc) rc = e->enqueue_step();
                 if (!rc) rc = e->capture();
                 if (rc) return rc;
-                continue;
+            } else {
+                rc = hipGraphLaunch(e->exec, e->main) == hipSuccess ? 0 : SZ_ERR_LAUNCH;
             }
-            rc = hipGraphLaunch(e->exec, e->main) == hipSuccess ? 0 : SZ_ERR_LAUNCH;
         } else {
             rc = e->enqueue_step();
         }
         if (rc) return rc;
+        // each member's simulation time moves on by its step
+        for (size_t m = 0; m < e->ch.size(); ++m)
+            sz_channel_set_time(e->ch[m], sz_channel_time(e->ch[m]) + e->dt[m]);
     }
     return order_after(e->sync_ev, e->main, e->user);
 }

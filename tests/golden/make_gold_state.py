@@ -7,7 +7,9 @@ in hdf5_read.py and writes channel_rhome_xyz_gold.json: the grid and
 scenario, the B-spline breakpoints, collocation points, integration
 weights and Gram (Galerkin L2) band, the initial state and the state after advancing to t = 0.000919
 with the rhome_xyz hybrid implicit operator, and the running means of the
-implicit (bulk constraint) forcing saved with the restart.
+implicit (bulk constraint) forcing saved with the restart; and copies
+initial.h5 (a data file) to channel_rhome_xyz_initial.h5 for the restart
+reader's tests.
 
 Usage: python tests/golden/make_gold_state.py [/root/reference]
 """
@@ -73,6 +75,10 @@ def read_case(path):
 
 def main(ref):
     base = os.path.join(ref, CASE)
+    # the initial restart file itself, for the restart reader's tests
+    import shutil
+    shutil.copyfile(os.path.join(base, "initial.h5"),
+                    os.path.join(HERE, "channel_rhome_xyz_initial.h5"))
     data = {
         "source": CASE + "/{initial,restart00000}.h5",
         "note": "one channel advanced from t = 0 to 0.000919 by perfect_advance "
