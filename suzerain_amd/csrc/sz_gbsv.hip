@@ -597,6 +597,20 @@ __device__ __forceinline__ R rowsum16(R v)
     return v;
 }
 
+// Sum over LPS = 16 or 32 lanes delivered to all of them: the DPP row sum,
+// then (32) the other row's sum by an LDS-free swizzle (xor 16 within 32).
+template <int LPS, typename R>
+__device__ __forceinline__ R lanesum(R v)
+{
+    v = rowsum16(v);
+    if constexpr (LPS == 32) {
+        static_assert(sizeof(R) == 4, "32-bit swizzle");
+        const int o = __builtin_amdgcn_ds_swizzle(__builtin_bit_cast(int, v), 0x401F);
+        v += __builtin_bit_cast(R, o);
+    }
+    return v;
+}
+
 // 1/z as conj(z)/|z|^2 after rescaling z by a power of two, so |z|^2
 // neither overflows nor underflows; float uses the hardware reciprocal.
 template <typename R>
@@ -612,7 +626,7 @@ __device__ __forceinline__ cpx<R> rcp_scaled(cpx<R> z)
     return {ldexp(zx * r, -e), ldexp(-zy * r, -e)};
 }
 
-template <typename R, int KL, int KV, bool UT, bool CJ>
+template <typename R, int KL, int KV, bool UT, bool CJ, int LPS = 16>
 __global__ __launch_bounds__(64)
 void k_gbtrs_t16(int n, const cpx<R> *__restrict__ ab, int ldab, long long sab,
                  const int *__restrict__ ipiv, long long sipiv,
@@ -621,13 +635,15 @@ void k_gbtrs_t16(int n, const cpx<R> *__restrict__ ab, int ldab, long long sab,
                  const sz::gsys *__restrict__ gs)
 {
     using T = cpx<R>;
-    constexpr int NS = (KV + 15) / 16 + 1;   // U^T register slots
-    constexpr int NM = (KL + 15) / 16;       // L^T terms per lane
+    static_assert(LPS == 16 || LPS == 32, "lanes per system");
+    constexpr int SPW = 64 / LPS;            // systems per wavefront
+    constexpr int NS = (KV + LPS - 1) / LPS + 1;   // U^T register slots
+    constexpr int NM = (KL + LPS - 1) / LPS;       // L^T terms per lane
     constexpr int PF = T16_PF;
-    static_assert(16 % PF == 0, "queue entries must repeat within a group");
+    static_assert(LPS % PF == 0, "queue entries must repeat within a group");
     extern __shared__ __align__(16) unsigned char smem[];
-    const int lane = threadIdx.x, l = lane & 15, grp = lane >> 4;
-    const int sys = blockIdx.x * 4 + grp;
+    const int lane = threadIdx.x, l = lane & (LPS - 1), grp = lane / LPS;
+    const int sys = blockIdx.x * SPW + grp;
     // Every lane loads from the storage of its system (of system 0 past the
     // end of the batch), unconditionally and at clamped addresses, so the
     // loads in flight are the same on every path and the queue drains only
@@ -652,9 +668,10 @@ void k_gbtrs_t16(int n, const cpx<R> *__restrict__ ab, int ldab, long long sab,
         bs = b + (size_t)s0 * (size_t)sb;
     }
     const int ns = (sys < nsys && !(skip && skip[sys] != 0)) ? nl : 0;
-    const int nw = max(max(__builtin_amdgcn_readlane(ns, 0), __builtin_amdgcn_readlane(ns, 16)),
-                       max(__builtin_amdgcn_readlane(ns, 32),
-                           __builtin_amdgcn_readlane(ns, 48)));
+    const int nw = LPS == 16
+        ? max(max(__builtin_amdgcn_readlane(ns, 0), __builtin_amdgcn_readlane(ns, 16)),
+              max(__builtin_amdgcn_readlane(ns, 32), __builtin_amdgcn_readlane(ns, 48)))
+        : max(__builtin_amdgcn_readlane(ns, 0), __builtin_amdgcn_readlane(ns, 32));
     if (nw == 0) return;
     T *x = reinterpret_cast<T *>(smem) + (size_t)grp * ldx;
 
@@ -662,8 +679,8 @@ void k_gbtrs_t16(int n, const cpx<R> *__restrict__ ab, int ldab, long long sab,
         R yx[NS], yy[NS], qx[PF][NS], qy[PF][NS];
 #pragma unroll
         for (int k = 0; k < NS; ++k) yx[k] = yy[k] = R(0);
-        // column c (wave-uniform, c mod 16 = u16) into one queue entry.
-        // Lane l of slot k reads row KV - u16 - 16k + l; lanes whose row
+        // column c (wave-uniform, c mod LPS = u16) into one queue entry.
+        // Lane l of slot k reads row KV - u16 - LPS k + l; lanes whose row
         // falls outside the U rows 0..KV of the column (masked at use in
         // any case) read the nearest U row instead, so the loads touch the
         // column's U lines only and stay inside the storage for every c.
@@ -671,10 +688,10 @@ void k_gbtrs_t16(int n, const cpx<R> *__restrict__ ab, int ldab, long long sab,
             const T *col = a + (size_t)min(c, nl - 1) * ldab;
 #pragma unroll
             for (int k = 0; k < NS; ++k) {
-                const int r0 = KV - u16 - 16 * k;
+                const int r0 = KV - u16 - LPS * k;
                 int row = r0 + l;
                 if (r0 < 0) row = max(row, 0);
-                if (r0 + 15 > KV) row = min(row, KV);
+                if (r0 + LPS - 1 > KV) row = min(row, KV);
                 const T v = sz::gload(col + row);
                 fx[k] = v.x;
                 fy[k] = v.y;
@@ -688,14 +705,14 @@ void k_gbtrs_t16(int n, const cpx<R> *__restrict__ ab, int ldab, long long sab,
             fetch(c, c, qx[c], qy[c]);
             __builtin_amdgcn_sched_barrier(0);
         }
-        // one group of 16 columns; HEAD: the first groups, where slot k > g
-        // holds rows above row 0
+        // one group of LPS columns; HEAD: the first groups, where slot
+        // k > g holds rows above row 0
         auto group = [&](int g, auto head) {
             constexpr bool HEAD = decltype(head)::value;
             const T bc = bn;
-            bn = sz::gload(bs + min(16 * g + 16 + l, nl - 1));
+            bn = sz::gload(bs + min(LPS * g + LPS + l, nl - 1));
 #pragma unroll
-            for (int u = 0; u < 16; ++u) {
+            for (int u = 0; u < LPS; ++u) {
                 const int q = u % PF;
                 // slots NS-1..1 in two independent chains, then slot 0 (the
                 // newest y, the only term that waits on the last step)
@@ -703,11 +720,11 @@ void k_gbtrs_t16(int n, const cpx<R> *__restrict__ ab, int ldab, long long sab,
                 R sx = R(0), sy = R(0);
 #pragma unroll
                 for (int k = NS - 1; k >= 0; --k) {
-                    // lanes with 1 <= u + 16k - l <= KV hold a term
-                    const int lo = u + 16 * k - KV, hi = u + 16 * k - 1;
+                    // lanes with 1 <= u + LPS k - l <= KV hold a term
+                    const int lo = u + LPS * k - KV, hi = u + LPS * k - 1;
                     if (hi < 0) continue;
                     R ax = qx[q][k], ay = CJ ? -qy[q][k] : qy[q][k];
-                    if (lo > 0 || hi < 15 || (HEAD && k > 0)) {
+                    if (lo > 0 || hi < LPS - 1 || (HEAD && k > 0)) {
                         const bool ok = l >= lo && l <= hi && (!HEAD || k == 0 || g >= k);
                         ax = ok ? ax : R(0);
                         ay = ok ? ay : R(0);
@@ -730,8 +747,8 @@ void k_gbtrs_t16(int n, const cpx<R> *__restrict__ ab, int ldab, long long sab,
                     sx = ax2[0] + ax2[1];
                     sy = ay2[0] + ay2[1];
                 }
-                sx = rowsum16(sx);
-                sy = rowsum16(sy);
+                sx = lanesum<LPS>(sx);
+                sy = lanesum<LPS>(sy);
                 // lane u's slot 0 holds the diagonal U(j, j)
                 const T dg = {qx[q][0], CJ ? -qy[q][0] : qy[q][0]};
                 const T r = sz::cmul(T{bc.x - sx, bc.y - sy}, rcp_scaled(dg));
@@ -739,11 +756,11 @@ void k_gbtrs_t16(int n, const cpx<R> *__restrict__ ab, int ldab, long long sab,
                     yx[0] = r.x;
                     yy[0] = r.y;
                 }
-                fetch(16 * g + u + PF, (u + PF) & 15, qx[q], qy[q]);
+                fetch(LPS * g + u + PF, (u + PF) & (LPS - 1), qx[q], qy[q]);
             }
-            if (16 * g + l < ns) {
-                x[16 * g + l].x = yx[0];
-                x[16 * g + l].y = yy[0];
+            if (LPS * g + l < ns) {
+                x[LPS * g + l].x = yx[0];
+                x[LPS * g + l].y = yy[0];
             }
 #pragma unroll
             for (int k = NS - 1; k > 0; --k) {
@@ -754,7 +771,7 @@ void k_gbtrs_t16(int n, const cpx<R> *__restrict__ ab, int ldab, long long sab,
         };
         // two loops rather than a branch per group, so the queue registers
         // carry from group to group without copies
-        const int ng = (nw + 15) / 16, nh = min(ng, NS - 1);
+        const int ng = (nw + LPS - 1) / LPS, nh = min(ng, NS - 1);
         int g = 0;
         for (; g < nh; ++g) group(g, std::true_type{});
         for (; g < ng; ++g) group(g, std::false_type{});
@@ -762,7 +779,7 @@ void k_gbtrs_t16(int n, const cpx<R> *__restrict__ ab, int ldab, long long sab,
         // retire them here, so the L^T queue starts from an empty counter
         __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
     } else {
-        for (int i = l; i < ns; i += 16) x[i] = sz::gload(bs + i);
+        for (int i = l; i < ns; i += LPS) x[i] = sz::gload(bs + i);
     }
     WAVE_SYNC();
 
@@ -770,14 +787,14 @@ void k_gbtrs_t16(int n, const cpx<R> *__restrict__ ab, int ldab, long long sab,
         R lx[PF][NM], ly[PF][NM];
         int lp[PF];
         // step t: column j = ns - 2 - t, terms r < min(KL, t + 1); a lane
-        // reads multiplier rows r = l + 16m, the last run clamped to the
+        // reads multiplier rows r = l + LPS m, the last run clamped to the
         // column, and the pivot of row j
         auto lfetch = [&](int t, R (&fx)[NM], R (&fy)[NM], int &p) {
             const int jl = max(nl - 2 - t, 0);
             const T *col = a + (size_t)jl * ldab + (KV + 1);
 #pragma unroll
             for (int m = 0; m < NM; ++m) {
-                const T v = sz::gload(col + min(l + 16 * m, KL - 1));
+                const T v = sz::gload(col + min(l + LPS * m, KL - 1));
                 fx[m] = v.x;
                 fy[m] = v.y;
             }
@@ -800,7 +817,7 @@ void k_gbtrs_t16(int n, const cpx<R> *__restrict__ ab, int ldab, long long sab,
                 R sx = R(0), sy = R(0);
 #pragma unroll
                 for (int m = 0; m < NM; ++m) {
-                    const int r = l + 16 * m;
+                    const int r = l + LPS * m;
                     const bool ok = j >= 0 && r < lm;
                     const T xv = x[ok ? j + 1 + r : 0];
                     const R vx = ok ? xv.x : R(0), vy = ok ? xv.y : R(0);
@@ -811,8 +828,8 @@ void k_gbtrs_t16(int n, const cpx<R> *__restrict__ ab, int ldab, long long sab,
                     sy = fma(ax, vy, sy);
                     sy = fma(ay, vx, sy);
                 }
-                sx = rowsum16(sx);
-                sy = rowsum16(sy);
+                sx = lanesum<LPS>(sx);
+                sy = lanesum<LPS>(sy);
                 if (l == 0 && j >= 0) {
                     int p = lp[q] - 1;
                     if (p < j || p >= ns) p = j;
@@ -831,7 +848,7 @@ void k_gbtrs_t16(int n, const cpx<R> *__restrict__ ab, int ldab, long long sab,
         }
     }
     WAVE_SYNC();
-    for (int i = l; i < ns; i += 16) sz::gstore(bs + i, x[i]);
+    for (int i = l; i < ns; i += LPS) sz::gstore(bs + i, x[i]);
 }
 
 // ---------------------------------------------------------------- wave
@@ -1399,6 +1416,21 @@ int trans_code(char t)
     }
 }
 
+// Lanes per system of the transposed single precision solve: 16, or 32
+// with SZ_T16_LANES=32.  Thirty-two lanes double the wavefronts in flight
+// but add a swizzle stage to every step's sum and unroll 32 steps (247
+// VGPRs instead of 200): measured on the precision form's 96^3 step 11.9 ms
+// against 11.2 with 16 lanes, and within 1% at 192^3, so 16 stays the
+// default.
+int t16_lanes(int)
+{
+    static const int forced = [] {
+        const char *e = std::getenv("SZ_T16_LANES");
+        return e ? std::atoi(e) : 0;
+    }();
+    return forced == 32 ? 32 : 16;
+}
+
 // The sixteen-lane transposed solve for the channel band shapes in single
 // precision (the refinement solves of the mixed precision solver); false
 // when the shape or precision has no such kernel.  gs != nullptr: grouped,
@@ -1414,15 +1446,24 @@ bool launch_gbtrs_t16(int tc, int n, int kl, int ku, const void *ab, int ldab,
         return false;
     } else {
         if (tc == 0 || ldx <= 0 || (!gs && n < 2)) return false;
-        const size_t lds = 4 * (size_t)ldx * sizeof(cpx<R>);
+        const int lps = t16_lanes(nsys);
+        const int spw = 64 / lps;
+        const size_t lds = (size_t)spw * (size_t)ldx * sizeof(cpx<R>);
         if (lds > LDS_MAX) return false;
-        const dim3 grid((nsys + 3) / 4);
+        const dim3 grid((nsys + spw - 1) / spw);
         const cpx<R> *a = reinterpret_cast<const cpx<R> *>(ab);
         cpx<R> *x = reinterpret_cast<cpx<R> *>(b);
 #define SZ_T16(KL_, KV_, CJ_)                                                   \
-        hipLaunchKernelGGL((k_gbtrs_t16<R, KL_, KV_, true, CJ_>), grid, dim3(64),  \
-                           lds, stream, n, a, ldab, sab, ipiv, sipiv, x, sb, skip, \
-                           nsys, ldx, gs)
+        do {                                                                   \
+            if (lps == 32)                                                     \
+                hipLaunchKernelGGL((k_gbtrs_t16<R, KL_, KV_, true, CJ_, 32>), grid, \
+                                   dim3(64), lds, stream, n, a, ldab, sab, ipiv, \
+                                   sipiv, x, sb, skip, nsys, ldx, gs);         \
+            else                                                               \
+                hipLaunchKernelGGL((k_gbtrs_t16<R, KL_, KV_, true, CJ_, 16>), grid, \
+                                   dim3(64), lds, stream, n, a, ldab, sab, ipiv, \
+                                   sipiv, x, sb, skip, nsys, ldx, gs);         \
+        } while (0)
         if (kl == 34 && ku == 34) {
             if (tc == 2) SZ_T16(34, 68, true);
             else SZ_T16(34, 68, false);
