@@ -38,7 +38,7 @@ def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--workload", choices=("rk3", "solve", "precision", "ensemble",
-                                          "serving"), default="rk3")
+                                          "serving", "physics"), default="rk3")
     p.add_argument("--steps", type=int, default=40)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-cpu", action="store_true")
@@ -749,6 +749,84 @@ def main_rk3(args):
 PREC_DT = {96: 0.02, 192: 0.01}
 
 
+def mms_workload(Ns=(12, 18, 24, 36), dt=5e-4):
+    """The manufactured-solution convergence gate (jobscripts/
+    mms_case.perfect.in: nsctpl isothermal channel, alpha = 1, k = 6,
+    Nx = Nz = 2 Ny / 3, one step of 5e-4 from the exact state): L2 error
+    per field against the exact solution for the FP64 and the mixed
+    precision solves, and the observed orders between successive Ny."""
+    from suzerain_amd.channel import Channel
+    from suzerain_amd.mms import Manufactured
+    out = {"Ny": list(Ns), "dt": dt, "k": 6}
+    for solver in ("zgbsv", PREC_SPEC):
+        errs = []
+        for N in Ns:
+            m = Manufactured.isothermal_channel().set(alpha=1.0)
+            Nxz = 2 * N // 3
+            sc = {q: getattr(m.s, q) for q in ("alpha", "beta", "gamma", "Ma", "Pr", "Re")}
+            ch = Channel(Nx=Nxz, Ny=N, Nz=Nxz, dNx=3 * Nxz // 2, dNz=3 * Nxz // 2, k=6,
+                         htdelta=0.0, Lx=m.s.Lx, Ly=m.s.Ly, Lz=m.s.Lz, scenario=sc,
+                         solver=solver)
+            ch.set_manufactured(m)
+            ch.set_time(0.0)
+            ch.manufactured_state(0.0, which=0)
+            ch.step(max_dt=dt)
+            ch.manufactured_state(ch.time, which=1)
+            ch.add_scaled(-1.0, 1, 0)
+            mean, fluct = ch.field_L2(0)
+            errs.append(np.hypot(mean, fluct))
+            del ch
+        errs = np.array(errs)
+        orders = [np.log(errs[i] / errs[i + 1]) / np.log(Ns[i + 1] / Ns[i])
+                  for i in range(len(Ns) - 1)]
+        key = "fp64" if solver == "zgbsv" else "mixed_precision"
+        out[key] = {"l2_error_per_field": [[float(f"{v:.4e}") for v in e] for e in errs],
+                    "observed_order_per_field": [[round(float(v), 3) for v in o]
+                                                 for o in orders],
+                    "min_order": round(float(np.min(orders)), 3)}
+    out["max_order_difference"] = float(f"""{np.max(np.abs(
+        np.array(out['fp64']['observed_order_per_field'])
+        - np.array(out['mixed_precision']['observed_order_per_field']))):.3e}""")
+    return out
+
+
+def constrained_workload(steps=5, dt=0.01):
+    """The 96^3 FP64 step with the channel driver's bulk constraints (rho
+    and rho u held at the laminar state's bulk values, main_advance.cpp:
+    259-272) against the unconstrained step: per-step time and the bulk
+    deviation after the steps."""
+    import torch
+    from suzerain_amd.channel import Channel, laminar_state
+    cfg = dict(Nx=96, Ny=96, Nz=96, dNx=144, dNz=144, k=8, htdelta=3.0)
+    out = {"steps": steps, "dt": dt}
+    W0 = None
+    for name, bulk in (("unconstrained", None), ("constrained", {"rho": 1.0, "rho_u": 1.0})):
+        ch = Channel(**cfg, bulk=bulk)
+        if W0 is None:
+            W0 = torch.from_numpy(laminar_state(ch)).to("cuda:0")
+        ch.set_state(W0)
+        ch.step(max_dt=dt)                      # untimed
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ch.step(max_dt=dt)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        r = {"ms_per_step": round(ms, 3)}
+        if bulk:
+            w = ch.bulk_weights()
+            s = ch.get_state(0).cpu().numpy()
+            r["bulk_deviation"] = {"rho": float(f"{abs(np.dot(w, s[4, 0, 0].real) - 1):.2e}"),
+                                   "rho_u": float(f"{abs(np.dot(w, s[1, 0, 0].real) - 1):.2e}")}
+            r["multipliers_last_substep"] = [float(f"{v:.4e}") for v in ch.constraint_multipliers()]
+        out[name] = r
+        del ch
+        torch.cuda.empty_cache()
+    out["overhead"] = round(out["constrained"]["ms_per_step"]
+                            / out["unconstrained"]["ms_per_step"] - 1, 4)
+    return out
+
+
 def extra_keys(args):
     """The precision form (96^3 and 192^3), the 48-member ensemble and the
     serving trace, each against the plain FP64 drop-in on the same work."""
@@ -769,6 +847,9 @@ def extra_keys(args):
     if args.requests:
         lo, hi = (int(x) for x in args.req_steps.split(","))
         out["serving"] = serving_workload(args.requests, args.rate, (lo, hi))
+    if getattr(args, "physics", True):
+        out["mms"] = mms_workload()
+        out["bulk_constraints"] = constrained_workload()
     return out
 
 
@@ -789,12 +870,14 @@ def main():
     if args.workload == "rk3":
         main_rk3(args)
         return
-    if args.workload in ("precision", "ensemble", "serving"):
+    if args.workload in ("precision", "ensemble", "serving", "physics"):
         keep = {"precision": ("members", "requests"),
                 "ensemble": ("prec_sizes", "requests"),
-                "serving": ("prec_sizes", "members")}[args.workload]
+                "serving": ("prec_sizes", "members"),
+                "physics": ("prec_sizes", "members", "requests")}[args.workload]
         for k in keep:
             setattr(args, k, "" if k == "prec_sizes" else 0)
+        args.physics = args.workload == "physics"
         print(json.dumps(extra_keys(args)), flush=True)
         return
     out = solve_workload(args.steps, args.warmup, cpu=not args.no_cpu)

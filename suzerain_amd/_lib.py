@@ -90,9 +90,6 @@ SIGNATURES = {
     "sz_bsplineop_gram": (i32, [vp, pd]),
     "sz_mms_isothermal_channel": (None, [vp]),
     "sz_mms_evaluate": (i32, [vp, i32, vp, vp, vp, f64, vp, vp, vp]),
-    "sz_internal_h5_write": (i32, [C.c_char_p, i32, vp, vp, vp, vp, vp, C.c_char_p, i32]),
-    "sz_internal_h5_read": (i32, [C.c_char_p, C.c_char_p, vp, vp, vp, i32, vp, i64,
-                                  C.c_char_p, i32]),
     "sz_bsplineop_D_T": (pd, [vp, i32]),
     "sz_bsplineop_D_T_device": (vp, [vp]),
     "sz_bsplineop_accumulate_complex_batched": (i32, [vp, i32, i32, pd, vp, i32,
@@ -165,6 +162,10 @@ def load():
         raise ImportError(
             f"{LIBPATH} is missing: build it with `python -m suzerain_amd.build`"
             " (there is no CPU fallback)")
+    # PyTorch (device memory and streams) brings its own HIP runtime under
+    # the same soname: load it first, so the library binds to that one
+    # runtime instead of loading a second copy from /opt/rocm next to it
+    import torch  # noqa: F401
     lib = C.CDLL(LIBPATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -473,7 +473,9 @@ sz_channel *sz_internal_channel_create(const sz_channel_spec *spec, void *stream
         h->bop = sz_bsplineop_alloc(sp.k, nbreak, bp.data(), 2, &info);
         if (!h->bop) throw std::runtime_error("sz_bsplineop_alloc failed");
         h->fft = sz_fft_plan_create(sp.Ny, sp.Nx, sp.dNx, sp.Nz, sp.dNz, &info);
-        if (!h->fft) throw std::runtime_error("sz_fft_plan_create failed");
+        if (!h->fft)
+            throw std::runtime_error("sz_fft_plan_create failed (info " +
+                                     std::to_string(info) + ")");
         sz_imexop_scenario is{sp.scenario.Re, sp.scenario.Pr, sp.scenario.Ma,
                               sp.scenario.alpha, sp.scenario.gamma};
         h->imx = sz_imexop_create(&is, h->bop, &info);
