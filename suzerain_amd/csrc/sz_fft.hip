@@ -28,6 +28,7 @@
 #include "sz_common.hpp"
 
 #include <cmath>
+#include <cstdlib>
 #include <new>
 #include <utility>
 #include <vector>
@@ -37,7 +38,9 @@ namespace {
 using sz::zd;
 constexpr int NT = 256;
 constexpr int MAXF = 16;
-constexpr size_t FFT_LDS = 72 * 1024;
+// LDS per block a plan may ask for: 80 KB admits the in-place 16-line z
+// tile at length 288 (78 KB, two blocks per CU)
+constexpr size_t FFT_LDS = 80 * 1024;
 
 struct radix_plan {
     int n, nf;
@@ -416,9 +419,17 @@ std::vector<zd> twiddles(int n)
 }
 
 // (length, tile) pairs compiled with constant sizes; they transform in place
+// (one LDS buffer).  144 and 288 are the dealiased lengths of 96^3 and 192^3.
+#define SZ_FFT_CONST_SIZES(X)                                                  \
+    X(48, 16) X(48, 8) X(72, 16) X(72, 8) X(96, 16) X(96, 8) X(144, 16)         \
+    X(144, 8) X(192, 16) X(192, 8) X(288, 16) X(288, 8)
+
 bool const_instance(int n, int yt)
 {
-    return (yt == 16 || yt == 8) && (n == 144 || n == 96 || n == 192 || n == 72 || n == 48);
+#define SZ_FFT_MATCH(NV, YV) if (n == NV && yt == YV) return true;
+    SZ_FFT_CONST_SIZES(SZ_FFT_MATCH)
+#undef SZ_FFT_MATCH
+    return false;
 }
 
 size_t lds_z_for(int dNz, int yt)
@@ -439,33 +450,21 @@ size_t lds_x(const sz_fft_plan *p) { return lds_x_for(p->g.dNx, p->g.xkeep, p->g
 
 }  // namespace
 
-// Launch a kernel template for the (length, tile) pairs compiled with
-// constant sizes, else its runtime-size instance.
-#define SZ_FFT_LAUNCH(KERN, N_, YT_, GRID, LDS, ST, ...)                        \
-    do {                                                                      \
-        if ((N_) == 144 && (YT_) == 16)                                       \
-            hipLaunchKernelGGL((KERN<144, 16>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
-        else if ((N_) == 144 && (YT_) == 8)                                   \
-            hipLaunchKernelGGL((KERN<144, 8>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
-        else if ((N_) == 96 && (YT_) == 16)                                   \
-            hipLaunchKernelGGL((KERN<96, 16>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
-        else if ((N_) == 96 && (YT_) == 8)                                    \
-            hipLaunchKernelGGL((KERN<96, 8>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
-        else if ((N_) == 192 && (YT_) == 16)                                  \
-            hipLaunchKernelGGL((KERN<192, 16>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
-        else if ((N_) == 192 && (YT_) == 8)                                   \
-            hipLaunchKernelGGL((KERN<192, 8>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
-        else if ((N_) == 72 && (YT_) == 16)                                   \
-            hipLaunchKernelGGL((KERN<72, 16>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
-        else if ((N_) == 72 && (YT_) == 8)                                    \
-            hipLaunchKernelGGL((KERN<72, 8>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
-        else if ((N_) == 48 && (YT_) == 16)                                   \
-            hipLaunchKernelGGL((KERN<48, 16>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
-        else if ((N_) == 48 && (YT_) == 8)                                    \
-            hipLaunchKernelGGL((KERN<48, 8>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
-        else                                                                  \
-            hipLaunchKernelGGL((KERN<0, 0>), GRID, dim3(NT), LDS, ST, __VA_ARGS__); \
-    } while (0)
+// Call f(N, YT) with the (length, tile) pair as compile-time constants when
+// it is one of the constant-size instances, else with (0, 0), the
+// runtime-size instance.
+template <typename F>
+void with_size(int n, int yt, F &&f)
+{
+#define SZ_FFT_CALL(NV, YV)                                                    \
+    if (n == NV && yt == YV) {                                                \
+        f(std::integral_constant<int, NV>{}, std::integral_constant<int, YV>{}); \
+        return;                                                               \
+    }
+    SZ_FFT_CONST_SIZES(SZ_FFT_CALL)
+#undef SZ_FFT_CALL
+    f(std::integral_constant<int, 0>{}, std::integral_constant<int, 0>{});
+}
 
 extern "C" {
 
@@ -490,9 +489,11 @@ sz_fft_plan *sz_fft_plan_create(int Ny, int Nx, int dNx, int Nz, int dNz,
     g.Xw = dNx / 2 + 1;
     g.xkeep = (Nx - 1) / 2 + 1;
     g.zabs = (Nz - 1) / 2;
-    // z: 16 lines per block (256-byte runs of y); x: 8 (measured faster
-    // for the c2r/r2c pair at 144)
+    // z: 16 lines per block (256-byte runs of y), or 8 with
+    // SZ_FFT_ZLINES=8; x: 8 (measured faster for the c2r/r2c pair at 144)
     int yt = 16;
+    if (const char *e = std::getenv("SZ_FFT_ZLINES"))
+        if (std::atoi(e) == 8) yt = 8;
     while (yt > 2 && lds_z_for(dNz, yt) > FFT_LDS) yt /= 2;
     int ytx = 8;
     while (ytx > 2 && lds_x_for(dNx, g.xkeep, ytx) > FFT_LDS) ytx /= 2;
@@ -542,13 +543,18 @@ int sz_fft_wave_to_physical(const sz_fft_plan *p, int nfields,
     const fft_geom &g = p->g;
     const int ntile = (g.Ny + g.yt - 1) / g.yt, ntx = (g.Ny + g.ytx - 1) / g.ytx;
     hipStream_t st = (hipStream_t)stream;
-    SZ_FFT_LAUNCH(k_fft_z, g.dNz, g.yt, dim3(g.xkeep * ntile, nfields), lds_z(p),
-                  st, g, p->pz, p->twz, +1,
-                  reinterpret_cast<const zd *>(wave), swave,
-                  reinterpret_cast<zd *>(mixed), smixed);
-    SZ_FFT_LAUNCH(k_fft_x_c2r, g.dNx, g.ytx, dim3(g.dNz * ntx, nfields), lds_x(p),
-                  st, g, p->px, p->twx,
-                  reinterpret_cast<const zd *>(mixed), smixed, phys, sphys);
+    with_size(g.dNz, g.yt, [&](auto N, auto Y) {
+        hipLaunchKernelGGL((k_fft_z<decltype(N)::value, decltype(Y)::value>),
+                           dim3(g.xkeep * ntile, nfields), dim3(NT), lds_z(p), st,
+                           g, p->pz, p->twz, +1, reinterpret_cast<const zd *>(wave),
+                           swave, reinterpret_cast<zd *>(mixed), smixed);
+    });
+    with_size(g.dNx, g.ytx, [&](auto N, auto Y) {
+        hipLaunchKernelGGL((k_fft_x_c2r<decltype(N)::value, decltype(Y)::value>),
+                           dim3(g.dNz * ntx, nfields), dim3(NT), lds_x(p), st,
+                           g, p->px, p->twx, reinterpret_cast<const zd *>(mixed),
+                           smixed, phys, sphys);
+    });
     return hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH;
 }
 
@@ -563,13 +569,18 @@ int sz_fft_physical_to_wave(const sz_fft_plan *p, int nfields,
     const fft_geom &g = p->g;
     const int ntile = (g.Ny + g.yt - 1) / g.yt, ntx = (g.Ny + g.ytx - 1) / g.ytx;
     hipStream_t st = (hipStream_t)stream;
-    SZ_FFT_LAUNCH(k_fft_x_r2c, g.dNx, g.ytx, dim3(g.dNz * ntx, nfields), lds_x(p),
-                  st, g, p->px, p->twx, phys, sphys,
-                  reinterpret_cast<zd *>(mixed), smixed);
-    SZ_FFT_LAUNCH(k_fft_z, g.dNz, g.yt, dim3(g.Xw * ntile, nfields), lds_z(p),
-                  st, g, p->pz, p->twz, -1,
-                  reinterpret_cast<const zd *>(mixed), smixed,
-                  reinterpret_cast<zd *>(wave), swave);
+    with_size(g.dNx, g.ytx, [&](auto N, auto Y) {
+        hipLaunchKernelGGL((k_fft_x_r2c<decltype(N)::value, decltype(Y)::value>),
+                           dim3(g.dNz * ntx, nfields), dim3(NT), lds_x(p), st,
+                           g, p->px, p->twx, phys, sphys,
+                           reinterpret_cast<zd *>(mixed), smixed);
+    });
+    with_size(g.dNz, g.yt, [&](auto N, auto Y) {
+        hipLaunchKernelGGL((k_fft_z<decltype(N)::value, decltype(Y)::value>),
+                           dim3(g.Xw * ntile, nfields), dim3(NT), lds_z(p), st,
+                           g, p->pz, p->twz, -1, reinterpret_cast<const zd *>(mixed),
+                           smixed, reinterpret_cast<zd *>(wave), swave);
+    });
     return hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH;
 }
 

@@ -306,6 +306,16 @@ __device__ inline zd element(const pack_tab &pt, const bops &w, const zd *Wr,
     return zd{ex, ey};
 }
 
+// v ^ off lane exchange of a double (every lane active), from the lane
+// index the caller holds
+__device__ __forceinline__ double xor_lane(double v, int lane, int off)
+{
+    const int addr = (lane ^ off) << 2;
+    const int lo = __builtin_amdgcn_ds_bpermute(addr, __double2loint(v));
+    const int hi = __builtin_amdgcn_ds_bpermute(addr, __double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
 // Extra outputs of the single precision assembly (factor storage of the
 // mixed precision solve): systems with skip[s] != 0 are left alone; for the
 // others the wall-row scalings as stored (rounded to float), indexed
@@ -336,7 +346,7 @@ __device__ __forceinline__ void
 pack_body(const table *__restrict__ tab, const bops &w, const refs &rf, zd phi,
           double kms, double kns, int S_, int4 fof_lo, int fof_hi, int rho_pos,
           int KL, int KU, OUT *__restrict__ P, int ldpapt, int chunk,
-          const wallbc &bc, zd *bcs, double *frob)
+          const wallbc &bc, zd *bcs, double *frob, int tid)
 {
     extern __shared__ double smem[];
     zd *alpha = reinterpret_cast<zd *>(smem);                   // MAXTERMS
@@ -352,7 +362,6 @@ pack_body(const table *__restrict__ tab, const bops &w, const refs &rf, zd phi,
     const int n = w.n, N = S * n;
     const int r0 = chunk * PK_RB;
     const int nr = min(PK_RB, n - r0);
-    const int tid = threadIdx.x;
     // field of interleave position p (selects: a small local array indexed
     // per lane would live in scratch)
     auto field_of = [&](int p) __attribute__((always_inline)) {
@@ -441,7 +450,7 @@ pack_body(const table *__restrict__ tab, const bops &w, const refs &rf, zd phi,
     }
     if (frob) {
         __shared__ double red[PK_NT / 64];
-        for (int off = 32; off >= 1; off >>= 1) fsum += __shfl_xor(fsum, off);
+        for (int off = 32; off >= 1; off >>= 1) fsum += xor_lane(fsum, tid & 63, off);
         if ((tid & 63) == 0) red[tid >> 6] = fsum;
         __syncthreads();
         if (tid == 0) {
@@ -466,7 +475,8 @@ void k_imexop_pack(const table *__restrict__ tab, bops w, refs rf, zd phi,
     pack_body<SS, OUT>(tab, w, rf, phi, km[s], kn[s], S_, fof_lo, fof_hi, rho_pos,
                        KL, KU, papt + (size_t)s * stride, ldpapt, chunk, bc,
                        ex.bcs ? ex.bcs + (size_t)s * 8 : nullptr,
-                       ex.frob ? ex.frob + (size_t)s * nchunks + chunk : nullptr);
+                       ex.frob ? ex.frob + (size_t)s * nchunks + chunk : nullptr,
+                       threadIdx.x);
 }
 
 // One member (ensemble case) of a grouped launch: its operator, wavenumbers,
@@ -499,24 +509,50 @@ __device__ __forceinline__ int member_of(const gmember *__restrict__ m, int nm,
     return lo;
 }
 
-// Grouped single precision packf of every member's five-field operator with
-// isothermal walls (systems with skip[s] != 0 left alone).
-__global__ __launch_bounds__(PK_NT)
-void k_imexop_pack_grouped(const gmember *__restrict__ mem, int nm,
-                           const int *__restrict__ skip,
-                           double *__restrict__ frob)
+// Member holding global system s: the last member whose first system is
+// <= s.
+__device__ __forceinline__ int member_of_sys(const gmember *__restrict__ m,
+                                             int nm, int s)
 {
-    const gmember &M = mem[member_of(mem, nm, blockIdx.x)];
-    const int b = blockIdx.x - M.blk0;
-    const int ls = b / M.nchunks, chunk = b - ls * M.nchunks;
-    const int s = M.sys0 + ls;
-    if (skip && skip[s]) return;                 // uniform over the block
-    const int N = 5 * M.w.n;
-    pack_body<5, sz::zf>(M.tab, M.w, M.rf, M.phi, M.km[ls], M.kn[ls], 5,
-                         make_int4(0, 1, 2, 3), 4, 4, M.KL, M.KU,
-                         M.lu + (size_t)ls * M.ldlu * N + M.KL, M.ldlu, chunk,
-                         M.bc, M.bcs + (size_t)ls * 8,
-                         frob + M.frob0 + (size_t)ls * M.nchunks + chunk);
+    int lo = 0, hi = nm - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (m[mid].sys0 <= s) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Grouped single precision packf of the systems list[0 .. *count) (global
+// system indices, any order): each block takes every gridDim.x-th entry and
+// assembles all of that system's row chunks, so a pass with few or no
+// systems to factor costs a handful of blocks instead of one per chunk.
+__global__ __launch_bounds__(PK_NT)
+void k_imexop_pack_listed(const gmember *__restrict__ mem, int nm,
+                          const int *__restrict__ list,
+                          const int *__restrict__ count,
+                          double *__restrict__ frob)
+{
+    // uniform values made explicit, so the member's descriptor stays scalar
+    const int cnt = __builtin_amdgcn_readfirstlane(*count);
+    for (int j = blockIdx.x; j < cnt; j += gridDim.x) {
+        const int s = __builtin_amdgcn_readfirstlane(list[j]);
+        const gmember &M = mem[member_of_sys(mem, nm, s)];
+        const int ls = s - M.sys0;
+        const int N = 5 * M.w.n;
+        for (int chunk = 0; chunk < M.nchunks; ++chunk) {
+            __syncthreads();    // the previous chunk's LDS tables are read out
+            // the thread index as a value the compiler cannot hold across
+            // chunks: what it derives is recomputed per chunk instead of
+            // occupying registers through the whole loop
+            int tid = threadIdx.x;
+            asm volatile("" : "+v"(tid));
+            pack_body<5, sz::zf>(M.tab, M.w, M.rf, M.phi, M.km[ls], M.kn[ls], 5,
+                                 make_int4(0, 1, 2, 3), 4, 4, M.KL, M.KU,
+                                 M.lu + (size_t)ls * M.ldlu * N + M.KL, M.ldlu,
+                                 chunk, M.bc, M.bcs + (size_t)ls * 8,
+                                 frob + M.frob0 + (size_t)ls * M.nchunks + chunk, tid);
+        }
+    }
 }
 
 // the x, D x buffers take most of a CU's LDS (one workgroup per CU), so the
@@ -636,38 +672,36 @@ void k_imexop_accumulate(const table *__restrict__ tab, bops w, refs rf, zd phi,
     acc_body<STAGED>(tab, w, rf, phi, km[s], kn[s], in, out, beta, beta_zero);
 }
 
-__device__ __forceinline__ double block_sum_ac(double v, double *red)
+// Block sum (AC_NT threads, tid = the thread index); every thread gets it.
+__device__ __forceinline__ double block_sum_ac(double v, double *red, int tid)
 {
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-    const int w = threadIdx.x >> 6;
+    const int lane = tid & 63;
+    for (int off = 32; off >= 1; off >>= 1) v += xor_lane(v, lane, off);
+    const int w = tid >> 6;
     __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[w] = v;
+    if (lane == 0) red[w] = v;
     __syncthreads();
     double t = 0;
     for (int i = 0; i < AC_NT / 64; ++i) t += red[i];
     return t;
 }
 
-// One refinement sweep of the mixed precision solve for every member's
-// active systems (one block each): x += P^T c (c the float correction in
-// a.PB), r = b - (M + phi L) x with the isothermal wall rows evaluated as
-// the constraints s (x_f - fac x_rho) = 0 whose scalings s the factors hold,
-// ||r||_2 and ||x||_2, the decision of dsgbsvx.def:186-227, and for the
-// systems that continue the next right hand side P r in float.  The operator
-// product is the accumulate body's arithmetic.
-__global__ __launch_bounds__(AC_NT)
-void k_imexop_sweep_grouped(const gmember *__restrict__ mem, int nm,
-                            sz_prec_sweep a)
+// One refinement sweep of the mixed precision solve for global system s,
+// member-local index ls, by the whole block: x += P^T c (c the float
+// correction in a.PB), r = b - (M + phi L) x with the isothermal wall rows
+// evaluated as the constraints s (x_f - fac x_rho) = 0 whose scalings s the
+// factors hold, ||r||_2 and ||x||_2, the decision of dsgbsvx.def:186-227,
+// and for the systems that continue the next right hand side P r in float.
+// The operator product is the accumulate body's arithmetic.  Returns the
+// system's new state (the same in every thread).
+__device__ __forceinline__ int sweep_body(const gmember &M, int ls, int s,
+                                          const sz_prec_sweep &a, int tid)
 {
     extern __shared__ double smem[];
-    const gmember &M = mem[member_of(mem, nm, blockIdx.x)];
-    const int ls = blockIdx.x - M.blk0;
-    const int s = M.sys0 + ls;
-    if (a.skip[s]) return;                       // uniform over the block
     const table *__restrict__ tab = M.tab;
     const bops &w = M.w;
     const refs &rf = M.rf;
-    const int n = w.n, N = 5 * n, tid = threadIdx.x;
+    const int n = w.n, N = 5 * n;
     zd *alpha = reinterpret_cast<zd *>(smem);       // MAXTERMS
     zd *xs = alpha + MAXTERMS;                      // 5 x n
     zd *Dx = xs + 5 * n;                            // 3 x 5 x n
@@ -758,8 +792,8 @@ void k_imexop_sweep_grouped(const gmember *__restrict__ mem, int nm,
         ss += v.x * v.x + v.y * v.y;
         xx += xs[e].x * xs[e].x + xs[e].y * xs[e].y;
     }
-    const double res = sqrt(block_sum_ac(ss, red));
-    const double xn = sqrt(block_sum_ac(xx, red));
+    const double res = sqrt(block_sum_ac(ss, red, tid));
+    const double xn = sqrt(block_sum_ac(xx, red, tid));
     const int st = sz_prec_decide(S, res, xn, a.tolsc, a.aiter, a.smax, apprx);
     if (st == SZ_P_ACTIVE) {
         __syncthreads();
@@ -771,6 +805,32 @@ void k_imexop_sweep_grouped(const gmember *__restrict__ mem, int nm,
     if (tid == 0) {
         a.S[s] = S;
         a.skip[s] = st == SZ_P_ACTIVE ? 0 : 1;
+    }
+    return st;
+}
+
+// One refinement sweep over the systems a.list[0 .. *a.count) (any order):
+// each block takes every gridDim.x-th entry, and the systems that continue
+// are appended to a.next (count *a.ncount, cleared by the caller), so that
+// sweeps after convergence cost a handful of blocks.
+__global__ __launch_bounds__(AC_NT) __attribute__((amdgpu_waves_per_eu(8)))
+void k_imexop_sweep_listed(const gmember *__restrict__ mem, int nm,
+                           sz_prec_sweep a)
+{
+    // uniform values made explicit, so the member's descriptor stays scalar
+    const int cnt = __builtin_amdgcn_readfirstlane(*a.count);
+    for (int j = blockIdx.x; j < cnt; j += gridDim.x) {
+        const int s = __builtin_amdgcn_readfirstlane(a.list[j]);
+        const gmember &M = mem[member_of_sys(mem, nm, s)];
+        __syncthreads();        // the previous system's LDS is read out
+        // the thread index as a value the compiler cannot hold across
+        // systems: what it derives is recomputed per system instead of
+        // occupying registers through the whole loop
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const int st = sweep_body(M, s - M.sys0, s, a, tid);
+        if (st == SZ_P_ACTIVE && tid == 0)
+            a.next[atomicAdd(a.ncount, 1)] = s;
     }
 }
 
@@ -1147,18 +1207,20 @@ int sz_internal_imexop_member(const sz_imexop *op, const double *phi,
     return 0;
 }
 
-// Single precision packf of every member's systems (nblocks = the sum of
-// nsys * nchunks; ldmax = the largest B-spline band leading dimension).
-int sz_internal_imexop_pack_grouped(const void *dmem, int nm, int nblocks,
-                                    int ldmax, const int *skip, double *frob,
-                                    void *stream)
+// Single precision packf of the listed systems (list, *count on the device;
+// count <= nsys): a grid of at most 4096 blocks strides over the list.
+int sz_internal_imexop_pack_listed(const void *dmem, int nm, int nsys, int ldmax,
+                                   const int *list, const int *count,
+                                   double *frob, void *stream)
 {
-    if (nblocks <= 0) return 0;
+    if (nsys <= 0) return 0;
     const size_t lds = sizeof(zd) * (MAXTERMS + PK_RB * MAXGROUPS)
                      + sizeof(double) * 3 * PK_RB * (size_t)ldmax;
     if (lds > 64 * 1024) return SZ_ERR_LDS;
-    hipLaunchKernelGGL(k_imexop_pack_grouped, dim3(nblocks), dim3(PK_NT), lds,
-                       (hipStream_t)stream, (const gmember *)dmem, nm, skip, frob);
+    const int grid = nsys < 4096 ? nsys : 4096;
+    hipLaunchKernelGGL(k_imexop_pack_listed, dim3(grid), dim3(PK_NT), lds,
+                       (hipStream_t)stream, (const gmember *)dmem, nm, list, count,
+                       frob);
     return launch_ok() ? 0 : SZ_ERR_LAUNCH;
 }
 
@@ -1180,18 +1242,20 @@ int sz_internal_imexop_apply_grouped(const void *dmem, int nm, int nsys,
     return launch_ok() ? 0 : SZ_ERR_LAUNCH;
 }
 
-// One refinement sweep of the mixed precision solve over every member's
-// systems (nsys blocks; nmax the largest n).
-int sz_internal_imexop_sweep_grouped(const void *dmem, int nm, int nsys,
-                                     int nmax, const sz_prec_sweep *a,
-                                     void *stream)
+// One refinement sweep of the mixed precision solve over the listed systems
+// (a->list, *a->count <= nsys; nmax the largest n): a grid of at most 1024
+// blocks strides over the list.
+int sz_internal_imexop_sweep_listed(const void *dmem, int nm, int nsys,
+                                    int nmax, const sz_prec_sweep *a,
+                                    void *stream)
 {
     if (nsys <= 0) return 0;
     // the sweep keeps the gather: staging the profiles would take the LDS
     // of a third resident block per CU, which costs more than it saves
     const size_t lds = sizeof(zd) * (MAXTERMS + 25 * (size_t)nmax);
     if (lds > 150 * 1024) return SZ_ERR_LDS;
-    hipLaunchKernelGGL(k_imexop_sweep_grouped, dim3(nsys), dim3(AC_NT), lds,
+    const int grid = nsys < 1024 ? nsys : 1024;
+    hipLaunchKernelGGL(k_imexop_sweep_listed, dim3(grid), dim3(AC_NT), lds,
                        (hipStream_t)stream, (const gmember *)dmem, nm, *a);
     return launch_ok() ? 0 : SZ_ERR_LAUNCH;
 }

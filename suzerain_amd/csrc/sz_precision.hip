@@ -64,14 +64,14 @@ extern "C" int sz_internal_imexop_member(
     const sz_imexop_ref *r, const sz_imexop_refld *ld, const sz_isothermal_bc *bc,
     int blk0, int sys0, int nsys, int KL, int KU, int ldlu, void *lu, void *bcs,
     int *apprx, long long v0, long long frob0, void *out);
-extern "C" int sz_internal_imexop_pack_grouped(const void *dmem, int nm,
-                                               int nblocks, int ldmax,
-                                               const int *skip, double *frob,
+extern "C" int sz_internal_imexop_pack_listed(const void *dmem, int nm,
+                                              int nsys, int ldmax,
+                                              const int *list, const int *count,
+                                              double *frob, void *stream);
+extern "C" int sz_internal_imexop_sweep_listed(const void *dmem, int nm,
+                                               int nsys, int nmax,
+                                               const sz_prec_sweep *a,
                                                void *stream);
-extern "C" int sz_internal_imexop_sweep_grouped(const void *dmem, int nm,
-                                                int nsys, int nmax,
-                                                const sz_prec_sweep *a,
-                                                void *stream);
 
 namespace {
 
@@ -132,6 +132,8 @@ struct p_args {
     int *skip, *fskip;
     const double *frob;
     const int *finfo;
+    int *plist, *pcount;         // systems to factor this pass, their count
+    int *slist, *scount;         // systems the first sweep takes, their count
 };
 
 // B <- the state's pencils with the wall rows of the energy and momentum
@@ -206,6 +208,7 @@ __global__ __launch_bounds__(PT) void k_p_init(p_args A)
         A.skip[s] = active ? 0 : 1;
         A.fskip[s] = need ? 0 : 1;
         if (need) {
+            A.plist[atomicAdd(A.pcount, 1)] = s;   // packed in any order
             M.fact[ls] = 'N';
             M.apprx[ls] = 0;
         } else if (active) {
@@ -243,6 +246,7 @@ __global__ void k_p_afterfactor(p_args A, int nsys)
     }
     const double eps = DBL_EPSILON * 0.5;
     S.tolconst = use_eps ? 1.0 : M.afrob[ls] * eps * sqrt(5.0 * d.n) * A.tolsc;
+    if (A.skip[s] == 0) A.slist[atomicAdd(A.scount, 1)] = s;   // any order
 }
 
 // counts[0] <- systems marked P_RESTART, counts[1] <- P_FAIL, counts[2] <-
@@ -415,7 +419,7 @@ struct sz_prec {
     std::vector<int> order;               // internal position -> live index
     std::vector<int> sys0, nchunks;       // per internal position
     std::vector<long long> v0, frob0;
-    int nsys = 0, nmax = 0, ldmax = 0, nblocks_pack = 0;
+    int nsys = 0, nmax = 0, ldmax = 0;
     long long nvec = 0, nfrob = 0;
     struct lu_class {
         int KL, KU, ldlu, s0, ns, nmax;
@@ -425,7 +429,6 @@ struct sz_prec {
     struct slot {
         unsigned char *gmem = nullptr;    // imexop member descriptors (one
                                           // block per system)
-        unsigned char *gpack = nullptr;   // the same with pack block offsets
         pmem *pm = nullptr;               // member descriptors
         std::vector<unsigned char> last;  // host copy of what was uploaded
     };
@@ -442,6 +445,9 @@ struct sz_prec {
     double *frob = nullptr;
     int *counts = nullptr, *hcounts = nullptr;
     int *list = nullptr;
+    int *plist = nullptr, *pcount = nullptr;   // systems to pack this pass
+    int *slist[2] = {nullptr, nullptr};        // systems to sweep, alternating
+    int *scount = nullptr;                     // their counts (2)
     double *skm = nullptr, *skn = nullptr;
     int *o_siter = nullptr, *o_diter = nullptr, *o_info = nullptr,
         *o_first = nullptr, *o_apprx = nullptr;
@@ -471,11 +477,11 @@ struct sz_prec {
     {
         void *ps[] = {D, B, X, PB, S, skip, fskip, finfo, frob, list, skm, skn,
                       o_siter, o_diter, o_info, o_first, o_apprx, o_fact, o_tolsc,
-                      o_res, lu64, pb64, ipiv64, info64, counts};
+                      o_res, lu64, pb64, ipiv64, info64, counts, plist, pcount,
+                      slist[0], slist[1], scount};
         for (void *q : ps) if (q) (void)hipFree(q);
         for (auto &sl : slots) {
             if (sl.gmem) (void)hipFree(sl.gmem);
-            if (sl.gpack) (void)hipFree(sl.gpack);
             if (sl.pm) (void)hipFree(sl.pm);
         }
         for (auto &c : classes)
@@ -515,7 +521,7 @@ void rebuild(sz_prec *h)
     h->nchunks.clear();
     h->v0.clear();
     h->frob0.clear();
-    h->nsys = h->nmax = h->ldmax = h->nblocks_pack = 0;
+    h->nsys = h->nmax = h->ldmax = 0;
     h->nvec = h->nfrob = 0;
     std::vector<pdesc> D;
     for (int i = 0; i < nl; ++i) {
@@ -537,7 +543,6 @@ void rebuild(sz_prec *h)
         h->nsys += m.g.nsys;
         h->nvec += (long long)m.g.nsys * N;
         h->nfrob += (long long)m.g.nsys * nch;
-        h->nblocks_pack += m.g.nsys * nch;
         h->nmax = std::max(h->nmax, m.g.n);
         // B-spline band leading dimension: KL = 5 (kl + 1) - 1
         h->ldmax = std::max(h->ldmax, (m.g.KL + 1) / 5 + (m.g.KU + 1) / 5 - 1);
@@ -551,6 +556,9 @@ void rebuild(sz_prec *h)
         h->realloc_pool(h->fskip, (size_t)c);
         h->realloc_pool(h->finfo, (size_t)c);
         h->realloc_pool(h->list, (size_t)c);
+        h->realloc_pool(h->plist, (size_t)c);
+        h->realloc_pool(h->slist[0], (size_t)c);
+        h->realloc_pool(h->slist[1], (size_t)c);
         h->realloc_pool(h->skm, (size_t)c);
         h->realloc_pool(h->skn, (size_t)c);
         h->realloc_pool(h->o_siter, (size_t)c);
@@ -582,12 +590,13 @@ void rebuild(sz_prec *h)
         const int gbytes = sz_internal_imexop_member_bytes();
         for (auto &sl : h->slots) {
             h->realloc_pool(sl.gmem, (size_t)gbytes * c);
-            h->realloc_pool(sl.gpack, (size_t)gbytes * c);
             h->realloc_pool(sl.pm, (size_t)c);
         }
         h->cap_mem = c;
     }
     if (!h->counts) h->realloc_pool(h->counts, 3);
+    if (!h->pcount) h->realloc_pool(h->pcount, 1);
+    if (!h->scount) h->realloc_pool(h->scount, 2);
     h->pool_bytes = (long long)h->cap_sys * (sizeof(pdesc) + sizeof(psys) + 64) +
                     h->cap_vec * (2 * sizeof(zd) + sizeof(zf)) +
                     h->cap_frob * (long long)sizeof(double);
@@ -823,10 +832,10 @@ int sz_prec_prepare(sz_prec *h, int slot, const sz_prec_problem *pr)
     sz_prec::slot &sl = h->slots[(size_t)slot];
     const int nl = (int)h->live.size();
     const int gbytes = sz_internal_imexop_member_bytes();
-    // [apply descriptors | pack descriptors | pmem]
+    // [member descriptors | pmem]
     const size_t gsz = (size_t)gbytes * nl;
-    std::vector<unsigned char> buf(2 * gsz + sizeof(pmem) * nl);
-    int blk = 0, pblk = 0;
+    std::vector<unsigned char> buf(gsz + sizeof(pmem) * nl);
+    int blk = 0;
     for (int i = 0; i < nl; ++i) {
         const int k = h->order[(size_t)i];
         const sz_prec::mstore &m = h->ms[(size_t)h->live[(size_t)k]];
@@ -839,13 +848,7 @@ int sz_prec_prepare(sz_prec *h, int slot, const sz_prec_problem *pr)
             m.g.nsys, m.g.KL, m.g.KU, ldlu, sd.lu, sd.bcs, sd.apprx, h->v0[(size_t)i],
             h->frob0[(size_t)i], buf.data() + (size_t)gbytes * i);
         if (rc) return rc;
-        rc = sz_internal_imexop_member(
-            q.op, phi2, q.km, q.kn, q.ref, q.ld, q.bc, pblk, h->sys0[(size_t)i],
-            m.g.nsys, m.g.KL, m.g.KU, ldlu, sd.lu, sd.bcs, sd.apprx, h->v0[(size_t)i],
-            h->frob0[(size_t)i], buf.data() + gsz + (size_t)gbytes * i);
-        if (rc) return rc;
-        blk += m.g.nsys;                          // one accumulate block a system
-        pblk += m.g.nsys * h->nchunks[(size_t)i]; // nchunks pack blocks a system
+        blk += m.g.nsys;                          // one sweep block a system
         pmem pm{};
         pm.state = q.state;
         pm.F = q.F;
@@ -863,16 +866,14 @@ int sz_prec_prepare(sz_prec *h, int slot, const sz_prec_problem *pr)
         pm.apprx = sd.apprx;
         pm.afrob = sd.afrob;
         pm.bcs = sd.bcs;
-        std::memcpy(buf.data() + 2 * gsz + sizeof(pmem) * i, &pm, sizeof(pm));
+        std::memcpy(buf.data() + gsz + sizeof(pmem) * i, &pm, sizeof(pm));
     }
     if (buf == sl.last) return 0;
     // the previous contents may still be read by enqueued work
     if (hipStreamSynchronize(h->stream) != hipSuccess ||
         (gsz && hipMemcpy(sl.gmem, buf.data(), gsz, hipMemcpyHostToDevice) !=
                     hipSuccess) ||
-        (gsz && hipMemcpy(sl.gpack, buf.data() + gsz, gsz, hipMemcpyHostToDevice) !=
-                    hipSuccess) ||
-        (nl && hipMemcpy(sl.pm, buf.data() + 2 * gsz, sizeof(pmem) * nl,
+        (nl && hipMemcpy(sl.pm, buf.data() + gsz, sizeof(pmem) * nl,
                          hipMemcpyHostToDevice) != hipSuccess))
         return SZ_ERR_LAUNCH;
     sl.last.swap(buf);
@@ -887,10 +888,11 @@ int single_pass(sz_prec *h, sz_prec::slot &sl, int slot, const p_args &A0, int p
     p_args A = A0;
     A.pass = pass;
     const int nm = (int)h->live.size();
+    hipLaunchKernelGGL(k_p_fill_int, dim3(1), dim3(PT), 0, st, h->pcount, 0, 1);
     hipLaunchKernelGGL(k_p_init, dim3(h->nsys), dim3(PT), 0, st, A);
     if (!ok()) return SZ_ERR_LAUNCH;
-    int rc = sz_internal_imexop_pack_grouped(sl.gpack, nm, h->nblocks_pack, h->ldmax,
-                                             h->fskip, h->frob, st);
+    int rc = sz_internal_imexop_pack_listed(sl.gmem, nm, h->nsys, h->ldmax, h->plist,
+                                            h->pcount, h->frob, st);
     if (rc) return rc;
     for (const auto &c : h->classes) {
         rc = sz_internal_gbtrf_grouped(1, c.KL, c.KU, nullptr, c.ldlu, nullptr,
@@ -898,6 +900,7 @@ int single_pass(sz_prec *h, sz_prec::slot &sl, int slot, const p_args &A0, int p
                                        c.gs[(size_t)slot], c.ns, h->p.lu_bs, st);
         if (rc) return rc;
     }
+    hipLaunchKernelGGL(k_p_fill_int, dim3(1), dim3(PT), 0, st, h->scount, 0, 1);
     hipLaunchKernelGGL(k_p_afterfactor, dim3(nb(h->nsys)), dim3(PT), 0, st, A,
                        h->nsys);
     if (!ok()) return SZ_ERR_LAUNCH;
@@ -919,7 +922,15 @@ int single_pass(sz_prec *h, sz_prec::slot &sl, int slot, const p_args &A0, int p
                                            c.gs[(size_t)slot], c.ns, c.nmax, st);
             if (rc) return rc;
         }
-        rc = sz_internal_imexop_sweep_grouped(sl.gmem, nm, h->nsys, h->nmax, &sw, st);
+        // the active systems alternate between two lists
+        const int cur = it & 1, nxt = cur ^ 1;
+        sw.list = h->slist[cur];
+        sw.count = h->scount + cur;
+        sw.next = h->slist[nxt];
+        sw.ncount = h->scount + nxt;
+        hipLaunchKernelGGL(k_p_fill_int, dim3(1), dim3(PT), 0, st, h->scount + nxt,
+                           0, 1);
+        rc = sz_internal_imexop_sweep_listed(sl.gmem, nm, h->nsys, h->nmax, &sw, st);
         if (rc) return rc;
     }
     return 0;
@@ -1051,6 +1062,10 @@ int sz_prec_solve(sz_prec *h, int slot, const sz_prec_problem *pr)
     A.fskip = h->fskip;
     A.frob = h->frob;
     A.finfo = h->finfo;
+    A.plist = h->plist;
+    A.pcount = h->pcount;
+    A.slist = h->slist[0];
+    A.scount = h->scount;
     // 1. right hand sides with the wall rows zeroed, x = 0
     hipLaunchKernelGGL(k_p_gather, dim3(h->nsys), dim3(PT), 0, st, A);
     if (!ok()) return SZ_ERR_LAUNCH;

@@ -77,6 +77,8 @@ struct sz_prec_sweep {
     int *skip;
     double tolsc;
     int aiter, smax;
+    const int *list, *count;     // the systems to sweep, their number
+    int *next, *ncount;          // the systems that continue, appended
 };
 
 // One member's systems: n B-spline coefficients per field (N = 5n), the

@@ -35,10 +35,23 @@ def forward_numpy(P, keep):
 
 
 @pytest.mark.parametrize("Ny,Nx,Nz", [(96, 96, 96), (12, 32, 48), (7, 64, 128),
-                                      (5, 384, 384), (9, 4, 6)])
+                                      (7, 192, 192), (6, 144, 144), (5, 384, 384),
+                                      (9, 4, 6)])
 def test_transforms_match_numpy(gpu, Ny, Nx, Nz):
+    _check_transforms(gpu, Ny, Nx, Nz)
+
+
+@pytest.mark.parametrize("Ny,Nx,Nz", [(96, 96, 96), (7, 192, 192)])
+def test_eight_line_z_tiles(gpu, monkeypatch, Ny, Nx, Nz):
+    # SZ_FFT_ZLINES=8 selects the 8-line z tile at plan creation
+    monkeypatch.setenv("SZ_FFT_ZLINES", "8")
+    _check_transforms(gpu, Ny, Nx, Nz)
+
+
+def _check_transforms(gpu, Ny, Nx, Nz):
     import torch
     from suzerain_amd.fft import PencilFFT
+    print(f"case Ny={Ny} Nx={Nx} Nz={Nz}", flush=True)
     dNx, dNz = 3 * Nx // 2, 3 * Nz // 2
     rng = np.random.default_rng(Ny + Nx)
     nf = 3
