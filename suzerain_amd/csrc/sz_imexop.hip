@@ -1242,9 +1242,24 @@ int sz_internal_imexop_apply_grouped(const void *dmem, int nm, int nsys,
     return launch_ok() ? 0 : SZ_ERR_LAUNCH;
 }
 
+// LDS of a sweep block for systems of up to n coefficients per field.
+static size_t sweep_lds(int n) { return sizeof(zd) * (MAXTERMS + 25 * (size_t)n); }
+
+// The largest n whose sweep blocks fit two to a CU (160 KB of LDS, 1 KB
+// allocation granules, the kernel's static LDS included): systems up to it
+// are swept in a launch of their own, so that the larger ones do not size
+// the LDS of every block.
+int sz_internal_imexop_sweep_pair_n(void)
+{
+    const size_t stat = 2048;           // static LDS of the sweep, rounded up
+    int n = 1;
+    while (2 * (((sweep_lds(n + 1) + stat) + 1023) / 1024 * 1024) <= 160 * 1024) ++n;
+    return n;
+}
+
 // One refinement sweep of the mixed precision solve over the listed systems
-// (a->list, *a->count <= nsys; nmax the largest n): a grid of at most 1024
-// blocks strides over the list.
+// (a->list, *a->count <= nsys; nmax the largest n among them): a grid of at
+// most 1024 blocks strides over the list.
 int sz_internal_imexop_sweep_listed(const void *dmem, int nm, int nsys,
                                     int nmax, const sz_prec_sweep *a,
                                     void *stream)
@@ -1252,7 +1267,7 @@ int sz_internal_imexop_sweep_listed(const void *dmem, int nm, int nsys,
     if (nsys <= 0) return 0;
     // the sweep keeps the gather: staging the profiles would take the LDS
     // of a third resident block per CU, which costs more than it saves
-    const size_t lds = sizeof(zd) * (MAXTERMS + 25 * (size_t)nmax);
+    const size_t lds = sweep_lds(nmax);
     if (lds > 150 * 1024) return SZ_ERR_LDS;
     const int grid = nsys < 1024 ? nsys : 1024;
     hipLaunchKernelGGL(k_imexop_sweep_listed, dim3(grid), dim3(AC_NT), lds,

@@ -68,6 +68,7 @@ extern "C" int sz_internal_imexop_pack_listed(const void *dmem, int nm,
                                               int nsys, int ldmax,
                                               const int *list, const int *count,
                                               double *frob, void *stream);
+extern "C" int sz_internal_imexop_sweep_pair_n(void);
 extern "C" int sz_internal_imexop_sweep_listed(const void *dmem, int nm,
                                                int nsys, int nmax,
                                                const sz_prec_sweep *a,
@@ -133,7 +134,10 @@ struct p_args {
     const double *frob;
     const int *finfo;
     int *plist, *pcount;         // systems to factor this pass, their count
-    int *slist, *scount;         // systems the first sweep takes, their count
+    // systems the first sweep takes, by LDS tier (n <= nsplit, larger), and
+    // their counts (scount[2 * tier])
+    int *slist0, *slist1, *scount;
+    int nsplit;
 };
 
 // B <- the state's pencils with the wall rows of the energy and momentum
@@ -246,7 +250,10 @@ __global__ void k_p_afterfactor(p_args A, int nsys)
     }
     const double eps = DBL_EPSILON * 0.5;
     S.tolconst = use_eps ? 1.0 : M.afrob[ls] * eps * sqrt(5.0 * d.n) * A.tolsc;
-    if (A.skip[s] == 0) A.slist[atomicAdd(A.scount, 1)] = s;   // any order
+    if (A.skip[s] == 0) {                                       // any order
+        const int t = d.n > A.nsplit;
+        (t ? A.slist1 : A.slist0)[atomicAdd(A.scount + 2 * t, 1)] = s;
+    }
 }
 
 // counts[0] <- systems marked P_RESTART, counts[1] <- P_FAIL, counts[2] <-
@@ -446,8 +453,11 @@ struct sz_prec {
     int *counts = nullptr, *hcounts = nullptr;
     int *list = nullptr;
     int *plist = nullptr, *pcount = nullptr;   // systems to pack this pass
-    int *slist[2] = {nullptr, nullptr};        // systems to sweep, alternating
-    int *scount = nullptr;                     // their counts (2)
+    // systems to sweep: [LDS tier][alternating buffer], counts at
+    // scount[2 * tier + buffer]; tier 1 holds the systems with n > nsplit
+    int *slist[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    int *scount = nullptr;
+    int nsplit = 0, tier_ns[2] = {0, 0}, tier_nmax[2] = {0, 0};
     double *skm = nullptr, *skn = nullptr;
     int *o_siter = nullptr, *o_diter = nullptr, *o_info = nullptr,
         *o_first = nullptr, *o_apprx = nullptr;
@@ -478,7 +488,7 @@ struct sz_prec {
         void *ps[] = {D, B, X, PB, S, skip, fskip, finfo, frob, list, skm, skn,
                       o_siter, o_diter, o_info, o_first, o_apprx, o_fact, o_tolsc,
                       o_res, lu64, pb64, ipiv64, info64, counts, plist, pcount,
-                      slist[0], slist[1], scount};
+                      slist[0][0], slist[0][1], slist[1][0], slist[1][1], scount};
         for (void *q : ps) if (q) (void)hipFree(q);
         for (auto &sl : slots) {
             if (sl.gmem) (void)hipFree(sl.gmem);
@@ -522,6 +532,8 @@ void rebuild(sz_prec *h)
     h->v0.clear();
     h->frob0.clear();
     h->nsys = h->nmax = h->ldmax = 0;
+    h->nsplit = sz_internal_imexop_sweep_pair_n();
+    h->tier_ns[0] = h->tier_ns[1] = h->tier_nmax[0] = h->tier_nmax[1] = 0;
     h->nvec = h->nfrob = 0;
     std::vector<pdesc> D;
     for (int i = 0; i < nl; ++i) {
@@ -544,6 +556,9 @@ void rebuild(sz_prec *h)
         h->nvec += (long long)m.g.nsys * N;
         h->nfrob += (long long)m.g.nsys * nch;
         h->nmax = std::max(h->nmax, m.g.n);
+        const int t = m.g.n > h->nsplit;
+        h->tier_ns[t] += m.g.nsys;
+        h->tier_nmax[t] = std::max(h->tier_nmax[t], m.g.n);
         // B-spline band leading dimension: KL = 5 (kl + 1) - 1
         h->ldmax = std::max(h->ldmax, (m.g.KL + 1) / 5 + (m.g.KU + 1) / 5 - 1);
     }
@@ -557,8 +572,8 @@ void rebuild(sz_prec *h)
         h->realloc_pool(h->finfo, (size_t)c);
         h->realloc_pool(h->list, (size_t)c);
         h->realloc_pool(h->plist, (size_t)c);
-        h->realloc_pool(h->slist[0], (size_t)c);
-        h->realloc_pool(h->slist[1], (size_t)c);
+        for (auto &tl : h->slist)
+            for (auto &l : tl) h->realloc_pool(l, (size_t)c);
         h->realloc_pool(h->skm, (size_t)c);
         h->realloc_pool(h->skn, (size_t)c);
         h->realloc_pool(h->o_siter, (size_t)c);
@@ -596,7 +611,7 @@ void rebuild(sz_prec *h)
     }
     if (!h->counts) h->realloc_pool(h->counts, 3);
     if (!h->pcount) h->realloc_pool(h->pcount, 1);
-    if (!h->scount) h->realloc_pool(h->scount, 2);
+    if (!h->scount) h->realloc_pool(h->scount, 4);
     h->pool_bytes = (long long)h->cap_sys * (sizeof(pdesc) + sizeof(psys) + 64) +
                     h->cap_vec * (2 * sizeof(zd) + sizeof(zf)) +
                     h->cap_frob * (long long)sizeof(double);
@@ -900,7 +915,7 @@ int single_pass(sz_prec *h, sz_prec::slot &sl, int slot, const p_args &A0, int p
                                        c.gs[(size_t)slot], c.ns, h->p.lu_bs, st);
         if (rc) return rc;
     }
-    hipLaunchKernelGGL(k_p_fill_int, dim3(1), dim3(PT), 0, st, h->scount, 0, 1);
+    hipLaunchKernelGGL(k_p_fill_int, dim3(1), dim3(PT), 0, st, h->scount, 0, 4);
     hipLaunchKernelGGL(k_p_afterfactor, dim3(nb(h->nsys)), dim3(PT), 0, st, A,
                        h->nsys);
     if (!ok()) return SZ_ERR_LAUNCH;
@@ -922,16 +937,20 @@ int single_pass(sz_prec *h, sz_prec::slot &sl, int slot, const p_args &A0, int p
                                            c.gs[(size_t)slot], c.ns, c.nmax, st);
             if (rc) return rc;
         }
-        // the active systems alternate between two lists
+        // the active systems alternate between two lists per LDS tier; the
+        // larger systems first
         const int cur = it & 1, nxt = cur ^ 1;
-        sw.list = h->slist[cur];
-        sw.count = h->scount + cur;
-        sw.next = h->slist[nxt];
-        sw.ncount = h->scount + nxt;
-        hipLaunchKernelGGL(k_p_fill_int, dim3(1), dim3(PT), 0, st, h->scount + nxt,
-                           0, 1);
-        rc = sz_internal_imexop_sweep_listed(sl.gmem, nm, h->nsys, h->nmax, &sw, st);
-        if (rc) return rc;
+        for (int t = 1; t >= 0; --t) {
+            if (h->tier_ns[t] == 0) continue;
+            sw.list = h->slist[t][cur];
+            sw.count = h->scount + 2 * t + cur;
+            sw.next = h->slist[t][nxt];
+            sw.ncount = h->scount + 2 * t + nxt;
+            hipLaunchKernelGGL(k_p_fill_int, dim3(1), dim3(PT), 0, st, sw.ncount, 0, 1);
+            rc = sz_internal_imexop_sweep_listed(sl.gmem, nm, h->tier_ns[t],
+                                                h->tier_nmax[t], &sw, st);
+            if (rc) return rc;
+        }
     }
     return 0;
 }
@@ -1064,8 +1083,10 @@ int sz_prec_solve(sz_prec *h, int slot, const sz_prec_problem *pr)
     A.finfo = h->finfo;
     A.plist = h->plist;
     A.pcount = h->pcount;
-    A.slist = h->slist[0];
+    A.slist0 = h->slist[0][0];
+    A.slist1 = h->slist[1][0];
     A.scount = h->scount;
+    A.nsplit = h->nsplit;
     // 1. right hand sides with the wall rows zeroed, x = 0
     hipLaunchKernelGGL(k_p_gather, dim3(h->nsys), dim3(PT), 0, st, A);
     if (!ok()) return SZ_ERR_LAUNCH;

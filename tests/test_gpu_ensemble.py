@@ -60,6 +60,35 @@ def test_members_equal_alone(gpu, graph):
         np.testing.assert_array_equal(got, want, err_msg=str(m))
 
 
+def test_members_on_both_sweep_tiers(gpu):
+    """A member with more coefficients than the two-blocks-per-CU LDS bound
+    of the refinement sweep is swept in a launch of its own, next to the
+    smaller ones: both still equal their solo runs bit for bit, replayed."""
+    from suzerain_amd._lib import load
+    from suzerain_amd.channel import DEFAULT_SCENARIO
+    from suzerain_amd.ensemble import Ensemble, default_dt
+    nsplit = load().sz_internal_imexop_sweep_pair_n()
+    big = dict(Nx=4, Ny=nsplit + 6, Nz=4, k=8, htdelta=3.0,
+               scenario=dict(DEFAULT_SCENARIO, Re=2000.0, Ma=0.8))
+    members = [big, _members()[0], dict(big, k=6, Nx=8)]
+    print(f"case sweep tiers: split n={nsplit}, member n {[c['Ny'] for c in members]}",
+          flush=True)
+    W = [_initial(c, gpu) for c in members]
+    dts = [default_dt(c) for c in members]
+    ens = Ensemble(members, dt=dts, solver=SPEC, graph=True)
+    for m in range(len(members)):
+        ens.set_state(m, W[m])
+    steps = 2
+    ens.step(steps)
+    ens.check()
+    for m, c in enumerate(members):
+        want = _alone(c, W[m], dts[m], steps, gpu)
+        got = ens.get_state(m).cpu().numpy()
+        print(f"member {m} (n={c['Ny']}, k={c['k']}): max |diff| "
+              f"{np.abs(got - want).max():.3e}", flush=True)
+        np.testing.assert_array_equal(got, want, err_msg=str(m))
+
+
 def test_autotune_leaves_states(gpu):
     from suzerain_amd.ensemble import Ensemble
     members = _members()
