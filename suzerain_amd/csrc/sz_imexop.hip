@@ -25,6 +25,7 @@
 #include "sz_precision.hpp"
 #include "../../include/suzerain_amd.h"
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -563,92 +564,15 @@ constexpr int AC_NT = 1024;
 // keeps the gather: the extra LDS would cost it a resident workgroup).
 #define REFL_BYTES(n) (sizeof(double) * NREF * (size_t)(n))
 
-// y_i <- (M + phi L) x + beta y_i for one system per workgroup.  Field f of
-// system s starts at in[f] + inoff(s) (complex elements, stride 1 in y).
-// The body of one accumulate workgroup: field j of x at in[j] (null: absent),
-// field i of y at out[i], wavenumbers kms, kns.
+// y_i <- (M + phi L) x + beta y_i for the systems s = blockIdx.x,
+// blockIdx.x + gridDim.x, ... < nsys (those with skip[s] != 0 left alone).
+// Field f of system s starts at in_f + inoff(s) (complex elements, stride 1
+// in y; a null in_f is an absent field).  A block stages the operator's
+// term tables (and, STAGED, the reference profiles at every point) once and
+// keeps them for all of its systems; per system it computes the term
+// weights of its wavenumbers, the 15 band products D x and the outputs.
 template <bool STAGED>
-__device__ __forceinline__ void
-acc_body(const table *__restrict__ tab, const bops &w, const refs &rf, zd phi,
-         double kms, double kns, const zd *const (&in)[5], zd *const (&out)[5],
-         zd beta, int beta_zero)
-{
-    extern __shared__ double smem[];
-    const int n = w.n;
-    zd *alpha = reinterpret_cast<zd *>(smem);       // MAXTERMS
-    zd *xs = alpha + MAXTERMS;                      // 5 x n
-    zd *Dx = xs + 5 * n;                            // 3 x 5 x n
-    double *refl = reinterpret_cast<double *>(Dx + 15 * n);   // NREF x n if STAGED
-    const int tid = threadIdx.x;
-    __shared__ int sbg0[25], sbng[25], sgop[MAXGROUPS], sgt0[MAXGROUPS], sgnt[MAXGROUPS];
-    __shared__ int stref[MAXTERMS], srld[NREF];
-    __shared__ const double *srp[NREF];
-    const long long io = 0, oo = 0;
-
-    for (int t = tid; t < tab->nterms; t += AC_NT) {
-        alpha[t] = sz::cmul(phi, kappa(tab->tk[t], kms, kns));
-        stref[t] = tab->tref[t];
-    }
-    for (int g = tid; g < tab->ngroups; g += AC_NT) {
-        sgop[g] = tab->gop[g];
-        sgt0[g] = tab->gt0[g];
-        sgnt[g] = tab->gnt[g];
-    }
-    if (tid < 25) {
-        sbg0[tid] = tab->bg0[tid];
-        sbng[tid] = tab->bng[tid];
-    }
-    if (tid == 0) {
-#pragma unroll
-        for (int q = 0; q < NREF; ++q) {        // static indices: no private copy
-            srp[q] = rf.p[q];
-            srld[q] = rf.ld[q];
-        }
-    }
-    for (int e = tid; e < 5 * n; e += AC_NT) {
-        const int f = e / n, y = e - f * n;
-        xs[e] = in[f] ? sz::gload(in[f] + io + y) : zd{0, 0};
-    }
-    __syncthreads();
-    if constexpr (STAGED) stage_refs<AC_NT>(srp, srld, n, refl);
-    for (int e = tid; e < 15 * n; e += AC_NT) {
-        const int op = e / (5 * n), rem = e - op * 5 * n;
-        const int f = rem / n, r = rem - f * n;
-        const zd *x = xs + f * n;
-        const double *row = w.dt + op * w.dstride + (long long)r * w.ld + w.max_ku - r;
-        double tx = 0, ty = 0;
-        const int c1 = min(n - 1, r + w.kl[op]);
-        for (int c = max(0, r - w.ku[op]); c <= c1; ++c) {
-            const double a = *sz::gmem(row + c);
-            tx = fma(a, x[c].x, tx);
-            ty = fma(a, x[c].y, ty);
-        }
-        Dx[e] = zd{tx, ty};
-    }
-    __syncthreads();
-    for (int e = tid; e < 5 * n; e += AC_NT) {
-        const int i = e / n, r = e - i * n;
-        if (!out[i]) continue;
-        zd acc{0, 0};
-        for (int j = 0; j < 5; ++j) {
-            if (!in[j]) continue;
-            const int b = i * 5 + j, g0 = sbg0[b], ng = sbng[b];
-            for (int g = g0; g < g0 + ng; ++g) {
-                const zd W = STAGED ? weight_lds(sgt0, sgnt, stref, refl, n, alpha, g, r)
-                                    : weight_staged(sgt0, sgnt, stref, srp, srld, alpha, g, r);
-                acc = sz::cfma(acc, W, Dx[(sgop[g] * 5 + j) * n + r]);
-            }
-        }
-        zd *yp = out[i] + oo + r;
-        zd by = beta_zero ? zd{0, 0} : sz::cmul(beta, sz::gload(yp));
-        const zd sm{by.x + acc.x, by.y + acc.y};
-        const zd m = Dx[(M * 5 + i) * n + r];
-        sz::gstore(yp, zd{sm.x + m.x, sm.y + m.y});
-    }
-}
-
-template <bool STAGED>
-__global__ __launch_bounds__(AC_NT)
+__global__ __launch_bounds__(AC_NT) __attribute__((amdgpu_waves_per_eu(8)))
 void k_imexop_accumulate(const table *__restrict__ tab, bops w, refs rf, zd phi,
                          const double *__restrict__ km,
                          const double *__restrict__ kn, const zd *in0,
@@ -657,19 +581,95 @@ void k_imexop_accumulate(const table *__restrict__ tab, bops w, refs rf, zd phi,
                          long long instride, zd beta, int beta_zero, zd *out0,
                          zd *out1, zd *out2, zd *out3, zd *out4,
                          const long long *__restrict__ outoff,
-                         long long outstride, const int *__restrict__ skip)
+                         long long outstride, const int *__restrict__ skip,
+                         int nsys)
 {
-    const int s = blockIdx.x;
-    if (skip && skip[s]) return;                // uniform over the block
-    const long long io = inoff ? inoff[s] : (long long)s * instride;
-    const long long oo = outoff ? outoff[s] : (long long)s * outstride;
-    const zd *const in[5] = {in0 ? in0 + io : nullptr, in1 ? in1 + io : nullptr,
-                             in2 ? in2 + io : nullptr, in3 ? in3 + io : nullptr,
-                             in4 ? in4 + io : nullptr};
-    zd *const out[5] = {out0 ? out0 + oo : nullptr, out1 ? out1 + oo : nullptr,
-                        out2 ? out2 + oo : nullptr, out3 ? out3 + oo : nullptr,
-                        out4 ? out4 + oo : nullptr};
-    acc_body<STAGED>(tab, w, rf, phi, km[s], kn[s], in, out, beta, beta_zero);
+    extern __shared__ double smem[];
+    const int n = w.n;
+    zd *alpha = reinterpret_cast<zd *>(smem);       // MAXTERMS
+    zd *xs = alpha + MAXTERMS;                      // 5 x n
+    zd *Dx = xs + 5 * n;                            // 3 x 5 x n
+    double *refl = reinterpret_cast<double *>(Dx + 15 * n);   // NREF x n if STAGED
+    __shared__ int sbg0[25], sbng[25], sgop[MAXGROUPS], sgt0[MAXGROUPS], sgnt[MAXGROUPS];
+    __shared__ int stref[MAXTERMS], srld[NREF];
+    __shared__ const double *srp[NREF];
+    {
+        const int tid = threadIdx.x;
+        for (int t = tid; t < tab->nterms; t += AC_NT) stref[t] = tab->tref[t];
+        for (int g = tid; g < tab->ngroups; g += AC_NT) {
+            sgop[g] = tab->gop[g];
+            sgt0[g] = tab->gt0[g];
+            sgnt[g] = tab->gnt[g];
+        }
+        if (tid < 25) {
+            sbg0[tid] = tab->bg0[tid];
+            sbng[tid] = tab->bng[tid];
+        }
+        if (tid == 0) {
+#pragma unroll
+            for (int q = 0; q < NREF; ++q) {    // static indices: no private copy
+                srp[q] = rf.p[q];
+                srld[q] = rf.ld[q];
+            }
+        }
+    }
+    __syncthreads();
+    if constexpr (STAGED) stage_refs<AC_NT>(srp, srld, n, refl);
+    const zd *const inb[5] = {in0, in1, in2, in3, in4};
+    zd *const outb[5] = {out0, out1, out2, out3, out4};
+    for (int s = blockIdx.x; s < nsys; s += gridDim.x) {
+        __syncthreads();        // the previous system's LDS is read out
+        if (skip && skip[s]) continue;           // uniform over the block
+        // the thread index as a value the compiler cannot hold across
+        // systems: what it derives is recomputed per system instead of
+        // occupying registers through the whole loop
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const long long io = inoff ? inoff[s] : (long long)s * instride;
+        const long long oo = outoff ? outoff[s] : (long long)s * outstride;
+        const double kms = km[s], kns = kn[s];
+        for (int t = tid; t < tab->nterms; t += AC_NT)
+            alpha[t] = sz::cmul(phi, kappa(tab->tk[t], kms, kns));
+        for (int e = tid; e < 5 * n; e += AC_NT) {
+            const int f = e / n, y = e - f * n;
+            xs[e] = inb[f] ? sz::gload(inb[f] + io + y) : zd{0, 0};
+        }
+        __syncthreads();
+        for (int e = tid; e < 15 * n; e += AC_NT) {
+            const int op = e / (5 * n), rem = e - op * 5 * n;
+            const int f = rem / n, r = rem - f * n;
+            const zd *x = xs + f * n;
+            const double *row = w.dt + op * w.dstride + (long long)r * w.ld + w.max_ku - r;
+            double tx = 0, ty = 0;
+            const int c1 = min(n - 1, r + w.kl[op]);
+            for (int c = max(0, r - w.ku[op]); c <= c1; ++c) {
+                const double a = *sz::gmem(row + c);
+                tx = fma(a, x[c].x, tx);
+                ty = fma(a, x[c].y, ty);
+            }
+            Dx[e] = zd{tx, ty};
+        }
+        __syncthreads();
+        for (int e = tid; e < 5 * n; e += AC_NT) {
+            const int i = e / n, r = e - i * n;
+            if (!outb[i]) continue;
+            zd acc{0, 0};
+            for (int j = 0; j < 5; ++j) {
+                if (!inb[j]) continue;
+                const int b = i * 5 + j, g0 = sbg0[b], ng = sbng[b];
+                for (int g = g0; g < g0 + ng; ++g) {
+                    const zd W = STAGED ? weight_lds(sgt0, sgnt, stref, refl, n, alpha, g, r)
+                                        : weight_staged(sgt0, sgnt, stref, srp, srld, alpha, g, r);
+                    acc = sz::cfma(acc, W, Dx[(sgop[g] * 5 + j) * n + r]);
+                }
+            }
+            zd *yp = outb[i] + oo + r;
+            zd by = beta_zero ? zd{0, 0} : sz::cmul(beta, sz::gload(yp));
+            const zd sm{by.x + acc.x, by.y + acc.y};
+            const zd m = Dx[(M * 5 + i) * n + r];
+            sz::gstore(yp, zd{sm.x + m.x, sm.y + m.y});
+        }
+    }
 }
 
 // Block sum (AC_NT threads, tid = the thread index); every thread gets it.
@@ -812,11 +812,16 @@ __device__ __forceinline__ int sweep_body(const gmember &M, int ls, int s,
 // One refinement sweep over the systems a.list[0 .. *a.count) (any order):
 // each block takes every gridDim.x-th entry, and the systems that continue
 // are appended to a.next (count *a.ncount, cleared by the caller), so that
-// sweeps after convergence cost a handful of blocks.
+// sweeps after convergence cost a handful of blocks.  A block collects its
+// continuing systems in LDS and appends them with one atomic at the end.
 __global__ __launch_bounds__(AC_NT) __attribute__((amdgpu_waves_per_eu(8)))
 void k_imexop_sweep_listed(const gmember *__restrict__ mem, int nm,
                            sz_prec_sweep a)
 {
+    constexpr int NPEND = 256;
+    __shared__ int pend[NPEND];               // written by thread 0 only
+    __shared__ int npend, pbase;
+    if (threadIdx.x == 0) npend = 0;
     // uniform values made explicit, so the member's descriptor stays scalar
     const int cnt = __builtin_amdgcn_readfirstlane(*a.count);
     for (int j = blockIdx.x; j < cnt; j += gridDim.x) {
@@ -829,31 +834,19 @@ void k_imexop_sweep_listed(const gmember *__restrict__ mem, int nm,
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         const int st = sweep_body(M, s - M.sys0, s, a, tid);
-        if (st == SZ_P_ACTIVE && tid == 0)
-            a.next[atomicAdd(a.ncount, 1)] = s;
+        if (st == SZ_P_ACTIVE && tid == 0) {
+            if (npend == NPEND) {             // full: flush (more than NPEND
+                const int b = atomicAdd(a.ncount, NPEND);   // systems a block)
+                for (int i = 0; i < NPEND; ++i) a.next[b + i] = pend[i];
+                npend = 0;
+            }
+            pend[npend++] = s;
+        }
     }
-}
-
-// Grouped y = (M + phi L) x of every member's systems (one block each) on
-// the compact (5, n) vectors at x + v0 + ls*5n, y likewise; systems with
-// skip[s] != 0 left alone.
-template <bool STAGED>
-__global__ __launch_bounds__(AC_NT)
-void k_imexop_apply_grouped(const gmember *__restrict__ mem, int nm,
-                            const zd *__restrict__ x, zd *__restrict__ y,
-                            const int *__restrict__ skip)
-{
-    const gmember &M = mem[member_of(mem, nm, blockIdx.x)];
-    const int ls = blockIdx.x - M.blk0;
-    const int s = M.sys0 + ls;
-    if (skip && skip[s]) return;                 // uniform over the block
-    const int n = M.w.n;
-    const long long o = M.v0 + (long long)ls * 5 * n;
-    const zd *const in[5] = {x + o, x + o + n, x + o + 2 * n, x + o + 3 * n,
-                             x + o + 4 * n};
-    zd *const out[5] = {y + o, y + o + n, y + o + 2 * n, y + o + 3 * n,
-                        y + o + 4 * n};
-    acc_body<STAGED>(M.tab, M.w, M.rf, M.phi, M.km[ls], M.kn[ls], in, out, zd{0, 0}, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) pbase = npend ? atomicAdd(a.ncount, npend) : 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < npend; i += AC_NT) a.next[pbase + i] = pend[i];
 }
 
 // Zero the permuted right hand side entries of the replaced wall equations.
@@ -880,6 +873,7 @@ struct sz_imexop {
     table host;
     table *dev;
     int dev_id;
+    int ncu;              // compute units of that device
 };
 
 extern "C" {
@@ -899,6 +893,9 @@ sz_imexop *sz_imexop_create(const sz_imexop_scenario *s, sz_bsplineop *w,
     op->w = w;
     build_table(op->scen, op->host);
     hipGetDevice(&op->dev_id);
+    if (hipDeviceGetAttribute(&op->ncu, hipDeviceAttributeMultiprocessorCount,
+                              op->dev_id) != hipSuccess || op->ncu < 1)
+        op->ncu = 256;
     if (hipMalloc(&op->dev, sizeof(table)) != hipSuccess ||
         hipMemcpy(op->dev, &op->host, sizeof(table), hipMemcpyHostToDevice)
             != hipSuccess) {
@@ -1098,17 +1095,28 @@ int sz_internal_imexop_accumulate_masked(
     const bops w = make_bops(op->w);
     size_t lds = sizeof(zd) * (MAXTERMS + 20 * (size_t)w.n);
     if (lds > 156 * 1024) return SZ_ERR_LDS;   // 160 KB less the staged tables
-    const bool staged = lds + REFL_BYTES(w.n) <= 156 * 1024;
+    // blocks resident per CU (two at most: 1024 threads each), with the
+    // kernel's static LDS and 1 KB allocation granules; the profiles are
+    // staged when that keeps as many blocks resident
+    auto per_cu = [](size_t b) {
+        const size_t a = (b + 2048 + 1023) / 1024 * 1024;
+        return (int)std::min<size_t>(2, 160 * 1024 / a);
+    };
+    const bool staged = lds + REFL_BYTES(w.n) <= 156 * 1024 &&
+                        per_cu(lds + REFL_BYTES(w.n)) >= per_cu(lds);
     if (staged) lds += REFL_BYTES(w.n);
+    // persistent blocks: as many as stay resident, each taking every
+    // grid-th system
+    const int grid = (int)std::min<long long>(nsys, (long long)op->ncu * per_cu(lds));
     const int bz = beta[0] == 0 && beta[1] == 0;
     hipLaunchKernelGGL((staged ? k_imexop_accumulate<true> : k_imexop_accumulate<false>),
-                       dim3(nsys), dim3(AC_NT), lds,
+                       dim3(grid), dim3(AC_NT), lds,
                        (hipStream_t)stream, op->dev, w, make_refs(r, ld),
                        zd{phi[0], phi[1]}, km, kn, (const zd *)in[0],
                        (const zd *)in[1], (const zd *)in[2], (const zd *)in[3],
                        (const zd *)in[4], inoff, instride, zd{beta[0], beta[1]},
                        bz, (zd *)out[0], (zd *)out[1], (zd *)out[2],
-                       (zd *)out[3], (zd *)out[4], outoff, outstride, skip);
+                       (zd *)out[3], (zd *)out[4], outoff, outstride, skip, nsys);
     return launch_ok() ? 0 : SZ_ERR_LAUNCH;
 }
 
@@ -1224,24 +1232,6 @@ int sz_internal_imexop_pack_listed(const void *dmem, int nm, int nsys, int ldmax
     return launch_ok() ? 0 : SZ_ERR_LAUNCH;
 }
 
-// y = (M + phi L) x for every member's systems (nsys blocks in all; nmax the
-// largest n) on compact vectors.
-int sz_internal_imexop_apply_grouped(const void *dmem, int nm, int nsys,
-                                     int nmax, const void *x, void *y,
-                                     const int *skip, void *stream)
-{
-    if (nsys <= 0) return 0;
-    size_t lds = sizeof(zd) * (MAXTERMS + 20 * (size_t)nmax);
-    if (lds > 156 * 1024) return SZ_ERR_LDS;
-    const bool staged = lds + REFL_BYTES(nmax) <= 156 * 1024;
-    if (staged) lds += REFL_BYTES(nmax);
-    hipLaunchKernelGGL((staged ? k_imexop_apply_grouped<true> : k_imexop_apply_grouped<false>),
-                       dim3(nsys), dim3(AC_NT), lds,
-                       (hipStream_t)stream, (const gmember *)dmem, nm,
-                       (const zd *)x, (zd *)y, skip);
-    return launch_ok() ? 0 : SZ_ERR_LAUNCH;
-}
-
 // LDS of a sweep block for systems of up to n coefficients per field.
 static size_t sweep_lds(int n) { return sizeof(zd) * (MAXTERMS + 25 * (size_t)n); }
 
@@ -1251,7 +1241,7 @@ static size_t sweep_lds(int n) { return sizeof(zd) * (MAXTERMS + 25 * (size_t)n)
 // the LDS of every block.
 int sz_internal_imexop_sweep_pair_n(void)
 {
-    const size_t stat = 2048;           // static LDS of the sweep, rounded up
+    const size_t stat = 3072;           // static LDS of the sweep, rounded up
     int n = 1;
     while (2 * (((sweep_lds(n + 1) + stat) + 1023) / 1024 * 1024) <= 160 * 1024) ++n;
     return n;

@@ -226,34 +226,54 @@ __global__ __launch_bounds__(PT) void k_p_init(p_args A)
 // After the masked pack + factorization: record ||A||_F and the factor state
 // of the systems just factored; a singular float factor sends the system to
 // the double precision fallback.  Every system's tolerance constant.
+// Append value to list (its length at *count) for the lanes whose flag is
+// set, in any order: one atomic per wavefront.  Every lane of the wavefront
+// calls it.
+__device__ __forceinline__ void wave_append(bool flag, int value, int *list,
+                                            int *count)
+{
+    const unsigned long long m = __ballot(flag);
+    if (m == 0) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, leader);
+    if (flag) list[base + __popcll(m & ((1ULL << lane) - 1ULL))] = value;
+}
+
 __global__ void k_p_afterfactor(p_args A, int nsys)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nsys) return;
-    psys &S = A.S[s];
-    const pdesc d = A.D[s];
-    const bool use_eps = A.tolsc == 0.0;
-    const pmem &M = A.PM[d.m];
-    const int ls = s - M.sys0;
-    if (A.fskip[s] == 0) {
-        const double *fr = A.frob + M.frob0 + (long long)ls * M.nchunks;
-        double t = 0;
-        for (int c = 0; c < M.nchunks; ++c) t += fr[c];
-        M.afrob[ls] = sqrt(t);
-        if (A.finfo[s] > 0) {
-            M.fact[ls] = 'N';
-            S.st = P_FAIL;
-            A.skip[s] = 1;
-        } else {
-            M.fact[ls] = 'S';
+    bool first0 = false, first1 = false;     // on the first sweep's lists
+    if (s < nsys) {
+        psys &S = A.S[s];
+        const pdesc d = A.D[s];
+        const bool use_eps = A.tolsc == 0.0;
+        const pmem &M = A.PM[d.m];
+        const int ls = s - M.sys0;
+        if (A.fskip[s] == 0) {
+            const double *fr = A.frob + M.frob0 + (long long)ls * M.nchunks;
+            double t = 0;
+            for (int c = 0; c < M.nchunks; ++c) t += fr[c];
+            M.afrob[ls] = sqrt(t);
+            if (A.finfo[s] > 0) {
+                M.fact[ls] = 'N';
+                S.st = P_FAIL;
+                A.skip[s] = 1;
+            } else {
+                M.fact[ls] = 'S';
+            }
+        }
+        const double eps = DBL_EPSILON * 0.5;
+        S.tolconst = use_eps ? 1.0 : M.afrob[ls] * eps * sqrt(5.0 * d.n) * A.tolsc;
+        if (A.skip[s] == 0) {
+            first1 = d.n > A.nsplit;
+            first0 = !first1;
         }
     }
-    const double eps = DBL_EPSILON * 0.5;
-    S.tolconst = use_eps ? 1.0 : M.afrob[ls] * eps * sqrt(5.0 * d.n) * A.tolsc;
-    if (A.skip[s] == 0) {                                       // any order
-        const int t = d.n > A.nsplit;
-        (t ? A.slist1 : A.slist0)[atomicAdd(A.scount + 2 * t, 1)] = s;
-    }
+    wave_append(first0, s, A.slist0, A.scount);
+    wave_append(first1, s, A.slist1, A.scount + 2);
 }
 
 // counts[0] <- systems marked P_RESTART, counts[1] <- P_FAIL, counts[2] <-
