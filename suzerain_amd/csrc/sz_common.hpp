@@ -74,8 +74,12 @@ __device__ inline cpx<R> cconj(cpx<R> a) { return {a.x, -a.y}; }
 // One system of a grouped launch: systems of different orders (different
 // ensemble members, each with storage of its own) share one launch; device
 // addresses of the system's band storage, pivots and right hand side.
+// layout 0: LAPACK band storage (column j at ab + j*ldab, U rows 0..kl+ku,
+// the multipliers below); 1: split, used by the single precision transposed
+// solve: the n columns of U (kl+ku+1 entries each) back to back, then the n
+// columns of multipliers (kl entries each).
 struct gsys {
-    int n, pad;
+    int n, layout;
     unsigned long long ab, ipiv, b;
 };
 

@@ -655,12 +655,20 @@ void k_gbtrs_t16(int n, const cpx<R> *__restrict__ ab, int ldab, long long sab,
     const int *pv;
     T *bs;
     int nl;
+    // column strides of U and of the multipliers, and where the latter start
+    int ldu = ldab, ldl = ldab;
+    long long loff = KV + 1;
     if (gs) {
         const sz::gsys g = gs[s0];
         nl = g.n;
         a = reinterpret_cast<const T *>(g.ab);
         pv = reinterpret_cast<const int *>(g.ipiv);
         bs = reinterpret_cast<T *>(g.b);
+        if (g.layout == 1) {
+            ldu = KV + 1;
+            ldl = KL;
+            loff = (long long)nl * (KV + 1);
+        }
     } else {
         nl = n;
         a = ab + (size_t)s0 * (size_t)sab;
@@ -685,7 +693,7 @@ void k_gbtrs_t16(int n, const cpx<R> *__restrict__ ab, int ldab, long long sab,
         // any case) read the nearest U row instead, so the loads touch the
         // column's U lines only and stay inside the storage for every c.
         auto fetch = [&](int c, int u16, R (&fx)[NS], R (&fy)[NS]) {
-            const T *col = a + (size_t)min(c, nl - 1) * ldab;
+            const T *col = a + (size_t)min(c, nl - 1) * ldu;
 #pragma unroll
             for (int k = 0; k < NS; ++k) {
                 const int r0 = KV - u16 - LPS * k;
@@ -791,7 +799,7 @@ void k_gbtrs_t16(int n, const cpx<R> *__restrict__ ab, int ldab, long long sab,
         // column, and the pivot of row j
         auto lfetch = [&](int t, R (&fx)[NM], R (&fy)[NM], int &p) {
             const int jl = max(nl - 2 - t, 0);
-            const T *col = a + (size_t)jl * ldab + (KV + 1);
+            const T *col = a + loff + (size_t)jl * ldl;
 #pragma unroll
             for (int m = 0; m < NM; ++m) {
                 const T v = sz::gload(col + min(l + LPS * m, KL - 1));
@@ -1436,6 +1444,15 @@ int t16_lanes(int)
 // when the shape or precision has no such kernel.  gs != nullptr: grouped,
 // every system of order 2 or more (the kernel clamps its loads to the
 // system's last column).
+// Band shapes and right hand side lengths the single precision transposed
+// solve takes (its LDS holds spw right hand sides of ldx elements).
+bool t16_covers(int kl, int ku, int ldx, int lps)
+{
+    const size_t lds = (size_t)(64 / lps) * (size_t)ldx * sizeof(cpx<float>);
+    return ldx > 0 && lds <= LDS_MAX &&
+           ((kl == 34 && ku == 34) || (kl == 24 && ku == 24));
+}
+
 template <typename R>
 bool launch_gbtrs_t16(int tc, int n, int kl, int ku, const void *ab, int ldab,
                       long long sab, const int *ipiv, long long sipiv, void *b,
@@ -1449,7 +1466,7 @@ bool launch_gbtrs_t16(int tc, int n, int kl, int ku, const void *ab, int ldab,
         const int lps = t16_lanes(nsys);
         const int spw = 64 / lps;
         const size_t lds = (size_t)spw * (size_t)ldx * sizeof(cpx<R>);
-        if (lds > LDS_MAX) return false;
+        if (!t16_covers(kl, ku, ldx, lps)) return false;
         const dim3 grid((nsys + spw - 1) / spw);
         const cpx<R> *a = reinterpret_cast<const cpx<R> *>(ab);
         cpx<R> *x = reinterpret_cast<cpx<R> *>(b);
@@ -1556,6 +1573,53 @@ int launch_gbtrf_grouped(int kl, int ku, void *ab, int ldab, int *ipiv, int *inf
     return hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH;
 }
 
+// In place, for the listed systems of a grouped launch (global indices
+// list[0 .. *count); those in [s0, s0 + ns) are taken, system s described by
+// gs[s - s0]): factors in LAPACK band storage -> the split layout of gsys
+// (layout 1).  Per system: the multipliers into temp, the U columns
+// compacted in ascending chunks through LDS (a chunk's new place ends no
+// later than its old one, so it never overwrites a column not yet read),
+// then the multipliers after them.
+constexpr int SPLIT_NT = 256, SPLIT_CH = 64;
+template <typename T>
+__global__ __launch_bounds__(SPLIT_NT)
+void k_band_split(int kl, int ku, const sz::gsys *__restrict__ gs, int s0, int ns,
+                  const int *__restrict__ list, const int *__restrict__ count,
+                  T *__restrict__ temp, long long tstride)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    T *stage = reinterpret_cast<T *>(smem);      // SPLIT_CH x (kl + ku + 1)
+    const int U = kl + ku + 1, LD = 2 * kl + ku + 1;
+    T *tb = temp + (size_t)blockIdx.x * (size_t)tstride;
+    const int cnt = *count;
+    for (int j = blockIdx.x; j < cnt; j += gridDim.x) {
+        const int s = list[j];
+        if (s < s0 || s >= s0 + ns) continue;           // uniform
+        const sz::gsys g = gs[s - s0];
+        T *a = reinterpret_cast<T *>(g.ab);
+        const int N = g.n;
+        const long long nL = (long long)N * kl;
+        for (long long e = threadIdx.x; e < nL; e += SPLIT_NT) {
+            const long long c = e / kl;
+            tb[e] = a[c * LD + U + (e - c * kl)];
+        }
+        for (int c0 = 0; c0 < N; c0 += SPLIT_CH) {
+            const int m = min(SPLIT_CH, N - c0) * U;
+            for (int e = threadIdx.x; e < m; e += SPLIT_NT) {
+                const int c = e / U;
+                stage[e] = a[(long long)(c0 + c) * LD + (e - c * U)];
+            }
+            __syncthreads();        // (also orders the multipliers' reads)
+            for (int e = threadIdx.x; e < m; e += SPLIT_NT)
+                a[(long long)c0 * U + e] = stage[e];
+            __syncthreads();
+        }
+        for (long long e = threadIdx.x; e < nL; e += SPLIT_NT)
+            a[(long long)N * U + e] = tb[e];
+        __syncthreads();            // temp and LDS are the next system's
+    }
+}
+
 template <typename R>
 int launch_gbtrs_grouped(char trans, int kl, int ku, const void *ab, int ldab,
                          const int *ipiv, void *b, const int *skip,
@@ -1592,6 +1656,30 @@ int sz_internal_gbtrf_grouped(int single, int kl, int ku, void *ab, int ldab,
                                            nsys, bs, (hipStream_t)stream);
     return launch_gbtrf_grouped<double>(kl, ku, ab, ldab, ipiv, info, skip, g, nsys,
                                         bs, (hipStream_t)stream);
+}
+
+// 1 when the single precision grouped transposed solve of this band shape
+// and largest order reads the split layout (gsys layout 1), else 0.
+int sz_internal_gbtrs_split_ok(int kl, int ku, int nmax)
+{
+    return t16_covers(kl, ku, nmax, t16_lanes(0)) ? 1 : 0;
+}
+
+// Factors of the listed systems (see k_band_split) from LAPACK band storage
+// to the split layout, single precision; temp holds grid * nmax * kl complex
+// floats.
+int sz_internal_band_split_grouped(int kl, int ku, const void *gs, int s0, int ns,
+                                   int nmax, const int *list, const int *count,
+                                   void *temp, int grid, void *stream)
+{
+    if (ns <= 0 || grid <= 0) return 0;
+    const size_t lds = (size_t)SPLIT_CH * (size_t)(kl + ku + 1) * sizeof(cpx<float>);
+    if (lds > LDS_MAX) return SZ_ERR_LDS;
+    hipLaunchKernelGGL(k_band_split<cpx<float>>, dim3(grid), dim3(SPLIT_NT), lds,
+                       (hipStream_t)stream, kl, ku, static_cast<const sz::gsys *>(gs),
+                       s0, ns, list, count, static_cast<cpx<float> *>(temp),
+                       (long long)nmax * kl);
+    return hipGetLastError() == hipSuccess ? 0 : SZ_ERR_LAUNCH;
 }
 
 int sz_internal_gbtrs_grouped(int single, char trans, int kl, int ku,

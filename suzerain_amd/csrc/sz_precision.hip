@@ -37,6 +37,11 @@ extern "C" int sz_internal_gbtrs_masked(int single, char trans, int n, int kl,
                                         void *b, int ldb, long long strideb,
                                         const int *skip, int nbatch,
                                         void *stream);
+extern "C" int sz_internal_gbtrs_split_ok(int kl, int ku, int nmax);
+extern "C" int sz_internal_band_split_grouped(int kl, int ku, const void *gs, int s0,
+                                              int ns, int nmax, const int *list,
+                                              const int *count, void *temp, int grid,
+                                              void *stream);
 extern "C" int sz_internal_gbtrf_grouped(int single, int kl, int ku, void *ab,
                                          int ldab, int *ipiv, int *info,
                                          const int *skip, const void *gs,
@@ -451,7 +456,12 @@ struct sz_prec {
     struct lu_class {
         int KL, KU, ldlu, s0, ns, nmax;
         std::vector<sz::gsys *> gs;       // per slot, device, ns entries
+        int split = 0;                    // factors kept in the split layout
     };
+    // per-block room for the multipliers while factors are split
+    static constexpr int SPLIT_GRID = 256;
+    zf *stemp = nullptr;
+    long long cap_stemp = 0;
     std::vector<lu_class> classes;
     struct slot {
         unsigned char *gmem = nullptr;    // imexop member descriptors (one
@@ -508,7 +518,8 @@ struct sz_prec {
         void *ps[] = {D, B, X, PB, S, skip, fskip, finfo, frob, list, skm, skn,
                       o_siter, o_diter, o_info, o_first, o_apprx, o_fact, o_tolsc,
                       o_res, lu64, pb64, ipiv64, info64, counts, plist, pcount,
-                      slist[0][0], slist[0][1], slist[1][0], slist[1][1], scount};
+                      slist[0][0], slist[0][1], slist[1][0], slist[1][1], scount,
+                      stemp};
         for (void *q : ps) if (q) (void)hipFree(q);
         for (auto &sl : slots) {
             if (sl.gmem) (void)hipFree(sl.gmem);
@@ -561,7 +572,7 @@ void rebuild(sz_prec *h)
         const int N = 5 * m.g.n, ldlu = 2 * m.g.KL + m.g.KU + 1;
         if (h->classes.empty() || h->classes.back().KL != m.g.KL ||
             h->classes.back().KU != m.g.KU)
-            h->classes.push_back({m.g.KL, m.g.KU, ldlu, h->nsys, 0, 0, {}});
+            h->classes.push_back({m.g.KL, m.g.KU, ldlu, h->nsys, 0, 0, {}, 0});
         auto &c = h->classes.back();
         h->sys0.push_back(h->nsys);
         h->v0.push_back(h->nvec);
@@ -638,6 +649,18 @@ void rebuild(sz_prec *h)
     if (!D.empty())
         (void)hipMemcpy(h->D, D.data(), D.size() * sizeof(pdesc),
                         hipMemcpyHostToDevice);
+    // the single precision solve reads split factors where it can: a
+    // class's factors are converted after every factorization
+    long long stemp = 0;
+    for (auto &c : h->classes) {
+        c.split = sz_internal_gbtrs_split_ok(c.KL, c.KU, c.nmax);
+        if (c.split)
+            stemp = std::max(stemp, (long long)sz_prec::SPLIT_GRID * c.nmax * c.KL);
+    }
+    if (stemp > h->cap_stemp) {
+        h->realloc_pool(h->stemp, (size_t)stemp);
+        h->cap_stemp = stemp;
+    }
     // grouped factorization / solve descriptors per slot and band shape
     for (auto &c : h->classes) {
         c.gs.assign(h->slots.size(), nullptr);
@@ -653,6 +676,7 @@ void rebuild(sz_prec *h)
                 for (int l = 0; l < m.g.nsys; ++l) {
                     sz::gsys g{};
                     g.n = N;
+                    g.layout = c.split;
                     g.ab = (unsigned long long)(sd.lu + (size_t)l * c.ldlu * N);
                     g.ipiv = (unsigned long long)(sd.ipiv + (size_t)l * N);
                     g.b = (unsigned long long)(h->PB + h->v0[(size_t)i] +
@@ -934,6 +958,12 @@ int single_pass(sz_prec *h, sz_prec::slot &sl, int slot, const p_args &A0, int p
                                        h->finfo + c.s0, h->fskip + c.s0,
                                        c.gs[(size_t)slot], c.ns, h->p.lu_bs, st);
         if (rc) return rc;
+        if (c.split) {
+            rc = sz_internal_band_split_grouped(
+                c.KL, c.KU, c.gs[(size_t)slot], c.s0, c.ns, c.nmax, h->plist,
+                h->pcount, h->stemp, std::min(c.ns, (int)sz_prec::SPLIT_GRID), st);
+            if (rc) return rc;
+        }
     }
     hipLaunchKernelGGL(k_p_fill_int, dim3(1), dim3(PT), 0, st, h->scount, 0, 4);
     hipLaunchKernelGGL(k_p_afterfactor, dim3(nb(h->nsys)), dim3(PT), 0, st, A,

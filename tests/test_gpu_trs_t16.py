@@ -155,3 +155,73 @@ def test_grouped_mixed_orders_match_oracle(gpu, kl, orders):
         want = _oracle("T", abd.cpu().numpy()[0], kl, kl, ipiv.cpu().numpy()[0], b)
         got = bd.cpu().numpy()
         assert _rel(got, want) <= RTOL, (s, N, _rel(got, want))
+
+
+def _split_host(lu, kl, ku):
+    """The split layout of one system's LAPACK band storage lu (N, ldab): the
+    N columns of U (rows 0..kl+ku) back to back, then the N columns of
+    multipliers (rows kl+ku+1..)."""
+    U = kl + ku + 1
+    return np.concatenate([lu[:, :U].reshape(-1), lu[:, U:].reshape(-1)])
+
+
+@pytest.mark.parametrize("kl,orders", [(34, [480, 70, 2, 17, 400, 35, 96]),
+                                       (24, [33, 96, 250, 2, 48])])
+def test_split_factors_convert_and_solve(gpu, kl, orders):
+    """Factors converted in place to the split layout by the grouped
+    conversion (the listed systems only, bit for bit the host restatement;
+    the others untouched), then the grouped solve reading them (descriptor
+    layout 1) against CGBTRS on the LAPACK storage."""
+    import torch
+    from suzerain_amd._lib import load
+    lib = load()
+    vp, i32 = C.c_void_p, C.c_int
+    conv = lib.sz_internal_band_split_grouped
+    conv.argtypes = [i32, i32, vp, i32, i32, i32, vp, vp, vp, i32, vp]
+    conv.restype = i32
+    solve = lib.sz_internal_gbtrs_grouped
+    solve.argtypes = [i32, C.c_char, i32, i32, vp, i32, vp, vp, vp, vp, i32, i32, vp]
+    solve.restype = i32
+    assert lib.sz_internal_gbtrs_split_ok(kl, kl, max(orders)) == 1
+    print(f"case split kl=ku={kl} orders={orders}", flush=True)
+    rng = np.random.default_rng(7 * kl + len(orders))
+    keep, recs = [], []
+    for N in orders:
+        ab = _bands(rng, 1, N, kl, kl)
+        abd, ipiv = _factor(gpu, ab, kl, kl)
+        b = (rng.standard_normal(N) + 1j * rng.standard_normal(N)).astype(np.complex64)
+        lu = abd.cpu().numpy()[0]
+        keep.append((N, abd, ipiv, b, _t(b, gpu), lu))
+    # two descriptor sets over the same storage: the conversion's (as
+    # factored, layout 0) and the solve's (layout 1); the system at
+    # position 1 is left out of the list and must stay as it was
+    gdt = np.dtype([("n", "<i4"), ("layout", "<i4"), ("ab", "<u8"), ("ipiv", "<u8"),
+                    ("b", "<u8")])
+    recs = [(N, 1, abd.data_ptr(), ipiv.data_ptr(), bd.data_ptr())
+            for N, abd, ipiv, b, bd, lu in keep]
+    gs = _t(np.array(recs, dtype=gdt).view(np.uint8), gpu)
+    s0 = 100                                  # global numbering of the class
+    listed = [s0 + s for s in range(len(orders)) if s != 1]
+    lst = _t(np.array(listed[::-1], dtype=np.int32), gpu)
+    cnt = _t(np.array([len(listed)], dtype=np.int32), gpu)
+    grid = 3
+    temp = torch.empty(grid * max(orders) * kl * 2, dtype=torch.float32, device=gpu)
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert conv(kl, kl, gs.data_ptr(), s0, len(orders), max(orders), lst.data_ptr(),
+                cnt.data_ptr(), temp.data_ptr(), grid, st) == 0
+    torch.cuda.synchronize()
+    for s, (N, abd, ipiv, b, bd, lu) in enumerate(keep):
+        got = abd.cpu().numpy()[0].reshape(-1)
+        want = lu.reshape(-1) if s == 1 else _split_host(lu, kl, kl)
+        same = (got == want) | (np.isnan(got) & np.isnan(want))
+        assert same.all(), (s, N)
+    # the unlisted system back in the split layout by the host, then solved
+    N1, abd1 = keep[1][0], keep[1][1]
+    abd1.copy_(_t(_split_host(keep[1][5], kl, kl).reshape(abd1.shape), gpu))
+    assert solve(1, b"T", kl, kl, None, 3 * kl + 1, None, None, None, gs.data_ptr(),
+                 len(orders), max(orders), st) == 0
+    torch.cuda.synchronize()
+    for s, (N, abd, ipiv, b, bd, lu) in enumerate(keep):
+        want = _oracle("T", lu, kl, kl, ipiv.cpu().numpy()[0], b)
+        got = bd.cpu().numpy()
+        assert _rel(got, want) <= RTOL, (s, N, _rel(got, want))
