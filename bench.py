@@ -57,6 +57,9 @@ def parse():
                    help="serving trace length")
     p.add_argument("--rate", type=float, default=2.0,
                    help="serving arrivals per second")
+    p.add_argument("--sat-rate", type=float, default=16.0,
+                   help="serving: a second run of the same requests at this "
+                        "rate, high enough to keep both servers busy (0: none)")
     p.add_argument("--req-steps", default="10,60",
                    help="serving: steps per request ~ U{lo..hi}")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -432,35 +435,59 @@ def field_errors(got, want):
 
 def t16_roofline(nsys, N, KL, reps=5):
     """The mixed precision path's dominant kernel, the single precision
-    transposed band solve (sixteen lanes per system), on nsys systems of
-    order N with KL = KU: one launch reads the U and L rows of every column
-    (N (2 KL + KL + 1) complex floats) and the pivots, and reads and writes
-    the right hand side.  Timed with HIP events on the stream it runs on
-    (torch's current stream) over `reps` launches; the factors are random
-    with a dominant diagonal and identity pivots (the kernel's work does not
-    depend on the values)."""
+    transposed band solve (sixteen lanes per system), launched as the
+    precision form launches it: grouped, on factors in the split layout
+    (each system's U columns, then its multiplier columns, contiguous), on
+    nsys systems of order N with KL = KU.  One launch reads the U and L rows
+    of every column (N (2 KL + KL + 1) complex floats) and the pivots, and
+    reads and writes the right hand side.  Timed with HIP events on the
+    stream it runs on (torch's current stream) over `reps` launches; the
+    factors are random with a dominant diagonal and identity pivots (the
+    kernel's work does not depend on the values)."""
+    import ctypes as C
     import torch
-    import suzerain_amd as sa
+    from suzerain_amd._lib import load
+    lib = load()
+    f = lib.sz_internal_gbtrs_grouped
+    vp, i32 = C.c_void_p, C.c_int
+    f.argtypes = [i32, C.c_char, i32, i32, vp, i32, vp, vp, vp, vp, i32, i32, vp]
+    f.restype = i32
     dev = torch.device("cuda:0")
     ldab = 3 * KL + 1
-    lu = torch.randn((nsys, N, ldab), dtype=torch.complex64, device=dev)
-    lu[:, :, 2 * KL] += 4.0 * (2 * KL + 1)
+    U = 2 * KL + 1
+    lu = torch.randn((nsys, N * ldab), dtype=torch.complex64, device=dev)
+    lu.view(nsys, N * ldab)[:, :N * U].view(nsys, N, U)[:, :, 2 * KL] += 4.0 * (2 * KL + 1)
     ipiv = torch.arange(1, N + 1, dtype=torch.int32, device=dev).repeat(nsys, 1)
-    b = torch.randn((nsys, 1, N), dtype=torch.complex64, device=dev)
-    sa.gbtrs("T", lu, KL, KL, ipiv, b, precision="c")
+    b = torch.randn((nsys, N), dtype=torch.complex64, device=dev)
+    gdt = np.dtype([("n", "<i4"), ("layout", "<i4"), ("ab", "<u8"), ("ipiv", "<u8"),
+                    ("b", "<u8")])
+    g = np.zeros(nsys, dtype=gdt)
+    g["n"], g["layout"] = N, 1
+    g["ab"] = lu.data_ptr() + np.arange(nsys, dtype=np.uint64) * (N * ldab * 8)
+    g["ipiv"] = ipiv.data_ptr() + np.arange(nsys, dtype=np.uint64) * (N * 4)
+    g["b"] = b.data_ptr() + np.arange(nsys, dtype=np.uint64) * (N * 8)
+    gs = torch.from_numpy(g.view(np.uint8)).to(dev)
     st = torch.cuda.current_stream()
+    sp = C.c_void_p(st.cuda_stream)
+
+    def launch():
+        rc = f(1, b"T", KL, KL, None, ldab, None, None, None, gs.data_ptr(), nsys, N, sp)
+        if rc:
+            raise RuntimeError(f"grouped transposed solve failed ({rc})")
+    launch()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(st)
     for _ in range(reps):
-        sa.gbtrs("T", lu, KL, KL, ipiv, b, precision="c")
+        launch()
     e1.record(st)
     e1.synchronize()
     ms = e0.elapsed_time(e1) / reps
     bytes_sys = N * (3 * KL + 1) * 8 + N * 4 + 2 * N * 8
     achieved = bytes_sys * nsys / (ms * 1e-3) / 1e9
-    del lu, ipiv, b
+    del lu, ipiv, b, gs
     torch.cuda.empty_cache()
     return {"bound": "hbm", "kernel": f"k_gbtrs_t16<float, {KL}, {2 * KL}, true, false>",
+            "layout": "split (U columns, then multiplier columns)",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "avg_launch_ms": round(ms, 4), "bytes_per_launch": bytes_sys * nsys,
@@ -618,25 +645,41 @@ def serving_requests(nreq, rate, steps_range, seed=20261204, large_frac=0.05,
 
 
 def _initial_state(cfg):
+    """A request's initial state, made when the trace is made (a request
+    arrives with its state): the server only uploads it."""
+    import torch
+    from suzerain_amd.channel import Channel, laminar_state
+    ch = Channel(**cfg, solver="none")
+    W = laminar_state(ch, seed=7 + cfg["Ny"])
+    del ch
+    torch.cuda.empty_cache()
+
     def init():
-        import torch
-        from suzerain_amd.channel import Channel, laminar_state
-        ch = Channel(**cfg, solver="none")
-        W = laminar_state(ch, seed=7 + cfg["Ny"])
-        del ch
         return torch.from_numpy(W).to("cuda:0")
     return init
 
 
-def serving_workload(nreq, rate, steps_range, budget=240e9):
+def serving_workload(nreq, rate, steps_range, budget=240e9, sat_rate=0.0):
     """Continuous batching of the trace within the HBM budget (FP32-refined,
     captured steps), then the plain FP64 drop-in serving the same trace
     (each request its own channel, live ones stepped round robin); final
-    states compared request by request."""
+    states compared request by request.  With sat_rate, the same requests
+    again arriving at that rate, which keeps both servers busy: their
+    sustained substeps/s."""
+    out = serving_run(nreq, rate, steps_range, budget)
+    out.update({"requests": nreq, "rate_per_s": rate, "budget_gb": budget / 1e9,
+                "spec": PREC_SPEC})
+    if sat_rate > 0:
+        sat = serving_run(nreq, sat_rate, steps_range, budget)
+        sat["rate_per_s"] = sat_rate
+        out["saturated"] = sat
+    return out
+
+
+def serving_run(nreq, rate, steps_range, budget):
     import torch
     from suzerain_amd.serving import Server, Fp64Server
-    out = {"requests": nreq, "rate_per_s": rate, "steps_range": list(steps_range),
-           "budget_gb": budget / 1e9, "spec": PREC_SPEC}
+    out = {"steps_range": list(steps_range)}
     reqs = serving_requests(nreq, rate, steps_range)
     out["large_requests"] = sum(r.cfg["Nx"] == 192 for r in reqs)
     out["member_steps"] = sum(r.steps for r in reqs)
@@ -846,7 +889,8 @@ def extra_keys(args):
         out["ensemble"] = ensemble_workload(args.members, args.ens_steps)
     if args.requests:
         lo, hi = (int(x) for x in args.req_steps.split(","))
-        out["serving"] = serving_workload(args.requests, args.rate, (lo, hi))
+        out["serving"] = serving_workload(args.requests, args.rate, (lo, hi),
+                                          sat_rate=args.sat_rate)
     if getattr(args, "physics", True):
         out["mms"] = mms_workload()
         out["bulk_constraints"] = constrained_workload()
