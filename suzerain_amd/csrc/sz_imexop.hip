@@ -274,6 +274,32 @@ __device__ __forceinline__ void stage_refs(const double *const *rp, const int *r
     }
 }
 
+// The band products of operator op at wall-normal point r for all five
+// fields: Dx[(op * 5 + f) * n + r] = sum over the band columns c (ascending,
+// a fused multiply-add per term from zero) of D_T(r, c) x_f[c], x_f at
+// xs + f n.  Each coefficient is loaded once for the five fields.
+__device__ __forceinline__ void band_products5(const bops &w, int op, int r,
+                                               const zd *xs, zd *Dx)
+{
+    const int n = w.n;
+    const double *row = w.dt + op * w.dstride + (long long)r * w.ld + w.max_ku - r;
+    double tx[5], ty[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) tx[f] = ty[f] = 0;
+    const int c1 = min(n - 1, r + w.kl[op]);
+    for (int c = max(0, r - w.ku[op]); c <= c1; ++c) {
+        const double a = *sz::gmem(row + c);
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+            const zd xv = xs[f * n + c];
+            tx[f] = fma(a, xv.x, tx[f]);
+            ty[f] = fma(a, xv.y, ty[f]);
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < 5; ++f) Dx[(op * 5 + f) * n + r] = zd{tx[f], ty[f]};
+}
+
 constexpr int PK_NT = 256;
 constexpr int PK_RB = 4;  // wall-normal rows (5 columns each) per workgroup
 
@@ -635,20 +661,8 @@ void k_imexop_accumulate(const table *__restrict__ tab, bops w, refs rf, zd phi,
             xs[e] = inb[f] ? sz::gload(inb[f] + io + y) : zd{0, 0};
         }
         __syncthreads();
-        for (int e = tid; e < 15 * n; e += AC_NT) {
-            const int op = e / (5 * n), rem = e - op * 5 * n;
-            const int f = rem / n, r = rem - f * n;
-            const zd *x = xs + f * n;
-            const double *row = w.dt + op * w.dstride + (long long)r * w.ld + w.max_ku - r;
-            double tx = 0, ty = 0;
-            const int c1 = min(n - 1, r + w.kl[op]);
-            for (int c = max(0, r - w.ku[op]); c <= c1; ++c) {
-                const double a = *sz::gmem(row + c);
-                tx = fma(a, x[c].x, tx);
-                ty = fma(a, x[c].y, ty);
-            }
-            Dx[e] = zd{tx, ty};
-        }
+        for (int e = tid; e < 3 * n; e += AC_NT)
+            band_products5(w, e / n, e % n, xs, Dx);
         __syncthreads();
         for (int e = tid; e < 5 * n; e += AC_NT) {
             const int i = e / n, r = e - i * n;
@@ -747,20 +761,8 @@ __device__ __forceinline__ int sweep_body(const gmember &M, int ls, int s,
         sz::gstore(x + e, v);
     }
     __syncthreads();
-    for (int e = tid; e < 15 * n; e += AC_NT) {
-        const int op = e / (5 * n), rem = e - op * 5 * n;
-        const int f = rem / n, r = rem - f * n;
-        const zd *xf = xs + f * n;
-        const double *row = w.dt + op * w.dstride + (long long)r * w.ld + w.max_ku - r;
-        double tx = 0, ty = 0;
-        const int c1 = min(n - 1, r + w.kl[op]);
-        for (int c = max(0, r - w.ku[op]); c <= c1; ++c) {
-            const double av = *sz::gmem(row + c);
-            tx = fma(av, xf[c].x, tx);
-            ty = fma(av, xf[c].y, ty);
-        }
-        Dx[e] = zd{tx, ty};
-    }
+    for (int e = tid; e < 3 * n; e += AC_NT)
+        band_products5(w, e / n, e % n, xs, Dx);
     __syncthreads();
     double ss = 0, xx = 0;
     for (int e = tid; e < N; e += AC_NT) {
