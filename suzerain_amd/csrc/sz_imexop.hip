@@ -1099,13 +1099,14 @@ int sz_internal_imexop_accumulate_masked(
     if (lds > 156 * 1024) return SZ_ERR_LDS;   // 160 KB less the staged tables
     // blocks resident per CU (two at most: 1024 threads each), with the
     // kernel's static LDS and 1 KB allocation granules; the profiles are
-    // staged when that keeps as many blocks resident
+    // staged whenever they fit: a persistent block stages them once for all
+    // of its systems, which costs less than gathering them per term even
+    // where it leaves one block per CU instead of two
     auto per_cu = [](size_t b) {
         const size_t a = (b + 2048 + 1023) / 1024 * 1024;
         return (int)std::min<size_t>(2, 160 * 1024 / a);
     };
-    const bool staged = lds + REFL_BYTES(w.n) <= 156 * 1024 &&
-                        per_cu(lds + REFL_BYTES(w.n)) >= per_cu(lds);
+    const bool staged = lds + REFL_BYTES(w.n) <= 156 * 1024;
     if (staged) lds += REFL_BYTES(w.n);
     // persistent blocks: as many as stay resident, each taking every
     // grid-th system
