@@ -706,10 +706,13 @@ __device__ __forceinline__ double block_sum_ac(double v, double *red, int tid)
 // evaluated as the constraints s (x_f - fac x_rho) = 0 whose scalings s the
 // factors hold, ||r||_2 and ||x||_2, the decision of dsgbsvx.def:186-227,
 // and for the systems that continue the next right hand side P r in float.
-// The operator product is the accumulate body's arithmetic.  Returns the
-// system's new state (the same in every thread).
+// The operator product is the accumulate body's arithmetic, with the
+// member's reference profiles in LDS when PROF (staged when stage is set:
+// the block's previous system belonged to another member), else gathered.
+// Returns the system's new state (the same in every thread).
+template <bool PROF>
 __device__ __forceinline__ int sweep_body(const gmember &M, int ls, int s,
-                                          const sz_prec_sweep &a, int tid)
+                                          const sz_prec_sweep &a, int tid, bool stage)
 {
     extern __shared__ double smem[];
     const table *__restrict__ tab = M.tab;
@@ -720,6 +723,7 @@ __device__ __forceinline__ int sweep_body(const gmember &M, int ls, int s,
     zd *xs = alpha + MAXTERMS;                      // 5 x n
     zd *Dx = xs + 5 * n;                            // 3 x 5 x n
     zd *rr = Dx + 15 * n;                           // 5 x n
+    double *refl = reinterpret_cast<double *>(rr + 5 * n);   // NREF x n
     __shared__ int sbg0[25], sbng[25], sgop[MAXGROUPS], sgt0[MAXGROUPS], sgnt[MAXGROUPS];
     __shared__ int stref[MAXTERMS], srld[NREF];
     __shared__ const double *srp[NREF];
@@ -761,6 +765,7 @@ __device__ __forceinline__ int sweep_body(const gmember &M, int ls, int s,
         sz::gstore(x + e, v);
     }
     __syncthreads();
+    if (PROF && stage) stage_refs<AC_NT>(srp, srld, n, refl);
     for (int e = tid; e < 3 * n; e += AC_NT)
         band_products5(w, e / n, e % n, xs, Dx);
     __syncthreads();
@@ -781,7 +786,8 @@ __device__ __forceinline__ int sweep_body(const gmember &M, int ls, int s,
             for (int j = 0; j < 5; ++j) {
                 const int bb = i * 5 + j, g0 = sbg0[bb], ng = sbng[bb];
                 for (int g = g0; g < g0 + ng; ++g) {
-                    const zd W = weight_staged(sgt0, sgnt, stref, srp, srld, alpha, g, r);
+                    const zd W = PROF ? weight_lds(sgt0, sgnt, stref, refl, n, alpha, g, r)
+                                      : weight_staged(sgt0, sgnt, stref, srp, srld, alpha, g, r);
                     acc = sz::cfma(acc, W, Dx[(sgop[g] * 5 + j) * n + r]);
                 }
             }
@@ -812,10 +818,13 @@ __device__ __forceinline__ int sweep_body(const gmember &M, int ls, int s,
 }
 
 // One refinement sweep over the systems a.list[0 .. *a.count) (any order):
-// each block takes every gridDim.x-th entry, and the systems that continue
+// each block takes a contiguous share of the list (whose systems mostly
+// belong to one member, so it restages the profiles rarely), and the
+// systems that continue
 // are appended to a.next (count *a.ncount, cleared by the caller), so that
 // sweeps after convergence cost a handful of blocks.  A block collects its
 // continuing systems in LDS and appends them with one atomic at the end.
+template <bool PROF>
 __global__ __launch_bounds__(AC_NT) __attribute__((amdgpu_waves_per_eu(8)))
 void k_imexop_sweep_listed(const gmember *__restrict__ mem, int nm,
                            sz_prec_sweep a)
@@ -826,16 +835,22 @@ void k_imexop_sweep_listed(const gmember *__restrict__ mem, int nm,
     if (threadIdx.x == 0) npend = 0;
     // uniform values made explicit, so the member's descriptor stays scalar
     const int cnt = __builtin_amdgcn_readfirstlane(*a.count);
-    for (int j = blockIdx.x; j < cnt; j += gridDim.x) {
+    const int per = (cnt + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int j0 = (int)blockIdx.x * per, j1 = min(cnt, j0 + per);
+    int staged = -1;                          // member whose profiles are in LDS
+    for (int j = j0; j < j1; ++j) {
         const int s = __builtin_amdgcn_readfirstlane(a.list[j]);
-        const gmember &M = mem[member_of_sys(mem, nm, s)];
+        const int mi = member_of_sys(mem, nm, s);
+        const gmember &M = mem[mi];
+        const bool stage = mi != staged;
+        staged = mi;
         __syncthreads();        // the previous system's LDS is read out
         // the thread index as a value the compiler cannot hold across
         // systems: what it derives is recomputed per system instead of
         // occupying registers through the whole loop
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
-        const int st = sweep_body(M, s - M.sys0, s, a, tid);
+        const int st = sweep_body<PROF>(M, s - M.sys0, s, a, tid, stage);
         if (st == SZ_P_ACTIVE && tid == 0) {
             if (npend == NPEND) {             // full: flush (more than NPEND
                 const int b = atomicAdd(a.ncount, NPEND);   // systems a block)
@@ -1236,17 +1251,39 @@ int sz_internal_imexop_pack_listed(const void *dmem, int nm, int nsys, int ldmax
 }
 
 // LDS of a sweep block for systems of up to n coefficients per field.
-static size_t sweep_lds(int n) { return sizeof(zd) * (MAXTERMS + 25 * (size_t)n); }
+// LDS of a sweep block for systems of up to n points, with (prof) or
+// without the reference profiles.
+static size_t sweep_lds(int n, bool prof = true)
+{
+    return sizeof(zd) * (MAXTERMS + 25 * (size_t)n) + (prof ? REFL_BYTES(n) : 0);
+}
 
-// The largest n whose sweep blocks fit two to a CU (160 KB of LDS, 1 KB
-// allocation granules, the kernel's static LDS included): systems up to it
-// are swept in a launch of their own, so that the larger ones do not size
-// the LDS of every block.
-int sz_internal_imexop_sweep_pair_n(void)
+// Sweep blocks resident per CU (two at most: 1,024 threads each) for systems
+// of up to n points, with or without the profiles (160 KB of LDS, 1 KB
+// granules, the kernel's static LDS included).
+static int sweep_per_cu(int n, bool prof)
 {
     const size_t stat = 3072;           // static LDS of the sweep, rounded up
+    const size_t a = (sweep_lds(n, prof) + stat + 1023) / 1024 * 1024;
+    return (int)std::min<size_t>(2, 160 * 1024 / a);
+}
+
+// Whether the sweep of systems of up to nmax points stages the members'
+// profiles: where two blocks per CU still fit with them, or where even
+// without them only one does.
+static bool sweep_prof(int nmax)
+{
+    return sweep_per_cu(nmax, true) == 2 || sweep_per_cu(nmax, false) < 2;
+}
+
+// The sweep's tier bounds: the largest n whose blocks fit two to a CU with
+// the profiles staged (which = 0) and with them gathered (which = 1).  Each
+// tier runs a launch of its own, so that the larger systems do not size the
+// LDS of every block.
+int sz_internal_imexop_sweep_tier_n(int which)
+{
     int n = 1;
-    while (2 * (((sweep_lds(n + 1) + stat) + 1023) / 1024 * 1024) <= 160 * 1024) ++n;
+    while (sweep_per_cu(n + 1, which == 0) == 2) ++n;
     return n;
 }
 
@@ -1258,12 +1295,15 @@ int sz_internal_imexop_sweep_listed(const void *dmem, int nm, int nsys,
                                     void *stream)
 {
     if (nsys <= 0) return 0;
-    // the sweep keeps the gather: staging the profiles would take the LDS
-    // of a third resident block per CU, which costs more than it saves
-    const size_t lds = sweep_lds(nmax);
-    if (lds > 150 * 1024) return SZ_ERR_LDS;
+    // the member's profiles staged in LDS where that keeps two blocks per
+    // CU, or where one is all that fits anyway (and they fit), else
+    // gathered per term
+    const bool prof = sweep_prof(nmax) && sweep_per_cu(nmax, true) >= 1;
+    const size_t lds = sweep_lds(nmax, prof);
+    if (lds > 156 * 1024) return SZ_ERR_LDS;
     const int grid = nsys < 1024 ? nsys : 1024;
-    hipLaunchKernelGGL(k_imexop_sweep_listed, dim3(grid), dim3(AC_NT), lds,
+    hipLaunchKernelGGL((prof ? k_imexop_sweep_listed<true> : k_imexop_sweep_listed<false>),
+                       dim3(grid), dim3(AC_NT), lds,
                        (hipStream_t)stream, (const gmember *)dmem, nm, *a);
     return launch_ok() ? 0 : SZ_ERR_LAUNCH;
 }

@@ -73,7 +73,7 @@ extern "C" int sz_internal_imexop_pack_listed(const void *dmem, int nm,
                                               int nsys, int ldmax,
                                               const int *list, const int *count,
                                               double *frob, void *stream);
-extern "C" int sz_internal_imexop_sweep_pair_n(void);
+extern "C" int sz_internal_imexop_sweep_tier_n(int which);
 extern "C" int sz_internal_imexop_sweep_listed(const void *dmem, int nm,
                                                int nsys, int nmax,
                                                const sz_prec_sweep *a,
@@ -139,11 +139,17 @@ struct p_args {
     const double *frob;
     const int *finfo;
     int *plist, *pcount;         // systems to factor this pass, their count
-    // systems the first sweep takes, by LDS tier (n <= nsplit, larger), and
-    // their counts (scount[2 * tier])
-    int *slist0, *slist1, *scount;
-    int nsplit;
+    // systems the first sweep takes, by LDS tier (n <= nsplit[0], n <=
+    // nsplit[1], larger), and their counts (scount[2 * tier])
+    int *slist[3];
+    int *scount;
+    int nsplit[2];
 };
+
+__host__ __device__ inline int sweep_tier(int n, const int (&split)[2])
+{
+    return n <= split[0] ? 0 : (n <= split[1] ? 1 : 2);
+}
 
 // B <- the state's pencils with the wall rows of the energy and momentum
 // equations zeroed (IsothermalPATPTEnforcer::rhs,
@@ -250,7 +256,8 @@ __device__ __forceinline__ void wave_append(bool flag, int value, int *list,
 __global__ void k_p_afterfactor(p_args A, int nsys)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    bool first0 = false, first1 = false;     // on the first sweep's lists
+    bool first = false;                      // on the first sweep's lists
+    int tier = 0;
     if (s < nsys) {
         psys &S = A.S[s];
         const pdesc d = A.D[s];
@@ -272,13 +279,12 @@ __global__ void k_p_afterfactor(p_args A, int nsys)
         }
         const double eps = DBL_EPSILON * 0.5;
         S.tolconst = use_eps ? 1.0 : M.afrob[ls] * eps * sqrt(5.0 * d.n) * A.tolsc;
-        if (A.skip[s] == 0) {
-            first1 = d.n > A.nsplit;
-            first0 = !first1;
-        }
+        first = A.skip[s] == 0;
+        tier = sweep_tier(d.n, A.nsplit);
     }
-    wave_append(first0, s, A.slist0, A.scount);
-    wave_append(first1, s, A.slist1, A.scount + 2);
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+        wave_append(first && tier == t, s, A.slist[t], A.scount + 2 * t);
 }
 
 // counts[0] <- systems marked P_RESTART, counts[1] <- P_FAIL, counts[2] <-
@@ -484,10 +490,10 @@ struct sz_prec {
     int *list = nullptr;
     int *plist = nullptr, *pcount = nullptr;   // systems to pack this pass
     // systems to sweep: [LDS tier][alternating buffer], counts at
-    // scount[2 * tier + buffer]; tier 1 holds the systems with n > nsplit
-    int *slist[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    // scount[2 * tier + buffer]; the tiers split at nsplit (sweep_tier)
+    int *slist[3][2] = {};
     int *scount = nullptr;
-    int nsplit = 0, tier_ns[2] = {0, 0}, tier_nmax[2] = {0, 0};
+    int nsplit[2] = {0, 0}, tier_ns[3] = {}, tier_nmax[3] = {};
     double *skm = nullptr, *skn = nullptr;
     int *o_siter = nullptr, *o_diter = nullptr, *o_info = nullptr,
         *o_first = nullptr, *o_apprx = nullptr;
@@ -518,8 +524,8 @@ struct sz_prec {
         void *ps[] = {D, B, X, PB, S, skip, fskip, finfo, frob, list, skm, skn,
                       o_siter, o_diter, o_info, o_first, o_apprx, o_fact, o_tolsc,
                       o_res, lu64, pb64, ipiv64, info64, counts, plist, pcount,
-                      slist[0][0], slist[0][1], slist[1][0], slist[1][1], scount,
-                      stemp};
+                      slist[0][0], slist[0][1], slist[1][0], slist[1][1],
+                      slist[2][0], slist[2][1], scount, stemp};
         for (void *q : ps) if (q) (void)hipFree(q);
         for (auto &sl : slots) {
             if (sl.gmem) (void)hipFree(sl.gmem);
@@ -563,8 +569,9 @@ void rebuild(sz_prec *h)
     h->v0.clear();
     h->frob0.clear();
     h->nsys = h->nmax = h->ldmax = 0;
-    h->nsplit = sz_internal_imexop_sweep_pair_n();
-    h->tier_ns[0] = h->tier_ns[1] = h->tier_nmax[0] = h->tier_nmax[1] = 0;
+    h->nsplit[0] = sz_internal_imexop_sweep_tier_n(0);
+    h->nsplit[1] = std::max(h->nsplit[0], sz_internal_imexop_sweep_tier_n(1));
+    for (int t = 0; t < 3; ++t) h->tier_ns[t] = h->tier_nmax[t] = 0;
     h->nvec = h->nfrob = 0;
     std::vector<pdesc> D;
     for (int i = 0; i < nl; ++i) {
@@ -587,7 +594,7 @@ void rebuild(sz_prec *h)
         h->nvec += (long long)m.g.nsys * N;
         h->nfrob += (long long)m.g.nsys * nch;
         h->nmax = std::max(h->nmax, m.g.n);
-        const int t = m.g.n > h->nsplit;
+        const int t = sweep_tier(m.g.n, h->nsplit);
         h->tier_ns[t] += m.g.nsys;
         h->tier_nmax[t] = std::max(h->tier_nmax[t], m.g.n);
         // B-spline band leading dimension: KL = 5 (kl + 1) - 1
@@ -642,7 +649,7 @@ void rebuild(sz_prec *h)
     }
     if (!h->counts) h->realloc_pool(h->counts, 3);
     if (!h->pcount) h->realloc_pool(h->pcount, 1);
-    if (!h->scount) h->realloc_pool(h->scount, 4);
+    if (!h->scount) h->realloc_pool(h->scount, 6);
     h->pool_bytes = (long long)h->cap_sys * (sizeof(pdesc) + sizeof(psys) + 64) +
                     h->cap_vec * (2 * sizeof(zd) + sizeof(zf)) +
                     h->cap_frob * (long long)sizeof(double);
@@ -965,7 +972,7 @@ int single_pass(sz_prec *h, sz_prec::slot &sl, int slot, const p_args &A0, int p
             if (rc) return rc;
         }
     }
-    hipLaunchKernelGGL(k_p_fill_int, dim3(1), dim3(PT), 0, st, h->scount, 0, 4);
+    hipLaunchKernelGGL(k_p_fill_int, dim3(1), dim3(PT), 0, st, h->scount, 0, 6);
     hipLaunchKernelGGL(k_p_afterfactor, dim3(nb(h->nsys)), dim3(PT), 0, st, A,
                        h->nsys);
     if (!ok()) return SZ_ERR_LAUNCH;
@@ -990,7 +997,7 @@ int single_pass(sz_prec *h, sz_prec::slot &sl, int slot, const p_args &A0, int p
         // the active systems alternate between two lists per LDS tier; the
         // larger systems first
         const int cur = it & 1, nxt = cur ^ 1;
-        for (int t = 1; t >= 0; --t) {
+        for (int t = 2; t >= 0; --t) {
             if (h->tier_ns[t] == 0) continue;
             sw.list = h->slist[t][cur];
             sw.count = h->scount + 2 * t + cur;
@@ -1133,10 +1140,10 @@ int sz_prec_solve(sz_prec *h, int slot, const sz_prec_problem *pr)
     A.finfo = h->finfo;
     A.plist = h->plist;
     A.pcount = h->pcount;
-    A.slist0 = h->slist[0][0];
-    A.slist1 = h->slist[1][0];
+    for (int t = 0; t < 3; ++t) A.slist[t] = h->slist[t][0];
     A.scount = h->scount;
-    A.nsplit = h->nsplit;
+    A.nsplit[0] = h->nsplit[0];
+    A.nsplit[1] = h->nsplit[1];
     // 1. right hand sides with the wall rows zeroed, x = 0
     hipLaunchKernelGGL(k_p_gather, dim3(h->nsys), dim3(PT), 0, st, A);
     if (!ok()) return SZ_ERR_LAUNCH;

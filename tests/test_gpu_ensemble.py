@@ -60,19 +60,24 @@ def test_members_equal_alone(gpu, graph):
         np.testing.assert_array_equal(got, want, err_msg=str(m))
 
 
-def test_members_on_both_sweep_tiers(gpu):
-    """A member with more coefficients than the two-blocks-per-CU LDS bound
-    of the refinement sweep is swept in a launch of its own, next to the
-    smaller ones: both still equal their solo runs bit for bit, replayed."""
+def test_members_on_every_sweep_tier(gpu):
+    """The refinement sweep runs one launch per LDS tier of system sizes
+    (profiles staged with two blocks per CU, gathered with two, staged with
+    one): members on every tier, side by side, still equal their solo runs
+    bit for bit, replayed."""
     from suzerain_amd._lib import load
     from suzerain_amd.channel import DEFAULT_SCENARIO
     from suzerain_amd.ensemble import Ensemble, default_dt
-    nsplit = load().sz_internal_imexop_sweep_pair_n()
-    big = dict(Nx=4, Ny=nsplit + 6, Nz=4, k=8, htdelta=3.0,
-               scenario=dict(DEFAULT_SCENARIO, Re=2000.0, Ma=0.8))
-    members = [big, _members()[0], dict(big, k=6, Nx=8)]
-    print(f"case sweep tiers: split n={nsplit}, member n {[c['Ny'] for c in members]}",
+    lib = load()
+    t0, t1 = lib.sz_internal_imexop_sweep_tier_n(0), lib.sz_internal_imexop_sweep_tier_n(1)
+    sc = dict(DEFAULT_SCENARIO, Re=2000.0, Ma=0.8)
+    members = [dict(Nx=4, Ny=t1 + 6, Nz=4, k=8, htdelta=3.0, scenario=sc),
+               _members()[0],
+               dict(Nx=8, Ny=t0 + 6, Nz=4, k=6, htdelta=3.0, scenario=sc),
+               dict(Nx=4, Ny=t0, Nz=8, k=8, htdelta=3.0, scenario=sc)]
+    print(f"case sweep tiers: bounds {t0}, {t1}; member n {[c['Ny'] for c in members]}",
           flush=True)
+    assert t0 < t0 + 6 <= t1 < t1 + 6
     W = [_initial(c, gpu) for c in members]
     dts = [default_dt(c) for c in members]
     ens = Ensemble(members, dt=dts, solver=SPEC, graph=True)
