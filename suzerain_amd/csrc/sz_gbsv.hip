@@ -941,14 +941,8 @@ void k_gbtrf_wave(int n, cpx<R> *__restrict__ ab, int ldab, long long sab,
     }
     const int lane = threadIdx.x;
 
-    // component-wise select (a select of whole structs goes through memory)
-    auto sel = [](bool c, T x, T y) __attribute__((always_inline)) -> T {
-        T o;
-        o.x = c ? x.x : y.x;
-        o.y = c ? x.y : y.y;
-        return o;
-    };
-    // c ? x : 0 (literal zeros: a zero struct kept live gets spilled)
+    // c ? x : 0, component-wise (a select of whole structs goes through
+    // memory; literal zeros: a zero struct kept live gets spilled)
     auto sel0 = [](bool c, T x) __attribute__((always_inline)) -> T {
         T o;
         o.x = c ? x.x : R(0);

@@ -909,14 +909,14 @@ sz_imexop *sz_imexop_create(const sz_imexop_scenario *s, sz_bsplineop *w,
     op->scen[3] = s->alpha; op->scen[4] = s->gamma;
     op->w = w;
     build_table(op->scen, op->host);
-    hipGetDevice(&op->dev_id);
+    (void)hipGetDevice(&op->dev_id);
     if (hipDeviceGetAttribute(&op->ncu, hipDeviceAttributeMultiprocessorCount,
                               op->dev_id) != hipSuccess || op->ncu < 1)
         op->ncu = 256;
     if (hipMalloc(&op->dev, sizeof(table)) != hipSuccess ||
         hipMemcpy(op->dev, &op->host, sizeof(table), hipMemcpyHostToDevice)
             != hipSuccess) {
-        hipFree(op->dev);
+        (void)hipFree(op->dev);
         delete op;
         *info = SZ_ERR_ALLOC;
         return nullptr;
@@ -927,7 +927,7 @@ sz_imexop *sz_imexop_create(const sz_imexop_scenario *s, sz_bsplineop *w,
 void sz_imexop_free(sz_imexop *op)
 {
     if (!op) return;
-    hipFree(op->dev);
+    (void)hipFree(op->dev);
     delete op;
 }
 
