@@ -528,9 +528,9 @@ int sz_ensemble_autotune(sz_ensemble *e, int steps)
     double best = -1;
     int best_ns = e->nstreams, best_bs = e->pp.lu_bs ? e->pp.lu_bs : 8;
     e->trial_ms.clear();
-    const int cand_ns[3] = {1, 2, 4};
+    const int cand_ns[4] = {1, 2, 3, 4};
     const int cand_bs[2] = {4, 8};
-    for (int a = 0; a < 3 && !rc; ++a) {
+    for (int a = 0; a < 4 && !rc; ++a) {
         if (cand_ns[a] > e->maxstreams) continue;
         for (int b = 0; b < 2 && !rc; ++b) {
             restore();
