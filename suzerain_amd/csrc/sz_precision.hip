@@ -12,9 +12,11 @@
 // the permuted order of the factors (index i <-> field i mod 5, row i / 5)
 // at the same offset, the pivots too.  Every system advances through a small
 // state machine in device memory; the host enqueues a fixed sequence
-// (gather, init, masked pack + factor, smax + 1 sweeps of {masked float
-// solve, update, masked residual}), then (outside graph capture) reads the
-// counters once to decide on a restart pass and the double fallback.
+// (gather, init, pack + factor + split layout of the systems on the device's
+// factoring list, smax + 1 sweeps of {masked float solve, update and
+// residual over the device's list of active systems}), then (outside graph
+// capture) reads the counters once to decide on a restart pass and the
+// double fallback.
 #include "sz_precision.hpp"
 
 #include <algorithm>
