@@ -759,7 +759,7 @@ __device__ __forceinline__ int sweep_body(const gmember &M, int ls, int s,
     for (int e = tid; e < N; e += AC_NT) {
         const int f = e / n, y = e - f * n;
         const sz::zf c = sz::gload(pb + y * 5 + f);   // (P^T c)_e = c_{q^-1(e)}
-        const zd xo = sz::gload(x + e);
+        const zd xo = a.first ? zd{0, 0} : sz::gload(x + e);
         const zd v = zd{xo.x + (double)c.x, xo.y + (double)c.y};
         xs[e] = v;
         sz::gstore(x + e, v);

@@ -155,8 +155,10 @@ __host__ __device__ inline int sweep_tier(int n, const int (&split)[2])
 
 // B <- the state's pencils with the wall rows of the energy and momentum
 // equations zeroed (IsothermalPATPTEnforcer::rhs,
-// apps/perfect/operator_hybrid_isothermal.cpp:513-524); X <- 0.  One block
-// per system.
+// apps/perfect/operator_hybrid_isothermal.cpp:513-524).  One block per
+// system.  X starts at zero: the first sweep of a pass takes it as zero
+// without reading it, and the systems no sweep visits have it zeroed by
+// k_p_init or k_p_afterfactor.
 __global__ __launch_bounds__(PT) void k_p_gather(p_args A)
 {
     const int s = blockIdx.x;
@@ -170,7 +172,6 @@ __global__ __launch_bounds__(PT) void k_p_gather(p_args A)
         zd v = st[f * M.F + y];
         if (f < 4 && ((y == 0 && lower) || (y == n - 1 && upper))) v = zd{0, 0};
         A.B[d.v + a] = v;
-        A.X[d.v + a] = zd{0, 0};
     }
 }
 
@@ -207,6 +208,8 @@ __global__ __launch_bounds__(PT) void k_p_init(p_args A)
     const bool above = res > tol0;
     const bool active = above && A.smax >= 0;
     const bool need = active && f != 'S';
+    if (!active && A.pass == 0)                       // no sweep visits it
+        for (int i = tid; i < N; i += PT) x[i] = zd{0, 0};
     if (active) {
         zf *pb = A.PB + d.v;
         for (int i = tid; i < N; i += PT) {
@@ -275,6 +278,8 @@ __global__ void k_p_afterfactor(p_args A, int nsys)
                 M.fact[ls] = 'N';
                 S.st = P_FAIL;
                 A.skip[s] = 1;
+                for (int i = 0; i < 5 * d.n; ++i)   // no sweep visits it
+                    A.X[d.v + i] = zd{0, 0};
             } else {
                 M.fact[ls] = 'S';
             }
@@ -987,6 +992,7 @@ int single_pass(sz_prec *h, sz_prec::slot &sl, int slot, const p_args &A0, int p
     sw.tolsc = h->p.tolsc;
     sw.aiter = h->p.aiter;
     sw.smax = h->p.smax;
+    sw.first = 0;
     for (int it = 0; it <= h->p.smax; ++it) {
         // c = (P A^T P^T)^-T P r in float, then x += P^T c, r = b - A x,
         // decision and the next float right hand side in one launch
@@ -999,6 +1005,7 @@ int single_pass(sz_prec *h, sz_prec::slot &sl, int slot, const p_args &A0, int p
         // the active systems alternate between two lists per LDS tier; the
         // larger systems first
         const int cur = it & 1, nxt = cur ^ 1;
+        sw.first = it == 0;                 // x = 0 before the first sweep
         for (int t = 2; t >= 0; --t) {
             if (h->tier_ns[t] == 0) continue;
             sw.list = h->slist[t][cur];

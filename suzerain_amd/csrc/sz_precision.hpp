@@ -77,6 +77,7 @@ struct sz_prec_sweep {
     int *skip;
     double tolsc;
     int aiter, smax;
+    int first;                   // the pass's first sweep: x is zero, unread
     const int *list, *count;     // the systems to sweep, their number
     int *next, *ncount;          // the systems that continue, appended
 };
