@@ -153,30 +153,13 @@ __host__ __device__ inline int sweep_tier(int n, const int (&split)[2])
     return n <= split[0] ? 0 : (n <= split[1] ? 1 : 2);
 }
 
-// B <- the state's pencils with the wall rows of the energy and momentum
-// equations zeroed (IsothermalPATPTEnforcer::rhs,
-// apps/perfect/operator_hybrid_isothermal.cpp:513-524).  One block per
-// system.  X starts at zero: the first sweep of a pass takes it as zero
-// without reading it, and the systems no sweep visits have it zeroed by
-// k_p_init or k_p_afterfactor.
-__global__ __launch_bounds__(PT) void k_p_gather(p_args A)
-{
-    const int s = blockIdx.x;
-    const pdesc d = A.D[s];
-    const pmem &M = A.PM[d.m];
-    const int n = d.n, N = 5 * n;
-    const zd *st = M.state + M.sysoff[s - M.sys0];
-    const int lower = M.wc.enforce[0], upper = M.wc.enforce[1];
-    for (int a = threadIdx.x; a < N; a += PT) {
-        const int f = a / n, y = a - f * n;
-        zd v = st[f * M.F + y];
-        if (f < 4 && ((y == 0 && lower) || (y == n - 1 && upper))) v = zd{0, 0};
-        A.B[d.v + a] = v;
-    }
-}
-
-// Start (pass 0: every system) or restart (pass 1: systems marked
-// P_RESTART, refactored from scratch) the single precision attempt.
+// Start (pass 0: every system; B <- the state's pencils with the wall rows
+// of the energy and momentum equations zeroed, IsothermalPATPTEnforcer::rhs,
+// apps/perfect/operator_hybrid_isothermal.cpp:513-524) or restart (pass 1:
+// systems marked P_RESTART, refactored from scratch) the single precision
+// attempt.  X starts at zero: the first sweep of a pass takes it as zero
+// without reading it, and the systems no sweep visits have it zeroed here
+// or by k_p_afterfactor.
 __global__ __launch_bounds__(PT) void k_p_init(p_args A)
 {
     __shared__ double red[PT / 64];
@@ -195,9 +178,21 @@ __global__ __launch_bounds__(PT) void k_p_init(p_args A)
     // read before the reduction's barriers: thread 0 rewrites it at the end
     const char f = A.pass == 1 ? 'N' : M.fact[ls];
     double ss = 0;
-    for (int i = tid; i < N; i += PT) {
-        ss += b[i].x * b[i].x + b[i].y * b[i].y;
-        if (A.pass == 1) x[i] = zd{0, 0};
+    if (A.pass == 0) {
+        const zd *st = M.state + M.sysoff[ls];
+        const int lower = M.wc.enforce[0], upper = M.wc.enforce[1];
+        for (int i = tid; i < N; i += PT) {
+            const int fi = i / n, y = i - fi * n;
+            zd v = st[fi * M.F + y];
+            if (fi < 4 && ((y == 0 && lower) || (y == n - 1 && upper))) v = zd{0, 0};
+            A.B[d.v + i] = v;
+            ss += v.x * v.x + v.y * v.y;
+        }
+    } else {
+        for (int i = tid; i < N; i += PT) {
+            ss += b[i].x * b[i].x + b[i].y * b[i].y;
+            x[i] = zd{0, 0};
+        }
     }
     const double res = sqrt(block_sum(ss, red));
     const double eps = DBL_EPSILON * 0.5;
@@ -1153,11 +1148,9 @@ int sz_prec_solve(sz_prec *h, int slot, const sz_prec_problem *pr)
     A.scount = h->scount;
     A.nsplit[0] = h->nsplit[0];
     A.nsplit[1] = h->nsplit[1];
-    // 1. right hand sides with the wall rows zeroed, x = 0
-    hipLaunchKernelGGL(k_p_gather, dim3(h->nsys), dim3(PT), 0, st, A);
-    if (!ok()) return SZ_ERR_LAUNCH;
-    // 2. single precision attempt with the slot's factors (refactoring the
-    //    systems without any)
+    // 1-2. right hand sides with the wall rows zeroed (gathered by the first
+    //    pass's init), single precision attempt with the slot's factors
+    //    (refactoring the systems without any)
     h->n_factored = h->n_restarted = h->n_fallback = 0;
     rc = single_pass(h, sl, slot, A, 0);
     if (rc) return rc;
