@@ -1301,7 +1301,15 @@ int sz_internal_imexop_sweep_listed(const void *dmem, int nm, int nsys,
     const bool prof = sweep_prof(nmax) && sweep_per_cu(nmax, true) >= 1;
     const size_t lds = sweep_lds(nmax, prof);
     if (lds > 156 * 1024) return SZ_ERR_LDS;
-    const int grid = nsys < 1024 ? nsys : 1024;
+    // as many blocks as stay resident, each with one contiguous share of the
+    // list (fewer member changes, so fewer restagings of the profiles)
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) !=
+            hipSuccess || ncu < 1)
+        ncu = 256;
+    const int grid = (int)std::min<long long>(
+        nsys, (long long)ncu * std::max(1, sweep_per_cu(nmax, prof)));
     hipLaunchKernelGGL((prof ? k_imexop_sweep_listed<true> : k_imexop_sweep_listed<false>),
                        dim3(grid), dim3(AC_NT), lds,
                        (hipStream_t)stream, (const gmember *)dmem, nm, *a);
